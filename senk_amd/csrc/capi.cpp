@@ -1,0 +1,1121 @@
+// C ABI glue: contexts, device memory, kernel entry points with the reference's
+// argument order, device ILU factorization, the solver twins and the handle API.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "solvers.hpp"
+
+struct senk_ctx_s {
+    senk_amd::Ctx c;
+    std::map<std::pair<const void *, int>, int> nnz_cache;  // rptr -> nnz (SpMV lane choice)
+};
+struct senk_trsv_plan_s {
+    senk_amd::Ctx *c = nullptr;
+    senk_amd::Schedule s;
+};
+struct senk_mat_s {
+    senk_ctx ctx = nullptr;
+    int N = 0, nnz = 0;
+    int *rptr = nullptr, *cind = nullptr;
+    double *v64 = nullptr;
+    float *v32 = nullptr;
+    bool has_sell = false;
+    senk_sellcs sell{};
+    senk_amd::MatView view;
+};
+struct senk_prec_s {
+    senk_ctx ctx = nullptr;
+    senk_amd::PrecView view;
+    int lnnz = 0, unnz = 0;
+    std::vector<void *> owned;
+    std::vector<int> hlrptr, hurptr;
+};
+
+namespace senk_amd {
+namespace {
+thread_local std::string g_last_error;
+}
+void record_hip_error(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    g_last_error = buf;
+}
+void record_error(const std::string &msg) { g_last_error = msg; }
+
+int ctx_work(Ctx *c, size_t bytes, void **out) {
+    if (c->work.bytes < bytes) {
+        if (c->work.p) SENK_HIP(hipFree(c->work.p));
+        c->work.p = nullptr;
+        c->work.bytes = 0;
+        SENK_HIP(hipMalloc(&c->work.p, bytes));
+        c->work.bytes = bytes;
+    }
+    *out = c->work.p;
+    return SENK_OK;
+}
+
+namespace {
+int nnz_of(senk_ctx ctx, const int *drptr, int N, int *nnz) {
+    auto key = std::make_pair((const void *)drptr, N);
+    auto it = ctx->nnz_cache.find(key);
+    if (it != ctx->nnz_cache.end()) {
+        *nnz = it->second;
+        return SENK_OK;
+    }
+    int h[2] = {0, 0};
+    SENK_HIP(hipMemcpyAsync(&h[0], drptr, sizeof(int), hipMemcpyDeviceToHost, ctx->c.stream));
+    SENK_HIP(hipMemcpyAsync(&h[1], drptr + N, sizeof(int), hipMemcpyDeviceToHost, ctx->c.stream));
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    *nnz = h[1] - h[0];
+    if (ctx->nnz_cache.size() > 256) ctx->nnz_cache.clear();
+    ctx->nnz_cache[key] = *nnz;
+    return SENK_OK;
+}
+double avg_row(int nnz, int N) { return N > 0 ? (double)nnz / N : 0.0; }
+
+Schedule colors(const int *cptr, int cnum, int bsize) {
+    Schedule s;
+    s.bsize = bsize;
+    s.nlev = cnum;
+    s.lev_ptr.assign(cptr, cptr + cnum + 1);
+    return s;
+}
+template <typename T>
+int dalloc(T **p, size_t n) {
+    SENK_HIP(hipMalloc((void **)p, sizeof(T) * (n ? n : 1)));
+    return SENK_OK;
+}
+template <typename T>
+int upload(senk_ctx ctx, T **p, const T *h, size_t n) {
+    SENK_TRY(dalloc(p, n));
+    if (n) SENK_HIP(hipMemcpyAsync(*p, h, sizeof(T) * n, hipMemcpyHostToDevice, ctx->c.stream));
+    return SENK_OK;
+}
+
+// Levels of bnl-row groups of a BCSR triangular factor (for the sequential BCSR solve).
+void bcsr_levels(const std::vector<int> &brptr, const std::vector<int> &bcind, int ngroups, int bnl, int bnw,
+                 bool upper, std::vector<int> &lev_ptr, std::vector<int> &blk) {
+    std::vector<int> depth((size_t)ngroups, 0);
+    int maxd = 0;
+    auto visit = [&](int g) {
+        int d = 0;
+        for (int b = brptr[g]; b < brptr[g + 1]; b++)
+            for (int l = 0; l < bnw; l++) {
+                const int og = (bcind[b] * bnw + l) / bnl;
+                if ((!upper && og < g) || (upper && og > g)) d = std::max(d, depth[og] + 1);
+            }
+        depth[g] = d;
+        maxd = std::max(maxd, d);
+    };
+    if (!upper) for (int g = 0; g < ngroups; g++) visit(g);
+    else for (int g = ngroups - 1; g >= 0; g--) visit(g);
+    if (upper) for (int g = 0; g < ngroups; g++) depth[g] = maxd - depth[g];
+    const int nlev = ngroups ? maxd + 1 : 0;
+    lev_ptr.assign((size_t)nlev + 1, 0);
+    for (int g = 0; g < ngroups; g++) lev_ptr[depth[g] + 1]++;
+    for (int l = 0; l < nlev; l++) lev_ptr[l + 1] += lev_ptr[l];
+    std::vector<int> pos(lev_ptr.begin(), lev_ptr.end() - 1);
+    blk.assign((size_t)ngroups, 0);
+    for (int g = 0; g < ngroups; g++) blk[pos[depth[g]]++] = g;
+}
+}  // namespace
+}  // namespace senk_amd
+
+using namespace senk_amd;
+
+#define CTX_CHECK(ctx) \
+    do {               \
+        if (!(ctx)) return SENK_ERR_ARG; \
+    } while (0)
+
+extern "C" {
+
+const char *senk_status_string(int s) {
+    switch (s) {
+        case SENK_OK: return "ok";
+        case SENK_ERR_ARG: return "invalid argument";
+        case SENK_ERR_HIP: return "HIP runtime error";
+        case SENK_ERR_ZERO_PIVOT: return "zero pivot in Ilu0";
+        case SENK_ERR_DIVISIBILITY: return "size not divisible by the block size";
+        case SENK_ERR_BREAKDOWN: return "Krylov breakdown (non-finite scalar)";
+        case SENK_ERR_NOT_CONVERGED: return "not converged";
+        case SENK_ERR_NO_DEVICE: return "no HIP device";
+        case SENK_ERR_ALLOC: return "allocation failure";
+        case SENK_ERR_UNSUPPORTED: return "unsupported shape";
+        case SENK_ERR_SCHEDULE: return "factor entry couples two blocks of one launch";
+        default: return "unknown status";
+    }
+}
+const char *senk_last_error(void) { return g_last_error.c_str(); }
+int senk_version(void) { return 100; }
+
+int senk_device_count(int *count) {
+    if (!count) return SENK_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return SENK_OK;
+}
+
+int senk_ctx_create(int device, senk_ctx *out) {
+    if (!out) return SENK_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SENK_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return SENK_ERR_ARG;
+    SENK_HIP(hipSetDevice(device));
+    auto *ctx = new senk_ctx_s();
+    Ctx &c = ctx->c;
+    c.device = device;
+    SENK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    SENK_HIP(hipMalloc(&c.red_partials, sizeof(double) * kRedMaxK * kRedMaxBlocks));
+    SENK_HIP(hipMalloc(&c.red_counters, sizeof(unsigned) * kRedSlots));
+    SENK_HIP(hipMemset(c.red_counters, 0, sizeof(unsigned) * kRedSlots));
+    SENK_HIP(hipMalloc(&c.dscalars, sizeof(double) * 64));
+    SENK_HIP(hipHostMalloc((void **)&c.hpinned, sizeof(double) * 64, hipHostMallocDefault));
+    SENK_HIP(hipHostMalloc((void **)&c.hpinned_i, sizeof(int) * 64, hipHostMallocDefault));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c.num_cus = prop.multiProcessorCount;
+    SENK_HIP(hipDeviceSynchronize());
+    *out = ctx;
+    return SENK_OK;
+}
+int senk_ctx_destroy(senk_ctx ctx) {
+    CTX_CHECK(ctx);
+    Ctx &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    (void)hipStreamSynchronize(c.stream);
+    if (c.work.p) (void)hipFree(c.work.p);
+    (void)hipFree(c.red_partials);
+    (void)hipFree(c.red_counters);
+    (void)hipFree(c.dscalars);
+    (void)hipHostFree(c.hpinned);
+    (void)hipHostFree(c.hpinned_i);
+    (void)hipStreamDestroy(c.stream);
+    delete ctx;
+    return SENK_OK;
+}
+int senk_ctx_synchronize(senk_ctx ctx) {
+    CTX_CHECK(ctx);
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SENK_OK;
+}
+int senk_device_malloc(senk_ctx ctx, size_t bytes, void **dptr) {
+    CTX_CHECK(ctx);
+    if (!dptr) return SENK_ERR_ARG;
+    SENK_HIP(hipMalloc(dptr, bytes ? bytes : 1));
+    return SENK_OK;
+}
+int senk_device_free(senk_ctx ctx, void *dptr) {
+    CTX_CHECK(ctx);
+    if (dptr) {
+        SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+        SENK_HIP(hipFree(dptr));
+    }
+    return SENK_OK;
+}
+int senk_memcpy_htod(senk_ctx ctx, void *dst, const void *src, size_t bytes) {
+    CTX_CHECK(ctx);
+    if (!bytes) return SENK_OK;
+    SENK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SENK_OK;
+}
+int senk_memcpy_dtoh(senk_ctx ctx, void *dst, const void *src, size_t bytes) {
+    CTX_CHECK(ctx);
+    if (!bytes) return SENK_OK;
+    SENK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SENK_OK;
+}
+int senk_memcpy_dtod(senk_ctx ctx, void *dst, const void *src, size_t bytes) {
+    CTX_CHECK(ctx);
+    if (!bytes) return SENK_OK;
+    SENK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
+    return SENK_OK;
+}
+int senk_memset_zero(senk_ctx ctx, void *dst, size_t bytes) {
+    CTX_CHECK(ctx);
+    if (!bytes) return SENK_OK;
+    SENK_HIP(hipMemsetAsync(dst, 0, bytes, ctx->c.stream));
+    return SENK_OK;
+}
+int senk_event_create(senk_ctx ctx, void **ev) {
+    CTX_CHECK(ctx);
+    hipEvent_t e;
+    SENK_HIP(hipEventCreate(&e));
+    *ev = (void *)e;
+    return SENK_OK;
+}
+int senk_event_destroy(void *ev) {
+    if (ev) SENK_HIP(hipEventDestroy((hipEvent_t)ev));
+    return SENK_OK;
+}
+int senk_event_record(senk_ctx ctx, void *ev) {
+    CTX_CHECK(ctx);
+    SENK_HIP(hipEventRecord((hipEvent_t)ev, ctx->c.stream));
+    return SENK_OK;
+}
+int senk_event_elapsed_ms(void *start, void *stop, float *ms) {
+    SENK_HIP(hipEventSynchronize((hipEvent_t)stop));
+    SENK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return SENK_OK;
+}
+
+// ---------------------------------------------------------------- SpMV
+int senk_spmv_csr_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dx,
+                      double *dy, int N) {
+    CTX_CHECK(ctx);
+    if (N < 0 || (N > 0 && (!dval || !dcind || !drptr || !dx || !dy))) return SENK_ERR_ARG;
+    int nnz = 0;
+    if (N > 0) SENK_TRY(nnz_of(ctx, drptr, N, &nnz));
+    return launch_spmv_csr<double>(&ctx->c, N, drptr, dcind, dval, dx, dy, avg_row(nnz, N));
+}
+int senk_spmv_csr_f32(senk_ctx ctx, const float *dval, const int *dcind, const int *drptr, const float *dx,
+                      float *dy, int N) {
+    CTX_CHECK(ctx);
+    if (N < 0 || (N > 0 && (!dval || !dcind || !drptr || !dx || !dy))) return SENK_ERR_ARG;
+    int nnz = 0;
+    if (N > 0) SENK_TRY(nnz_of(ctx, drptr, N, &nnz));
+    return launch_spmv_csr<float>(&ctx->c, N, drptr, dcind, dval, dx, dy, avg_row(nnz, N));
+}
+int senk_spmv_csr_diag_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                           const double *ddiag, const double *dx, double *dy, int N) {
+    CTX_CHECK(ctx);
+    if (N < 0) return SENK_ERR_ARG;
+    return launch_spmv_csr_diag_f64(&ctx->c, N, drptr, dcind, dval, ddiag, dx, dy);
+}
+int senk_spmv_bcsr_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
+                       double *dy, int N, int bnl, int bnw) {
+    CTX_CHECK(ctx);
+    if (bnl <= 0 || bnw <= 0 || N < 0) return SENK_ERR_ARG;
+    if (N % bnl || N % bnw) return SENK_ERR_DIVISIBILITY;
+    return launch_spmv_bcsr<double>(&ctx->c, N, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
+}
+int senk_spmv_bcsr_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr, const float *dx,
+                       float *dy, int N, int bnl, int bnw) {
+    CTX_CHECK(ctx);
+    if (bnl <= 0 || bnw <= 0 || N < 0) return SENK_ERR_ARG;
+    if (N % bnl || N % bnw) return SENK_ERR_DIVISIBILITY;
+    return launch_spmv_bcsr<float>(&ctx->c, N, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
+}
+int senk_spmv_sell_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *dwid, int len,
+                       const double *dx, double *dy, int N) {
+    CTX_CHECK(ctx);
+    if (len <= 0 || N < 0) return SENK_ERR_ARG;
+    const int ns = (N + len - 1) / len;
+    return launch_spmv_sell<double>(&ctx->c, N, len, ns, dwid, dcind, dval, nullptr, dx, dy, N % len != 0);
+}
+int senk_spmv_sell_f32(senk_ctx ctx, const float *dval, const int *dcind, const int *dwid, int len,
+                       const float *dx, float *dy, int N) {
+    CTX_CHECK(ctx);
+    if (len <= 0 || N < 0) return SENK_ERR_ARG;
+    const int ns = (N + len - 1) / len;
+    return launch_spmv_sell<float>(&ctx->c, N, len, ns, dwid, dcind, dval, nullptr, dx, dy, N % len != 0);
+}
+int senk_sellcs_upload(senk_ctx ctx, const senk_sellcs *h, int want_f32, senk_sellcs *d) {
+    CTX_CHECK(ctx);
+    if (!h || !d) return SENK_ERR_ARG;
+    std::memset(d, 0, sizeof(*d));
+    d->N = h->N;
+    d->C = h->C;
+    d->sigma = h->sigma;
+    d->nslices = h->nslices;
+    d->nnz_padded = h->nnz_padded;
+    SENK_TRY(upload(ctx, &d->slice_ptr, h->slice_ptr, (size_t)h->nslices + 1));
+    SENK_TRY(upload(ctx, &d->cind, h->cind, (size_t)h->nnz_padded));
+    SENK_TRY(upload(ctx, &d->perm, h->perm, (size_t)h->nslices * h->C));
+    if (h->val64) SENK_TRY(upload(ctx, &d->val64, h->val64, (size_t)h->nnz_padded));
+    if (want_f32 && d->val64) {
+        SENK_TRY(dalloc(&d->val32, (size_t)h->nnz_padded));
+        SENK_TRY(launch_convert<double, float>(&ctx->c, h->nnz_padded, d->val64, d->val32, nullptr));
+    }
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SENK_OK;
+}
+int senk_sellcs_free_device(senk_ctx ctx, senk_sellcs *d) {
+    CTX_CHECK(ctx);
+    if (!d) return SENK_ERR_ARG;
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    (void)hipFree(d->slice_ptr);
+    (void)hipFree(d->cind);
+    (void)hipFree(d->perm);
+    (void)hipFree(d->val64);
+    (void)hipFree(d->val32);
+    std::memset(d, 0, sizeof(*d));
+    return SENK_OK;
+}
+int senk_spmv_sellcs_f64(senk_ctx ctx, const senk_sellcs *d, const double *dx, double *dy) {
+    CTX_CHECK(ctx);
+    if (!d || !d->val64) return SENK_ERR_ARG;
+    return launch_spmv_sell<double>(&ctx->c, d->N, d->C, d->nslices, d->slice_ptr, d->cind, d->val64, d->perm, dx,
+                                    dy, false);
+}
+int senk_spmv_sellcs_f32(senk_ctx ctx, const senk_sellcs *d, const float *dx, float *dy) {
+    CTX_CHECK(ctx);
+    if (!d || !d->val32) return SENK_ERR_ARG;
+    return launch_spmv_sell<float>(&ctx->c, d->N, d->C, d->nslices, d->slice_ptr, d->cind, d->val32, d->perm, dx,
+                                   dy, false);
+}
+
+// ---------------------------------------------------------------- SpTRSV
+#define TRSV_ABMC(NAME, T, UP)                                                                                   \
+    int NAME(senk_ctx ctx, const T *dval, const int *dcind, const int *drptr, const T *dx, T *dy, int N,         \
+             const int *cptr, int cnum, int bsize) {                                                             \
+        CTX_CHECK(ctx);                                                                                          \
+        if (!cptr || cnum < 0 || bsize <= 0 || N < 0) return SENK_ERR_ARG;                                       \
+        if (N % bsize) return SENK_ERR_DIVISIBILITY;                                                             \
+        if (cptr[cnum] * bsize > N) return SENK_ERR_ARG;                                                         \
+        Schedule s = colors(cptr, cnum, bsize);                                                                  \
+        return launch_sptrsv<T>(&ctx->c, s, UP, drptr, dcind, dval, dx, dy, nullptr, nullptr);                   \
+    }
+TRSV_ABMC(senk_sptrsv_csr_l_abmc_f64, double, false)
+TRSV_ABMC(senk_sptrsv_csr_u_abmc_f64, double, true)
+TRSV_ABMC(senk_sptrsv_csr_l_abmc_f32, float, false)
+TRSV_ABMC(senk_sptrsv_csr_u_abmc_f32, float, true)
+
+int senk_sptrsv_csr_l_amc_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dx,
+                              double *dy, int N, const int *cptr, int cnum) {
+    return senk_sptrsv_csr_l_abmc_f64(ctx, dval, dcind, drptr, dx, dy, N, cptr, cnum, 1);
+}
+int senk_sptrsv_csr_u_amc_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dx,
+                              double *dy, int N, const int *cptr, int cnum) {
+    return senk_sptrsv_csr_u_abmc_f64(ctx, dval, dcind, drptr, dx, dy, N, cptr, cnum, 1);
+}
+
+int senk_trsv_plan_create(senk_ctx ctx, const int *rptr, const int *cind, int N, int upper, senk_trsv_plan *out) {
+    CTX_CHECK(ctx);
+    if (!rptr || !cind || !out || N < 0) return SENK_ERR_ARG;
+    auto *p = new senk_trsv_plan_s();
+    p->c = &ctx->c;
+    std::vector<int> blk;
+    p->s.bsize = 1;
+    p->s.nlev = host::levels(rptr, cind, N, 1, upper != 0, p->s.lev_ptr, blk);
+    const int st = upload(ctx, &p->s.dblk, blk.data(), blk.size());
+    if (st != SENK_OK) {
+        delete p;
+        return st;
+    }
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    *out = p;
+    return SENK_OK;
+}
+int senk_trsv_plan_levels(senk_trsv_plan plan) { return plan ? plan->s.nlev : -1; }
+int senk_trsv_plan_destroy(senk_trsv_plan plan) {
+    if (!plan) return SENK_ERR_ARG;
+    (void)hipFree(plan->s.dblk);
+    delete plan;
+    return SENK_OK;
+}
+int senk_sptrsv_csr_l_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind, const int *drptr,
+                          const double *dx, double *dy, int N) {
+    CTX_CHECK(ctx);
+    if (!plan || N < 0) return SENK_ERR_ARG;
+    return launch_sptrsv<double>(&ctx->c, plan->s, false, drptr, dcind, dval, dx, dy, nullptr, nullptr);
+}
+int senk_sptrsv_csr_u_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind, const int *drptr,
+                          const double *dx, double *dy, int N) {
+    CTX_CHECK(ctx);
+    if (!plan || N < 0) return SENK_ERR_ARG;
+    return launch_sptrsv<double>(&ctx->c, plan->s, true, drptr, dcind, dval, dx, dy, nullptr, nullptr);
+}
+int senk_sptrsv_csr_l_bj_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind,
+                             const int *drptr, const double *dx, double *dy, int N, int bnum) {
+    if (bnum <= 0) return SENK_ERR_ARG;
+    return senk_sptrsv_csr_l_f64(ctx, plan, dval, dcind, drptr, dx, dy, N);
+}
+int senk_sptrsv_csr_u_bj_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind,
+                             const int *drptr, const double *dx, double *dy, int N, int bnum) {
+    if (bnum <= 0) return SENK_ERR_ARG;
+    return senk_sptrsv_csr_u_f64(ctx, plan, dval, dcind, drptr, dx, dy, N);
+}
+int senk_sptrsv_bcsr_l_abmc_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
+                                const double *dx, double *dy, int N, const int *cptr, int cnum, int bsize, int bnl,
+                                int bnw) {
+    CTX_CHECK(ctx);
+    if (!cptr || cnum < 0 || bsize <= 0 || bnl <= 0 || bnw <= 0) return SENK_ERR_ARG;
+    if (N % bsize || bsize % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
+    Schedule s = colors(cptr, cnum, bsize);
+    return launch_sptrsv_bcsr<double>(&ctx->c, s, false, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
+}
+int senk_sptrsv_bcsr_u_abmc_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
+                                const double *dx, double *dy, int N, const int *cptr, int cnum, int bsize, int bnl,
+                                int bnw) {
+    CTX_CHECK(ctx);
+    if (!cptr || cnum < 0 || bsize <= 0 || bnl <= 0 || bnw <= 0) return SENK_ERR_ARG;
+    if (N % bsize || bsize % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
+    Schedule s = colors(cptr, cnum, bsize);
+    return launch_sptrsv_bcsr<double>(&ctx->c, s, true, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
+}
+// Sequential BCSR forms: the group-level dependency levels are computed from a host
+// copy of the block pattern on every call (the reference runs these serially too).
+static int bcsr_seq(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
+                    double *dy, int N, int bnl, int bnw, bool upper) {
+    CTX_CHECK(ctx);
+    if (bnl <= 0 || bnw <= 0 || N < 0) return SENK_ERR_ARG;
+    if (N % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
+    const int ng = N / bnl;
+    std::vector<int> brptr((size_t)ng + 1);
+    SENK_HIP(hipMemcpyAsync(brptr.data(), dbrptr, sizeof(int) * (ng + 1), hipMemcpyDeviceToHost, ctx->c.stream));
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    std::vector<int> bcind((size_t)std::max(brptr[ng], 1));
+    if (brptr[ng])
+        SENK_HIP(hipMemcpyAsync(bcind.data(), dbcind, sizeof(int) * brptr[ng], hipMemcpyDeviceToHost, ctx->c.stream));
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    Schedule s;
+    s.bsize = bnl;
+    std::vector<int> blk;
+    bcsr_levels(brptr, bcind, ng, bnl, bnw, upper, s.lev_ptr, blk);
+    s.nlev = (int)s.lev_ptr.size() - 1;
+    SENK_TRY(upload(ctx, &s.dblk, blk.data(), blk.size()));
+    const int st = launch_sptrsv_bcsr<double>(&ctx->c, s, upper, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    (void)hipFree(s.dblk);
+    return st;
+}
+int senk_sptrsv_bcsr_l_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
+                           double *dy, int N, int bnl, int bnw) {
+    return bcsr_seq(ctx, dbval, dbcind, dbrptr, dx, dy, N, bnl, bnw, false);
+}
+int senk_sptrsv_bcsr_u_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
+                           double *dy, int N, int bnl, int bnw) {
+    return bcsr_seq(ctx, dbval, dbcind, dbrptr, dx, dy, N, bnl, bnw, true);
+}
+
+// ---------------------------------------------------------------- BLAS-1
+int senk_copy_f64(senk_ctx ctx, const double *dx, double *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_copy<double>(&ctx->c, N, dx, dy);
+}
+int senk_scal_f64(senk_ctx ctx, double a, double *dx, int N) {
+    CTX_CHECK(ctx);
+    return launch_scal<double>(&ctx->c, N, a, dx);
+}
+int senk_axpy_f64(senk_ctx ctx, double a, const double *dx, double *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_axpby<double>(&ctx->c, N, a, dx, 1.0, dy);
+}
+int senk_axpby_f64(senk_ctx ctx, double a, const double *dx, double b, double *dy, int N) {
+    CTX_CHECK(ctx);
+    // b == 1 runs y += a*x, which is a*x + 1*y bit for bit
+    return launch_axpby<double>(&ctx->c, N, a, dx, b, dy);
+}
+int senk_axpyz_f64(senk_ctx ctx, double a, const double *dx, const double *dy, double *dz, int N) {
+    CTX_CHECK(ctx);
+    return launch_axpyz<double>(&ctx->c, N, a, dx, dy, dz);
+}
+int senk_dot_f64_dev(senk_ctx ctx, const double *dx, const double *dy, int N, double *dres) {
+    CTX_CHECK(ctx);
+    if (N <= 0) {
+        SENK_HIP(hipMemsetAsync(dres, 0, sizeof(double), ctx->c.stream));
+        return SENK_OK;
+    }
+    return launch_multidot<double>(&ctx->c, N, 1, dx, (size_t)N, dy, dres, 11);
+}
+int senk_dot_f64(senk_ctx ctx, const double *dx, const double *dy, int N, double *result) {
+    CTX_CHECK(ctx);
+    if (!result) return SENK_ERR_ARG;
+    SENK_TRY(senk_dot_f64_dev(ctx, dx, dy, N, ctx->c.dscalars));
+    SENK_HIP(hipMemcpyAsync(ctx->c.hpinned, ctx->c.dscalars, sizeof(double), hipMemcpyDeviceToHost, ctx->c.stream));
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    *result = ctx->c.hpinned[0];
+    return SENK_OK;
+}
+int senk_nrm2_f64_dev(senk_ctx ctx, const double *dx, int N, double *dres) { return senk_dot_f64_dev(ctx, dx, dx, N, dres); }
+int senk_nrm2_f64(senk_ctx ctx, const double *dx, int N, double *result) {
+    SENK_TRY(senk_dot_f64(ctx, dx, dx, N, result));
+    *result = std::sqrt(*result);
+    return SENK_OK;
+}
+int senk_hadprod_f64(senk_ctx ctx, const double *dx, double *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_hadprod_f64(&ctx->c, N, dx, dy);
+}
+int senk_haddiv_f64(senk_ctx ctx, const double *dx, double *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_haddiv_f64(&ctx->c, N, dx, dy);
+}
+int senk_copy_f32(senk_ctx ctx, const float *dx, float *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_copy<float>(&ctx->c, N, dx, dy);
+}
+int senk_scal_f32(senk_ctx ctx, float a, float *dx, int N) {
+    CTX_CHECK(ctx);
+    return launch_scal<float>(&ctx->c, N, a, dx);
+}
+int senk_axpy_f32(senk_ctx ctx, float a, const float *dx, float *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_axpby<float>(&ctx->c, N, a, dx, 1.0, dy);
+}
+int senk_dot_f32(senk_ctx ctx, const float *dx, const float *dy, int N, double *result) {
+    CTX_CHECK(ctx);
+    if (!result) return SENK_ERR_ARG;
+    if (N <= 0) {
+        *result = 0;
+        return SENK_OK;
+    }
+    SENK_TRY(launch_multidot<float>(&ctx->c, N, 1, dx, (size_t)N, dy, ctx->c.dscalars, 11));
+    SENK_HIP(hipMemcpyAsync(ctx->c.hpinned, ctx->c.dscalars, sizeof(double), hipMemcpyDeviceToHost, ctx->c.stream));
+    SENK_HIP(hipStreamSynchronize(ctx->c.stream));
+    *result = ctx->c.hpinned[0];
+    return SENK_OK;
+}
+int senk_nrm2_f32(senk_ctx ctx, const float *dx, int N, double *result) {
+    SENK_TRY(senk_dot_f32(ctx, dx, dx, N, result));
+    *result = std::sqrt(*result);
+    return SENK_OK;
+}
+int senk_convert_f64_f32(senk_ctx ctx, const double *din, float *dout, int n) {
+    CTX_CHECK(ctx);
+    return launch_convert<double, float>(&ctx->c, n, din, dout, nullptr);
+}
+int senk_convert_f32_f64(senk_ctx ctx, const float *din, double *dout, int n) {
+    CTX_CHECK(ctx);
+    return launch_convert<float, double>(&ctx->c, n, din, dout, nullptr);
+}
+int senk_scaling_dev(senk_ctx ctx, double *dval, const int *dcind, const int *drptr, const double *dld,
+                     const double *drd, int N) {
+    CTX_CHECK(ctx);
+    return launch_scaling_f64(&ctx->c, N, drptr, dcind, dval, dld, drd);
+}
+
+// ---------------------------------------------------------------- ILU(0) numeric
+}  // extern "C"
+template <typename T>
+static int ilu0_impl(senk_ctx ctx, T *dval, const int *dcind, const int *drptr, int N, const int *cptr, int cnum,
+                     int bsize, const int *hrptr, const int *hcind) {
+    CTX_CHECK(ctx);
+    if (N < 0 || !dval || !dcind || !drptr) return SENK_ERR_ARG;
+    Ctx *c = &ctx->c;
+    Schedule s;
+    std::vector<int> blk;
+    if (cptr && cnum > 0) {
+        if (bsize <= 0) return SENK_ERR_ARG;
+        if (N % bsize) return SENK_ERR_DIVISIBILITY;
+        s = colors(cptr, cnum, bsize);
+        if (hrptr && hcind && !host::schedule_ok(hrptr, hcind, N, s, blk, true, false)) return SENK_ERR_SCHEDULE;
+    } else {
+        if (!hrptr || !hcind) return SENK_ERR_ARG;
+        s.bsize = 1;
+        s.nlev = host::levels(hrptr, hcind, N, 1, false, s.lev_ptr, blk);
+        SENK_TRY(upload(ctx, &s.dblk, blk.data(), blk.size()));
+    }
+    int *dpos = nullptr, *derr = nullptr;
+    SENK_TRY(dalloc(&dpos, (size_t)N));
+    SENK_TRY(dalloc(&derr, 1));
+    SENK_HIP(hipMemsetAsync(derr, 0, sizeof(int), c->stream));
+    SENK_TRY(launch_find_diag(c, N, drptr, dcind, dpos));
+    SENK_TRY(launch_ilu0<T>(c, s, N, drptr, dcind, dpos, dval, derr));
+    int herr = 0;
+    SENK_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(dpos);
+    (void)hipFree(derr);
+    if (s.dblk) (void)hipFree(s.dblk);
+    if (herr == 1) return SENK_ERR_ZERO_PIVOT;
+    if (herr == 2) return SENK_ERR_UNSUPPORTED;
+    return SENK_OK;
+}
+extern "C" {
+int senk_ilu0_f64(senk_ctx ctx, double *dval, const int *dcind, const int *drptr, int N, const int *cptr, int cnum,
+                  int bsize, const int *hrptr, const int *hcind) {
+    return ilu0_impl<double>(ctx, dval, dcind, drptr, N, cptr, cnum, bsize, hrptr, hcind);
+}
+int senk_ilu0_f32(senk_ctx ctx, float *dval, const int *dcind, const int *drptr, int N, const int *cptr, int cnum,
+                  int bsize, const int *hrptr, const int *hcind) {
+    return ilu0_impl<float>(ctx, dval, dcind, drptr, N, cptr, cnum, bsize, hrptr, hcind);
+}
+
+// ---------------------------------------------------------------- solver twins
+static int finish(senk_ctx ctx, const MatView &A, const double *db, const double *dx, double nrm_b, senk_result *res) {
+    double tr = 0;
+    SENK_TRY(true_relres(&ctx->c, A, db, dx, nrm_b, &tr));
+    res->true_relres = tr;
+    return res->status;
+}
+static int raw_mat(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, int N, MatView *A) {
+    A->N = N;
+    A->rptr = drptr;
+    A->cind = dcind;
+    A->v64 = dval;
+    int nnz = 0;
+    SENK_TRY(nnz_of(ctx, drptr, N, &nnz));
+    A->nnz = nnz;
+    A->avg_row = avg_row(nnz, N);
+    return SENK_OK;
+}
+static int run_twin(senk_ctx ctx, const MatView &A, const PrecView &M, SolveParams &p, const double *db, double *dx,
+                    senk_result *res, double *hist, int hist_cap) {
+    senk_result tmp;
+    if (!res) res = &tmp;
+    SENK_TRY(krylov_solve<double>(&ctx->c, A, M, db, dx, p, res, hist, hist_cap));
+    return finish(ctx, A, db, dx, p.nrm_b, res);
+}
+
+int senk_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *db, double *dx,
+                    double nrm_b, int outer, int m, int N, double epsilon, int verbose, senk_result *res, double *hist,
+                    int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || m <= 0 || outer < 0) return SENK_ERR_ARG;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    M.N = N;
+    SolveParams p;
+    p.solver = SENK_GMRES;
+    p.m = m;
+    p.outer = outer;
+    p.eps = epsilon;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+static void ilu_view(PrecView &M, int N, const double *dlval, const int *dlcind, const int *dlrptr, const double *duval,
+                     const int *ducind, const int *durptr) {
+    M.N = N;
+    M.kind = 1;
+    M.l64 = dlval;
+    M.lcind = dlcind;
+    M.lrptr = dlrptr;
+    M.u64 = duval;
+    M.ucind = ducind;
+    M.urptr = durptr;
+}
+int senk_ilu_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dlval,
+                        const int *dlcind, const int *dlrptr, const double *duval, const int *ducind,
+                        const int *durptr, senk_trsv_plan lplan, senk_trsv_plan uplan, const double *db, double *dx,
+                        double nrm_b, int outer, int m, int N, double epsilon, int verbose, senk_result *res,
+                        double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || m <= 0 || !lplan || !uplan) return SENK_ERR_ARG;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    ilu_view(M, N, dlval, dlcind, dlrptr, duval, ducind, durptr);
+    M.ls = lplan->s;
+    M.us = uplan->s;
+    SolveParams p;
+    p.solver = SENK_GMRES;
+    p.m = m;
+    p.outer = outer;
+    p.eps = epsilon;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+int senk_abmc_ilu_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                             const double *dlval, const int *dlcind, const int *dlrptr, const double *duval,
+                             const int *ducind, const int *durptr, const int *cptr, int cnum, int bsize,
+                             const double *db, double *dx, double nrm_b, int outer, int m, int N, double epsilon,
+                             int verbose, senk_result *res, double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || m <= 0 || !cptr || cnum <= 0 || bsize <= 0) return SENK_ERR_ARG;
+    if (N % bsize) return SENK_ERR_DIVISIBILITY;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    ilu_view(M, N, dlval, dlcind, dlrptr, duval, ducind, durptr);
+    M.ls = colors(cptr, cnum, bsize);
+    M.us = M.ls;
+    SolveParams p;
+    p.solver = SENK_GMRES;
+    p.m = m;
+    p.outer = outer;
+    p.eps = epsilon;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+int senk_abmc_ilub_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                              const double *dblval, const int *dblcind, const int *dblrptr, const double *dbuval,
+                              const int *dbucind, const int *dburptr, const int *cptr, int cnum, int bsize,
+                              const double *db, double *dx, double nrm_b, int outer, int m, int N, double epsilon,
+                              int bnl, int bnw, int verbose, senk_result *res, double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || m <= 0 || !cptr || cnum <= 0 || bsize <= 0 || bnl <= 0 || bnw <= 0) return SENK_ERR_ARG;
+    if (N % bsize || bsize % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    M.N = N;
+    M.kind = 2;
+    M.bnl = bnl;
+    M.bnw = bnw;
+    M.bl64 = dblval;
+    M.blcind = dblcind;
+    M.blrptr = dblrptr;
+    M.bu64 = dbuval;
+    M.bucind = dbucind;
+    M.burptr = dburptr;
+    M.ls = colors(cptr, cnum, bsize);
+    M.us = M.ls;
+    SolveParams p;
+    p.solver = SENK_GMRES;
+    p.m = m;
+    p.outer = outer;
+    p.eps = epsilon;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+int senk_gcrm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *db, double *dx,
+                  double nrm_b, int outer, int m, int N, double epsilon, int verbose, senk_result *res, double *hist,
+                  int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || m <= 0) return SENK_ERR_ARG;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    M.N = N;
+    SolveParams p;
+    p.solver = SENK_GCR;
+    p.m = m;
+    p.outer = outer;
+    p.eps = epsilon;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+int senk_abmc_ilu_gcrm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dlval,
+                           const int *dlcind, const int *dlrptr, const double *duval, const int *ducind,
+                           const int *durptr, const int *cptr, int cnum, int bsize, const double *db, double *dx,
+                           double nrm_b, int outer, int m, int N, double epsilon, int verbose, senk_result *res,
+                           double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || m <= 0 || !cptr || cnum <= 0 || bsize <= 0) return SENK_ERR_ARG;
+    if (N % bsize) return SENK_ERR_DIVISIBILITY;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    ilu_view(M, N, dlval, dlcind, dlrptr, duval, ducind, durptr);
+    M.ls = colors(cptr, cnum, bsize);
+    M.us = M.ls;
+    SolveParams p;
+    p.solver = SENK_GCR;
+    p.m = m;
+    p.outer = outer;
+    p.eps = epsilon;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+int senk_bicgstab_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *db,
+                      double *dx, double nrm_b, int max_iter, int N, double epsilon, int verbose, senk_result *res,
+                      double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || max_iter < 0) return SENK_ERR_ARG;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    M.N = N;
+    SolveParams p;
+    p.solver = SENK_BICGSTAB;
+    p.max_iter = max_iter;
+    p.eps = epsilon;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+int senk_abmc_ilu_bicgstab_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                               const double *dlval, const int *dlcind, const int *dlrptr, const double *duval,
+                               const int *ducind, const int *durptr, const int *cptr, int cnum, int bsize,
+                               const double *db, double *dx, double nrm_b, int max_iter, int N, double epsilon,
+                               int verbose, senk_result *res, double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || max_iter < 0 || !cptr || cnum <= 0 || bsize <= 0) return SENK_ERR_ARG;
+    if (N % bsize) return SENK_ERR_DIVISIBILITY;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    ilu_view(M, N, dlval, dlcind, dlrptr, duval, ducind, durptr);
+    M.ls = colors(cptr, cnum, bsize);
+    M.us = M.ls;
+    SolveParams p;
+    p.solver = SENK_BICGSTAB;
+    p.max_iter = max_iter;
+    p.eps = epsilon;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+
+// ---------------------------------------------------------------- handle API
+int senk_mat_create(senk_ctx ctx, int N, const int *rptr, const int *cind, const double *val, int flags,
+                    senk_mat *out) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || !rptr || !cind || !val || !out) return SENK_ERR_ARG;
+    auto *A = new senk_mat_s();
+    A->ctx = ctx;
+    A->N = N;
+    A->nnz = rptr[N];
+    int st = SENK_OK;
+    do {
+        if ((st = upload(ctx, &A->rptr, rptr, (size_t)N + 1))) break;
+        if ((st = upload(ctx, &A->cind, cind, (size_t)A->nnz))) break;
+        if ((st = upload(ctx, &A->v64, val, (size_t)A->nnz))) break;
+        if (flags & 1) {
+            if ((st = dalloc(&A->v32, (size_t)A->nnz))) break;
+            if ((st = launch_convert<double, float>(&ctx->c, A->nnz, A->v64, A->v32, nullptr))) break;
+        }
+        if (flags & 2) {
+            senk_sellcs h;
+            if ((st = senk_csr2sellcs(val, cind, rptr, N, 64, 1024, &h))) break;
+            st = senk_sellcs_upload(ctx, &h, flags & 1, &A->sell);
+            senk_sellcs_free_host(&h);
+            if (st) break;
+            A->has_sell = true;
+        }
+        if (hipStreamSynchronize(ctx->c.stream) != hipSuccess) st = SENK_ERR_HIP;
+    } while (0);
+    if (st) {
+        senk_mat_destroy(A);
+        return st;
+    }
+    MatView &v = A->view;
+    v.N = N;
+    v.nnz = A->nnz;
+    v.rptr = A->rptr;
+    v.cind = A->cind;
+    v.v64 = A->v64;
+    v.v32 = A->v32;
+    v.avg_row = avg_row(A->nnz, N);
+    v.sell = A->has_sell ? &A->sell : nullptr;
+    *out = A;
+    return SENK_OK;
+}
+int senk_mat_destroy(senk_mat A) {
+    if (!A) return SENK_ERR_ARG;
+    (void)hipStreamSynchronize(A->ctx->c.stream);
+    (void)hipFree(A->rptr);
+    (void)hipFree(A->cind);
+    (void)hipFree(A->v64);
+    (void)hipFree(A->v32);
+    if (A->has_sell) senk_sellcs_free_device(A->ctx, &A->sell);
+    delete A;
+    return SENK_OK;
+}
+int senk_mat_spmv_f64(senk_mat A, const double *dx, double *dy) {
+    if (!A) return SENK_ERR_ARG;
+    return spmv<double>(&A->ctx->c, A->view, dx, dy);
+}
+int senk_mat_spmv_f32(senk_mat A, const float *dx, float *dy) {
+    if (!A || !A->v32) return SENK_ERR_ARG;
+    return spmv<float>(&A->ctx->c, A->view, dx, dy);
+}
+int senk_mat_nnz(senk_mat A, int *nnz) {
+    if (!A || !nnz) return SENK_ERR_ARG;
+    *nnz = A->nnz;
+    return SENK_OK;
+}
+
+int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *cind, const double *val,
+                         const int *cptr, int cnum, int bsize, const senk_ilu_opts *opts, senk_prec *out) {
+    CTX_CHECK(ctx);
+    if (!A || !rptr || !cind || !val || !opts || !out) return SENK_ERR_ARG;
+    const int N = A->N;
+    if (bsize <= 0 || N % bsize) return SENK_ERR_DIVISIBILITY;
+    Ctx *c = &ctx->c;
+    // host symbolic pattern of the factor
+    const int nnzA = rptr[N];
+    double *fv = (double *)std::malloc(sizeof(double) * (nnzA ? nnzA : 1));
+    int *fc = (int *)std::malloc(sizeof(int) * (nnzA ? nnzA : 1)), *fr = (int *)std::malloc(sizeof(int) * (N + 1));
+    std::memcpy(fv, val, sizeof(double) * nnzA);
+    std::memcpy(fc, cind, sizeof(int) * nnzA);
+    std::memcpy(fr, rptr, sizeof(int) * (N + 1));
+    int st = SENK_OK;
+    if (opts->kind == SENK_ILUP && opts->p > 0) st = senk_alloc_level_zero(&fv, &fc, &fr, N, opts->p);
+    else if (opts->kind == SENK_ILUB) st = senk_alloc_block_zero(&fv, &fc, &fr, N, opts->bnl, opts->bnw);
+    if (st) {
+        std::free(fv); std::free(fc); std::free(fr);
+        return st;
+    }
+    auto *M = new senk_prec_s();
+    M->ctx = ctx;
+    PrecView &v = M->view;
+    v.N = N;
+    v.kind = 1;
+    const int fnnz = fr[N];
+    // schedules: the colors when they keep every factor entry out of the blocks of its
+    // own launch, otherwise the dependency levels of the factor's blocks
+    std::vector<int> noblk, lblk, ublk;
+    Schedule cs;
+    bool use_colors = cptr && cnum > 0;
+    if (use_colors) {
+        cs = colors(cptr, cnum, bsize);
+        use_colors = host::schedule_ok(fr, fc, N, cs, noblk, true, true);
+    }
+    if (use_colors) {
+        v.ls = cs;
+        v.us = cs;
+    } else {
+        v.ls.bsize = v.us.bsize = bsize;
+        v.ls.nlev = host::levels(fr, fc, N, bsize, false, v.ls.lev_ptr, lblk);
+        v.us.nlev = host::levels(fr, fc, N, bsize, true, v.us.lev_ptr, ublk);
+    }
+    // host split pattern (row pointers)
+    M->hlrptr.assign((size_t)N + 1, 0);
+    M->hurptr.assign((size_t)N + 1, 0);
+    for (int i = 0; i < N; i++) {
+        int nl = 0, nu = 0;
+        for (int j = fr[i]; j < fr[i + 1]; j++) (fc[j] < i ? nl : nu)++;
+        M->hlrptr[i + 1] = M->hlrptr[i] + nl;
+        M->hurptr[i + 1] = M->hurptr[i] + nu;
+    }
+    M->lnnz = M->hlrptr[N];
+    M->unnz = M->hurptr[N];
+    int *dfr = nullptr, *dfc = nullptr;
+    double *dfv = nullptr;
+    int *lr = nullptr, *lc = nullptr, *ur = nullptr, *uc = nullptr;
+    double *l64 = nullptr, *u64 = nullptr;
+    do {
+        if ((st = upload(ctx, &dfr, fr, (size_t)N + 1))) break;
+        if ((st = upload(ctx, &dfc, fc, (size_t)fnnz))) break;
+        if ((st = upload(ctx, &dfv, fv, (size_t)fnnz))) break;
+        if (!lblk.empty()) {
+            if ((st = upload(ctx, &v.ls.dblk, lblk.data(), lblk.size()))) break;
+            M->owned.push_back(v.ls.dblk);
+        }
+        if (!ublk.empty()) {
+            if ((st = upload(ctx, &v.us.dblk, ublk.data(), ublk.size()))) break;
+            M->owned.push_back(v.us.dblk);
+        }
+        // numeric factorization on the device (dependency-respecting schedule)
+        int *dpos = nullptr, *derr = nullptr;
+        if ((st = dalloc(&dpos, (size_t)N))) break;
+        if ((st = dalloc(&derr, 1))) break;
+        if (hipMemsetAsync(derr, 0, sizeof(int), c->stream) != hipSuccess) { st = SENK_ERR_HIP; break; }
+        if ((st = launch_find_diag(c, N, dfr, dfc, dpos))) break;
+        if ((st = launch_ilu0<double>(c, v.ls, N, dfr, dfc, dpos, dfv, derr))) break;
+        int herr = 0;
+        if (hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { st = SENK_ERR_HIP; break; }
+        (void)hipFree(dpos);
+        (void)hipFree(derr);
+        if (herr == 1) { st = SENK_ERR_ZERO_PIVOT; break; }
+        if (herr == 2) { st = SENK_ERR_UNSUPPORTED; break; }
+        // split into L (strict lower) and U (inverted diagonal first)
+        if ((st = upload(ctx, &lr, M->hlrptr.data(), (size_t)N + 1))) break;
+        if ((st = upload(ctx, &ur, M->hurptr.data(), (size_t)N + 1))) break;
+        if ((st = dalloc(&lc, (size_t)M->lnnz))) break;
+        if ((st = dalloc(&uc, (size_t)M->unnz))) break;
+        if ((st = dalloc(&l64, (size_t)M->lnnz))) break;
+        if ((st = dalloc(&u64, (size_t)M->unnz))) break;
+        M->owned.insert(M->owned.end(), {lr, ur, lc, uc, l64, u64});
+        if ((st = launch_split_ldu<double>(c, N, dfr, dfc, dfv, lr, l64, lc, ur, u64, uc))) break;
+        v.lrptr = lr; v.lcind = lc; v.l64 = l64;
+        v.urptr = ur; v.ucind = uc; v.u64 = u64;
+        if (opts->precision & (1 << SENK_FACTOR_F32)) {
+            float *l32 = nullptr, *u32 = nullptr;
+            if ((st = dalloc(&l32, (size_t)M->lnnz))) break;
+            if ((st = dalloc(&u32, (size_t)M->unnz))) break;
+            M->owned.insert(M->owned.end(), {l32, u32});
+            if ((st = launch_convert<double, float>(c, M->lnnz, l64, l32, nullptr))) break;
+            if ((st = launch_convert<double, float>(c, M->unnz, u64, u32, nullptr))) break;
+            v.l32 = l32;
+            v.u32 = u32;
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) st = SENK_ERR_HIP;
+    } while (0);
+    (void)hipFree(dfr);
+    (void)hipFree(dfc);
+    (void)hipFree(dfv);
+    std::free(fv);
+    std::free(fc);
+    std::free(fr);
+    if (st) {
+        senk_prec_destroy(M);
+        return st;
+    }
+    *out = M;
+    return SENK_OK;
+}
+int senk_prec_destroy(senk_prec M) {
+    if (!M) return SENK_ERR_ARG;
+    (void)hipStreamSynchronize(M->ctx->c.stream);
+    for (void *p : M->owned) (void)hipFree(p);
+    delete M;
+    return SENK_OK;
+}
+int senk_prec_apply_f64(senk_prec M, const double *dx, double *dy) {
+    if (!M) return SENK_ERR_ARG;
+    return prec_apply<double>(&M->ctx->c, M->view, dx, nullptr, dy, nullptr, nullptr);
+}
+int senk_prec_apply_f32(senk_prec M, const float *dx, float *dy) {
+    if (!M || !M->view.l32) return SENK_ERR_ARG;
+    return prec_apply<float>(&M->ctx->c, M->view, dx, nullptr, dy, nullptr, nullptr);
+}
+int senk_prec_factor_nnz(senk_prec M, int *lnnz, int *unnz, int *nlev_l, int *nlev_u) {
+    if (!M) return SENK_ERR_ARG;
+    if (lnnz) *lnnz = M->lnnz;
+    if (unnz) *unnz = M->unnz;
+    if (nlev_l) *nlev_l = M->view.ls.nlev;
+    if (nlev_u) *nlev_u = M->view.us.nlev;
+    return SENK_OK;
+}
+int senk_prec_download(senk_prec M, int *lrptr, int *lcind, double *lval, int *urptr, int *ucind, double *uval) {
+    if (!M) return SENK_ERR_ARG;
+    const int N = M->view.N;
+    Ctx *c = &M->ctx->c;
+    if (lrptr) SENK_HIP(hipMemcpyAsync(lrptr, M->view.lrptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+    if (lcind) SENK_HIP(hipMemcpyAsync(lcind, M->view.lcind, sizeof(int) * M->lnnz, hipMemcpyDeviceToHost, c->stream));
+    if (lval) SENK_HIP(hipMemcpyAsync(lval, M->view.l64, sizeof(double) * M->lnnz, hipMemcpyDeviceToHost, c->stream));
+    if (urptr) SENK_HIP(hipMemcpyAsync(urptr, M->view.urptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+    if (ucind) SENK_HIP(hipMemcpyAsync(ucind, M->view.ucind, sizeof(int) * M->unnz, hipMemcpyDeviceToHost, c->stream));
+    if (uval) SENK_HIP(hipMemcpyAsync(uval, M->view.u64, sizeof(double) * M->unnz, hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    return SENK_OK;
+}
+
+void senk_solve_opts_default(senk_solve_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->solver = SENK_GMRES;
+    o->m = 30;
+    o->max_iter = 6000;
+    o->eps = 1e-10;
+    o->ortho = SENK_MGS;
+    o->mixed = 0;
+    o->factor_prec = SENK_FACTOR_F64;
+    o->inner_eps = 1e-5;
+    o->inner_max = 600;
+    o->check_every = 0;
+}
+int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *dx, const senk_solve_opts *o,
+               senk_result *res, double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (!A || !db || !dx || !o || !res) return SENK_ERR_ARG;
+    double nrm_b = 0;
+    SENK_TRY(senk_nrm2_f64(ctx, db, A->N, &nrm_b));
+    PrecView none;
+    none.N = A->N;
+    const PrecView &pv = M ? M->view : none;
+    if (o->mixed) {
+        if (!A->v32 || (M && !M->view.l32)) return SENK_ERR_ARG;
+        SENK_TRY(ir_solve(&ctx->c, A->view, pv, db, dx, *o, nrm_b, res, hist, hist_cap));
+    } else {
+        SolveParams p;
+        p.solver = o->solver;
+        p.m = o->m;
+        p.outer = (o->max_iter + o->m - 1) / o->m;
+        p.max_iter = o->max_iter;
+        p.eps = o->eps;
+        p.nrm_b = nrm_b;
+        p.ortho = o->ortho;
+        p.check_every = o->check_every;
+        p.verbose = o->verbose;
+        SENK_TRY(krylov_solve<double>(&ctx->c, A->view, pv, db, dx, p, res, hist, hist_cap));
+    }
+    double tr = 0;
+    SENK_TRY(true_relres(&ctx->c, A->view, db, dx, nrm_b, &tr));
+    res->true_relres = tr;
+    return res->status;
+}
+
+}  // extern "C"

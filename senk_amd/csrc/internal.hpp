@@ -1,0 +1,140 @@
+// Internal declarations shared by the HIP kernels, the host setup code and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "senk_amd.h"
+
+namespace senk_amd {
+
+// ----------------------------------------------------------------- errors
+void record_hip_error(hipError_t e, const char *what, const char *file, int line);
+void record_error(const std::string &msg);
+
+#define SENK_HIP(...)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (__VA_ARGS__);                                                  \
+        if (e_ != hipSuccess) {                                                         \
+            ::senk_amd::record_hip_error(e_, #__VA_ARGS__, __FILE__, __LINE__);         \
+            return SENK_ERR_HIP;                                                        \
+        }                                                                               \
+    } while (0)
+#define SENK_TRY(...)                                                                   \
+    do {                                                                                \
+        int s_ = (__VA_ARGS__);                                                         \
+        if (s_ != SENK_OK) return s_;                                                   \
+    } while (0)
+#define SENK_LAUNCH_CHECK() SENK_HIP(hipGetLastError())
+
+// ----------------------------------------------------------------- device buffers
+// A grow-only device arena owned by a context: solver work vectors are carved out of
+// it, so repeated solves allocate nothing after the first.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// Reduction scratch: partial sums per workgroup plus the arrival counter of the
+// "last workgroup finishes the reduction" pattern.  Counters are reset by the last
+// workgroup itself, so they are zero between kernels.
+constexpr int kRedMaxBlocks = 2048;
+constexpr int kRedMaxK = 40;       // simultaneous dot products per pass
+constexpr int kRedSlots = 16;      // independent counters (kernels in flight)
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *red_partials = nullptr;   // kRedMaxK * kRedMaxBlocks
+    unsigned *red_counters = nullptr; // kRedSlots
+    double *dscalars = nullptr;       // 64 general device scalars
+    double *hpinned = nullptr;        // 64 pinned host doubles
+    int *hpinned_i = nullptr;         // 64 pinned host ints
+    DevBuf work;                      // solver arena
+    int num_cus = 256;
+};
+
+int ctx_work(Ctx *c, size_t bytes, void **out);
+
+// ----------------------------------------------------------------- schedules
+// Blocks of `bsize` consecutive rows run in `nlev` launches; launch l covers
+// blocks blk[lev_ptr[l] .. lev_ptr[l+1]) (blk == nullptr: identity).  For an upper
+// solve the launches run from the last to the first.
+struct Schedule {
+    int bsize = 1;
+    int nlev = 0;
+    std::vector<int> lev_ptr;  // host
+    int *dblk = nullptr;       // device, optional
+};
+
+// ----------------------------------------------------------------- launchers (kernels.hip)
+template <typename T>
+int launch_spmv_csr(Ctx *c, int N, const int *rptr, const int *cind, const T *val, const T *x, T *y,
+                    double avg_row, const int *stop = nullptr);
+// y = alpha*A*x + beta*z (z may alias y); optional ||y||^2 into dresult slot.
+int launch_spmv_csr_axpby_f64(Ctx *c, int N, const int *rptr, const int *cind, const double *val,
+                              const double *x, double alpha, const double *z, double beta, double *y,
+                              double *dnrm2, double avg_row);
+int launch_spmv_csr_diag_f64(Ctx *c, int N, const int *rptr, const int *cind, const double *val,
+                             const double *diag, const double *x, double *y);
+template <typename T>
+int launch_spmv_bcsr(Ctx *c, int N, int bnl, int bnw, const int *brptr, const int *bcind, const T *bval,
+                     const T *x, T *y);
+template <typename T>
+int launch_spmv_sell(Ctx *c, int N, int C, int nslices, const int *wid, const int *cind, const T *val,
+                     const int *perm, const T *x, T *y, bool last_slice_short, const int *stop = nullptr);
+
+template <typename T>
+int launch_sptrsv(Ctx *c, const Schedule &s, bool upper, const int *rptr, const int *cind, const T *val,
+                  const T *x, T *y, const double *in_scale, const int *stop_flag);
+template <typename T>
+int launch_sptrsv_bcsr(Ctx *c, const Schedule &s, bool upper, int bnl, int bnw, const int *brptr,
+                       const int *bcind, const T *bval, const T *x, T *y);
+template <typename T>
+int launch_ilu0(Ctx *c, const Schedule &s, int N, const int *rptr, const int *cind, const int *dpos, T *val,
+                int *derr);
+int launch_find_diag(Ctx *c, int N, const int *rptr, const int *cind, int *dpos);
+template <typename T>
+int launch_split_ldu(Ctx *c, int N, const int *rptr, const int *cind, const T *val, const int *lrptr,
+                     T *lval, int *lcind, const int *urptr, T *uval, int *ucind);
+
+// blas1
+template <typename T>
+int launch_axpby(Ctx *c, int N, double a, const T *x, double b, T *y);       // y = a x + b y
+template <typename T>
+int launch_axpyz(Ctx *c, int N, double a, const T *x, const T *y, T *z);     // z = a x + y
+template <typename T>
+int launch_scal(Ctx *c, int N, double a, T *x);
+template <typename T>
+int launch_copy(Ctx *c, int N, const T *x, T *y);
+int launch_hadprod_f64(Ctx *c, int N, const double *x, double *y);
+int launch_haddiv_f64(Ctx *c, int N, const double *x, double *y);
+template <typename TI, typename TO>
+int launch_convert(Ctx *c, int N, const TI *in, TO *out, const double *dscale);
+// result[k] = sum_i x_k[i]*y[i] for k < K (x_k = xs + k*ldx), FP64 accumulation,
+// deterministic.  result is a device pointer.
+template <typename T>
+int launch_multidot(Ctx *c, int N, int K, const T *xs, size_t ldx, const T *y, double *result, int slot);
+int launch_scaling_f64(Ctx *c, int N, const int *rptr, const int *cind, double *val, const double *ld,
+                       const double *rd);
+
+// ----------------------------------------------------------------- host setup (host_setup.cpp)
+namespace host {
+int abmc(const int *cind, const int *rptr, int N, int bsize, bool isSym, bool connect, std::vector<int> &size_color,
+         std::vector<int> &LP, std::vector<int> &RP);
+int amc(const int *cind, const int *rptr, int N, bool isSym, std::vector<int> &size_color, std::vector<int> &LP,
+        std::vector<int> &RP);
+// Dependency levels of the blocks of bsize rows of a triangular pattern (lower: columns
+// < row; upper: columns > row).  Returns levels in execution order.
+int levels(const int *rptr, const int *cind, int N, int bsize, bool upper, std::vector<int> &lev_ptr,
+           std::vector<int> &blk);
+// Checks that no entry couples two different blocks of the same launch.
+bool schedule_ok(const int *rptr, const int *cind, int N, const Schedule &s, const std::vector<int> &blk,
+                 bool lower_part, bool upper_part);
+}  // namespace host
+
+}  // namespace senk_amd

@@ -1,0 +1,1048 @@
+// Device-resident Krylov drivers: GMRES(m)/FGMRES(m) (senk_gmres.hpp), GCR(m)/FGCR(m)
+// (senk_gcr.hpp), BiCGStab (senk_bicgstab.hpp) and FP32-inner / FP64-refinement
+// mixed precision.
+//
+// Vectors never leave the device.  Scalars (Hessenberg, Givens, alpha/beta/omega)
+// live in device memory and are produced by the last workgroup of the reduction that
+// needs them.  The stop test writes a device flag that every later kernel of the
+// iteration checks on entry; the host polls that flag through pinned memory a few
+// iterations behind the device, so the queue never drains while it waits.
+//
+// GMRES keeps V_k unnormalised and stores sig_k = 1/||.||: every consumer multiplies
+// by sig_k on the fly (the preconditioner input, the Gram-Schmidt coefficients and the
+// final combination), which removes the separate scaling pass of the reference's
+// Scal(1/h, V_{j+1}).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "solvers.hpp"
+
+namespace senk_amd {
+
+namespace {
+
+template <typename T> const T *mvals(const MatView &A);
+template <> const double *mvals<double>(const MatView &A) { return A.v64; }
+template <> const float *mvals<float>(const MatView &A) { return A.v32; }
+template <typename T> const T *svals(const senk_sellcs &s);
+template <> const double *svals<double>(const senk_sellcs &s) { return s.val64; }
+template <> const float *svals<float>(const senk_sellcs &s) { return s.val32; }
+template <typename T> const T *bvals(const MatView &A);
+template <> const double *bvals<double>(const MatView &A) { return A.bv64; }
+template <> const float *bvals<float>(const MatView &A) { return A.bv32; }
+
+// ----------------------------------------------------------------- small kernels
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale_copy(int N, const T *__restrict__ in, T *__restrict__ out,
+                                                       const double *__restrict__ dscale, const int *stop) {
+    if (stop && *stop) return;
+    const double sc = dscale ? *dscale : 1.0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
+        out[i] = dscale ? (T)((double)in[i] * sc) : in[i];
+}
+inline int ew_grid(long long n) {
+    long long b = (n + kBlock * 4LL - 1) / (kBlock * 4LL);
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (int)b;
+}
+
+__global__ void k_state_set(int *st, int a, int b, int c) {
+    st[0] = a;
+    st[1] = b;
+    st[2] = c;
+}
+
+// Residual y = b - A x (x == nullptr means x = 0) with ||y||^2 in FP64; the last
+// workgroup stores nrm2 = ||y||^2 and, optionally, inv = 1/||y||.
+template <typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_residual(int N, const int *__restrict__ rptr,
+                                                     const int *__restrict__ cind, const T *__restrict__ val,
+                                                     const T *__restrict__ x, const T *__restrict__ b,
+                                                     T *__restrict__ y, double *partials, unsigned *counter,
+                                                     double *nrm2, double *inv) {
+    const int rows_per_pass = (int)((long long)gridDim.x * kBlock / G);
+    const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const int q = (int)(gid % G);
+    double ss = 0;
+    for (int row = (int)(gid / G); row < N; row += rows_per_pass) {
+        T acc = 0;
+        if (x) {
+            const int e = rptr[row + 1];
+            for (int j = rptr[row] + q; j < e; j += G) acc += val[j] * x[cind[j]];
+#pragma unroll
+            for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+        }
+        if (q == 0) {
+            // Axpby(1, b, -1, y): y = 1*b + (-1)*(A x)   (senk_gmres.hpp:215-216)
+            const T r = (T)1 * b[row] + (T)(-1) * acc;
+            y[row] = r;
+            ss += (double)r * (double)r;
+        }
+    }
+    double v[1] = {ss}, t[1];
+    if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) {
+        *nrm2 = t[0];
+        if (inv) *inv = 1.0 / sqrt(t[0]);
+    }
+}
+
+template <typename T>
+int residual(Ctx *c, const MatView &A, const T *x, const T *b, T *y, double *nrm2, double *inv) {
+    constexpr int G = 8;
+    const T *v = mvals<T>(A);
+    if (!v) return SENK_ERR_ARG;
+    long long threads = (long long)A.N * G;
+    int grid = (int)((threads + kBlock - 1) / kBlock);
+    if (grid > kRedMaxBlocks) grid = kRedMaxBlocks;
+    k_residual<T, G><<<grid, kBlock, 0, c->stream>>>(A.N, A.rptr, A.cind, v, x, b, y, c->red_partials,
+                                                    c->red_counters + 1, nrm2, inv);
+    SENK_LAUNCH_CHECK();
+    return SENK_OK;
+}
+
+// ----------------------------------------------------------------- host-side stop polling
+struct Poller {
+    static constexpr int kRing = 8;
+    Ctx *c;
+    const int *dstate;
+    int *hring = nullptr;  // pinned, kRing * 4 ints
+    hipEvent_t ev[kRing] = {};
+    long long issued = 0, checked = 0;
+    bool stopped = false;
+    int init(Ctx *cc, const int *ds) {
+        c = cc;
+        dstate = ds;
+        SENK_HIP(hipHostMalloc((void **)&hring, sizeof(int) * 4 * kRing, hipHostMallocDefault));
+        for (int i = 0; i < kRing; i++) SENK_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+        return SENK_OK;
+    }
+    ~Poller() {
+        for (int i = 0; i < kRing; i++)
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+        if (hring) (void)hipHostFree(hring);
+    }
+    void reset() { issued = checked = 0; stopped = false; }
+    int mark() {
+        const int slot = (int)(issued % kRing);
+        SENK_HIP(hipMemcpyAsync(hring + 4 * slot, dstate, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+        SENK_HIP(hipEventRecord(ev[slot], c->stream));
+        issued++;
+        return SENK_OK;
+    }
+    // Wait until at most `lag` marks are unchecked; returns stop seen.
+    int poll(int lag, bool *stop) {
+        while (!stopped && issued - checked > lag) {
+            const int slot = (int)(checked % kRing);
+            SENK_HIP(hipEventSynchronize(ev[slot]));
+            if (hring[4 * slot]) stopped = true;
+            checked++;
+        }
+        *stop = stopped;
+        return SENK_OK;
+    }
+};
+constexpr int kLag = 2;
+
+// Device workspace carved from the context arena.
+struct Carver {
+    char *base;
+    size_t off = 0, cap;
+    template <typename T>
+    T *take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T *p = (T *)(base + off);
+        off += sizeof(T) * n;
+        return p;
+    }
+};
+template <typename T>
+size_t bytes_of(size_t n) {
+    return ((sizeof(T) * n + 255) & ~(size_t)255) + 256;
+}
+
+// ================================================================= GMRES kernels
+// One modified Gram-Schmidt step fused with the next inner product:
+//   w -= (H[k-1] * sig[k-1]) V_{k-1}      (skipped for k == 0)
+//   H[k] = sig[k] * (V_k, w)               (k <= j)
+//   H[j+1] = ||w||, sig[j+1] = 1/||w||     (k == j+1)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mgs(int N, const T *__restrict__ Vprev, const double *hprev,
+                                                const double *sprev, const T *__restrict__ Vk, const double *sk,
+                                                T *w, double *hout, double *sig_out, const int *stop,
+                                                double *partials, unsigned *counter) {
+    if (*stop) return;
+    const T coef = Vprev ? (T)(*hprev * *sprev) : (T)0;
+    double acc = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        T wi = w[i];
+        if (Vprev) {
+            wi = wi - coef * Vprev[i];
+            w[i] = wi;
+        }
+        acc += Vk ? (double)Vk[i] * (double)wi : (double)wi * (double)wi;
+    }
+    double v[1] = {acc}, t[1];
+    if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) {
+        if (Vk) {
+            *hout = *sk * t[0];
+        } else {
+            const double h = sqrt(t[0]);
+            *hout = h;
+            *sig_out = 1.0 / h;
+        }
+    }
+}
+
+// Classical Gram-Schmidt with re-orthogonalisation (CGS2), one pass over V_0..V_{K-1}
+// (K = j+1 <= kRedMaxK):
+//   coef_in != nullptr:  w -= sum_k coef_in[k]*sig[k] V_k  first
+//   mode 0: hout[k] = sig[k]*(V_k, w)
+//   mode 1: hout[k] += add_from[k] (k < K), hout[K] = ||w||, sig_out = 1/||w||
+template <typename T, int KMAX>
+__global__ __launch_bounds__(kBlock) void k_cgs(int N, int K, const T *__restrict__ V, size_t ld,
+                                                const double *sig, const double *coef_in, T *w, double *hout,
+                                                int mode, const double *add_from, double *sig_out, const int *stop,
+                                                double *partials, unsigned *counter) {
+    if (*stop) return;
+    __shared__ double cf[KMAX];
+    if (coef_in && threadIdx.x < K) cf[threadIdx.x] = coef_in[threadIdx.x] * sig[threadIdx.x];
+    __syncthreads();
+    double acc[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) acc[k] = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        T wi = w[i];
+        if (coef_in) {
+            double s = (double)wi;
+#pragma unroll
+            for (int k = 0; k < KMAX; k++)
+                if (k < K) s -= cf[k] * (double)V[(size_t)k * ld + i];
+            wi = (T)s;
+            w[i] = wi;
+        }
+        if (mode == 0) {
+#pragma unroll
+            for (int k = 0; k < KMAX; k++)
+                if (k < K) acc[k] += (double)V[(size_t)k * ld + i] * (double)wi;
+        } else {
+            acc[0] += (double)wi * (double)wi;
+        }
+    }
+    __shared__ double tot[KMAX];
+    const int nred = mode == 0 ? K : 1;
+    if (grid_reduce_n<KMAX>(acc, nred, partials, counter, tot)) {
+        if (mode == 0) {
+            if (threadIdx.x < K) hout[threadIdx.x] = sig[threadIdx.x] * tot[threadIdx.x];
+        } else {
+            if (add_from && threadIdx.x < K) hout[threadIdx.x] += add_from[threadIdx.x];
+            if (threadIdx.x == 0) {
+                const double h = sqrt(tot[0]);
+                hout[K] = h;
+                *sig_out = 1.0 / h;
+            }
+        }
+    }
+}
+
+// Givens update of Hessenberg column j and the stop test (senk_gmres.hpp:230-246),
+// FP64, one thread.
+__global__ void k_gmres_tail(int j, int m, double *H, double *cs, double *sn, double *e, double tol, double nrm_b,
+                             int *state, double *hist, int hidx, int hcap) {
+    if (threadIdx.x != 0 || state[0]) return;
+    double *hc = H + (size_t)j * (m + 1);
+    for (int k = 0; k < j; k++) {
+        const double a = hc[k], b = hc[k + 1];
+        hc[k] = cs[k] * a - sn[k] * b;
+        hc[k + 1] = sn[k] * a + cs[k] * b;
+    }
+    const double a = hc[j], b = hc[j + 1];
+    const double r = sqrt(a * a + b * b);
+    cs[j] = a / r;
+    sn[j] = -b / r;
+    hc[j] = r;
+    hc[j + 1] = 0;
+    e[j + 1] = sn[j] * e[j];
+    e[j] = cs[j] * e[j];
+    const double ae = fabs(e[j + 1]);
+    if (hist && hidx < hcap) hist[hidx] = ae / nrm_b;
+    state[1] = j + 1;
+    if (!isfinite(ae) || !isfinite(r)) {
+        state[2] = 1;
+        state[0] = 1;
+    } else if (ae <= tol) {
+        state[0] = 1;
+    }
+}
+
+// Cycle end: y = Trsv(H, e) over the jdone columns (senk_blas2.hpp:24-34) and the
+// combination coefficients y_k * sig_k (or y_k for FGMRES's Z vectors).
+__global__ void k_gmres_y(int m, const double *H, const double *e, const double *sig, double *y, double *coef,
+                          const int *state, int use_sig) {
+    if (threadIdx.x != 0) return;
+    const int J = state[1];
+    for (int i = J - 1; i >= 0; i--) {
+        double t = e[i];
+        for (int jj = J - 1; jj > i; jj--) t -= H[(size_t)jj * (m + 1) + i] * y[jj];
+        y[i] = t / H[(size_t)i * (m + 1) + i];
+    }
+    for (int k = 0; k < J; k++) coef[k] = use_sig ? y[k] * sig[k] : y[k];
+}
+
+// out = sum_{k<J} coef_k V_k (accumulate: out += ..., in the reference's sequential
+// Axpy order).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_combine(int N, const T *__restrict__ V, size_t ld,
+                                                    const double *__restrict__ coef, const int *state, T *out,
+                                                    int accumulate) {
+    const int J = state[1];
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        double acc = accumulate ? (double)out[i] : 0.0;
+        if (!accumulate && J > 0) {
+            acc = coef[0] * (double)V[i];
+            for (int k = 1; k < J; k++) acc += coef[k] * (double)V[(size_t)k * ld + i];
+        } else {
+            for (int k = 0; k < J; k++) acc += coef[k] * (double)V[(size_t)k * ld + i];
+        }
+        out[i] = (T)acc;
+    }
+}
+
+__global__ void k_gmres_start(const double *nrm2, double *e, double *sig, int *state) {
+    if (threadIdx.x != 0) return;
+    const double n = sqrt(*nrm2);
+    e[0] = n;
+    sig[0] = 1.0 / n;
+    state[0] = 0;
+    state[1] = 0;
+    state[2] = 0;
+}
+
+}  // namespace
+
+// ================================================================= dispatch
+template <typename T>
+int spmv(Ctx *c, const MatView &A, const T *x, T *y, const int *stop) {
+    if (A.sell) {
+        const T *sv = svals<T>(*A.sell);
+        if (sv)
+            return launch_spmv_sell<T>(c, A.N, A.sell->C, A.sell->nslices, A.sell->slice_ptr, A.sell->cind, sv,
+                                       A.sell->perm, x, y, false, stop);
+    }
+    if (A.bnl) {
+        const T *bv = bvals<T>(A);
+        if (bv) return launch_spmv_bcsr<T>(c, A.N, A.bnl, A.bnw, A.brptr, A.bcind, bv, x, y);
+    }
+    const T *v = mvals<T>(A);
+    if (!v) return SENK_ERR_ARG;
+    return launch_spmv_csr<T>(c, A.N, A.rptr, A.cind, v, x, y, A.avg_row, stop);
+}
+template int spmv<double>(Ctx *, const MatView &, const double *, double *, const int *);
+template int spmv<float>(Ctx *, const MatView &, const float *, float *, const int *);
+
+int learned_apply_f32(Ctx *c, const PrecView &M, const float *r_in, const double *in_scale, float *out,
+                      const int *stop);
+int learned_apply_f64(Ctx *c, const PrecView &M, const double *r_in, const double *in_scale, double *out,
+                      const int *stop);
+
+template <typename T>
+int learned_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T *out, const int *stop);
+template <>
+int learned_apply<float>(Ctx *c, const PrecView &M, const float *in, const double *s, float *out, const int *stop) {
+    return learned_apply_f32(c, M, in, s, out, stop);
+}
+template <>
+int learned_apply<double>(Ctx *c, const PrecView &M, const double *in, const double *s, double *out,
+                          const int *stop) {
+    return learned_apply_f64(c, M, in, s, out, stop);
+}
+
+template <typename T>
+int prec_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T *out, T *tmp, const int *stop) {
+    const int N = M.N;
+    if (M.kind == 0) {
+        k_scale_copy<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, out, in_scale, stop);
+        SENK_LAUNCH_CHECK();
+    } else if (M.kind == 1) {
+        const T *lv = std::is_same<T, double>::value ? (const T *)M.l64 : (const T *)M.l32;
+        const T *uv = std::is_same<T, double>::value ? (const T *)M.u64 : (const T *)M.u32;
+        if (!lv || !uv) return SENK_ERR_ARG;
+        SENK_TRY(launch_sptrsv<T>(c, M.ls, false, M.lrptr, M.lcind, lv, in, out, in_scale, stop));
+        SENK_TRY(launch_sptrsv<T>(c, M.us, true, M.urptr, M.ucind, uv, out, out, nullptr, stop));
+    } else if (M.kind == 2) {
+        const T *lv = std::is_same<T, double>::value ? (const T *)M.bl64 : (const T *)M.bl32;
+        const T *uv = std::is_same<T, double>::value ? (const T *)M.bu64 : (const T *)M.bu32;
+        if (!lv || !uv) return SENK_ERR_ARG;
+        const T *src = in;
+        if (in_scale) {
+            k_scale_copy<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, tmp, in_scale, stop);
+            SENK_LAUNCH_CHECK();
+            src = tmp;
+        }
+        SENK_TRY(launch_sptrsv_bcsr<T>(c, M.ls, false, M.bnl, M.bnw, M.blrptr, M.blcind, lv, src, out));
+        SENK_TRY(launch_sptrsv_bcsr<T>(c, M.us, true, M.bnl, M.bnw, M.burptr, M.bucind, uv, out, out));
+    } else {
+        return SENK_ERR_ARG;
+    }
+    if (M.learned) SENK_TRY(learned_apply<T>(c, M, in, in_scale, out, stop));
+    return SENK_OK;
+}
+template int prec_apply<double>(Ctx *, const PrecView &, const double *, const double *, double *, double *,
+                                const int *);
+template int prec_apply<float>(Ctx *, const PrecView &, const float *, const double *, float *, float *,
+                               const int *);
+
+// ================================================================= GMRES / FGMRES
+template <typename T>
+static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
+                 senk_result *res, double *hist, int hist_cap) {
+    const int N = A.N, m = p.m;
+    const bool flexible = p.solver == SENK_FGMRES;
+    const int hcap = p.outer * m;
+    size_t need = bytes_of<T>((size_t)(m + 1) * N) + (flexible ? bytes_of<T>((size_t)m * N) : 0) +
+                  3 * bytes_of<T>(N) + bytes_of<double>((size_t)(m + 1) * m) + 7 * bytes_of<double>(m + 2) +
+                  bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(8);
+    void *wsp;
+    SENK_TRY(ctx_work(c, need, &wsp));
+    Carver cv{(char *)wsp, 0, need};
+    T *V = cv.take<T>((size_t)(m + 1) * N);
+    T *Z = flexible ? cv.take<T>((size_t)m * N) : nullptr;
+    T *t = cv.take<T>(N), *u = cv.take<T>(N), *tmp = cv.take<T>(N);
+    double *H = cv.take<double>((size_t)(m + 1) * m);
+    double *cs = cv.take<double>(m + 2), *sn = cv.take<double>(m + 2), *e = cv.take<double>(m + 2);
+    double *y = cv.take<double>(m + 2), *coef = cv.take<double>(m + 2), *sig = cv.take<double>(m + 2);
+    double *h2 = cv.take<double>(m + 2);
+    double *dhist = cv.take<double>(hcap + 1);
+    int *state = cv.take<int>(8);
+    double *dsc = cv.take<double>(8);
+
+    Poller poll;
+    SENK_TRY(poll.init(c, state));
+    hipEvent_t e0, e1;
+    SENK_HIP(hipEventCreate(&e0));
+    SENK_HIP(hipEventCreate(&e1));
+    SENK_HIP(hipEventRecord(e0, c->stream));
+
+    const double tol = p.nrm_b * p.eps;
+    const bool prec = M.kind != 0 || M.learned;
+    const bool use_cgs = p.ortho == SENK_CGS2;
+    int total = p.outer * m, converged = 0, breakdown = 0, cycles = 0;
+    int hstate[4] = {0, 0, 0, 0};
+    for (int i = 0; i < p.outer; i++) {
+        cycles++;
+        SENK_TRY(residual<T>(c, A, x, b, V, dsc, nullptr));
+        k_gmres_start<<<1, 64, 0, c->stream>>>(dsc, e, sig, state);
+        SENK_LAUNCH_CHECK();
+        poll.reset();
+        bool stop = false;
+        for (int j = 0; j < m && !stop; j++) {
+            T *vj = V + (size_t)j * N, *w = V + (size_t)(j + 1) * N;
+            T *z = flexible ? Z + (size_t)j * N : t;
+            // z = M^-1 (sig_j V_j);  w = A z
+            SENK_TRY(prec_apply<T>(c, M, vj, sig + j, z, tmp, state));
+            SENK_TRY(spmv<T>(c, A, z, w, state));
+            double *hc = H + (size_t)j * (m + 1);
+            if (use_cgs && j + 1 <= kRedMaxK) {
+                const int K = j + 1, grid = red_grid(N, 8);
+                // h1 = V^T w;  w -= V h1, h2 = V^T w;  w -= V h2, H[:,j] = h1 + h2, ||w||
+                k_cgs<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, K, V, N, sig, nullptr, w, hc, 0, nullptr,
+                                                                   nullptr, state, c->red_partials,
+                                                                   c->red_counters + 2);
+                SENK_LAUNCH_CHECK();
+                k_cgs<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, K, V, N, sig, hc, w, h2, 0, nullptr, nullptr,
+                                                                   state, c->red_partials, c->red_counters + 2);
+                SENK_LAUNCH_CHECK();
+                k_cgs<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, K, V, N, sig, h2, w, hc, 1, h2, sig + j + 1,
+                                                                   state, c->red_partials, c->red_counters + 2);
+                SENK_LAUNCH_CHECK();
+            } else {
+                const int grid = red_grid(N, 8);
+                for (int k = 0; k <= j + 1; k++) {
+                    const T *Vprev = k > 0 ? V + (size_t)(k - 1) * N : nullptr;
+                    const T *Vk = k <= j ? V + (size_t)k * N : nullptr;
+                    k_mgs<T><<<grid, kBlock, 0, c->stream>>>(N, Vprev, k > 0 ? hc + k - 1 : nullptr,
+                                                             k > 0 ? sig + k - 1 : nullptr, Vk, sig + k, w, hc + k,
+                                                             sig + j + 1, state, c->red_partials,
+                                                             c->red_counters + 2);
+                    SENK_LAUNCH_CHECK();
+                }
+            }
+            k_gmres_tail<<<1, 64, 0, c->stream>>>(j, m, H, cs, sn, e, tol, p.nrm_b, state, dhist, i * m + j, hcap);
+            SENK_LAUNCH_CHECK();
+            SENK_TRY(poll.mark());
+            SENK_TRY(poll.poll(kLag, &stop));
+        }
+        // cycle end: x += M^-1 (V y)   (senk_gmres.hpp:248-255), or x += V y / Z y
+        k_gmres_y<<<1, 64, 0, c->stream>>>(m, H, e, sig, y, coef, state, flexible ? 0 : 1);
+        SENK_LAUNCH_CHECK();
+        if (flexible) {
+            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, Z, N, coef, state, x, 1);
+            SENK_LAUNCH_CHECK();
+        } else if (!prec) {
+            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, V, N, coef, state, x, 1);
+            SENK_LAUNCH_CHECK();
+        } else {
+            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, V, N, coef, state, u, 0);
+            SENK_LAUNCH_CHECK();
+            SENK_TRY(prec_apply<T>(c, M, u, nullptr, t, tmp, nullptr));
+            SENK_TRY(launch_axpby<T>(c, N, 1.0, t, 1.0, x));
+        }
+        SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+        SENK_HIP(hipStreamSynchronize(c->stream));
+        if (hstate[0]) {
+            breakdown = hstate[2];
+            if (!breakdown) {
+                converged = 1;
+                total = i * m + hstate[1];
+            }
+            break;
+        }
+    }
+    SENK_HIP(hipEventRecord(e1, c->stream));
+    SENK_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SENK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int nh = std::min(hcap, converged ? total : cycles * m);
+    std::vector<double> hh((size_t)std::max(nh, 1));
+    if (nh > 0) SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * nh, hipMemcpyDeviceToHost));
+    if (hist) for (int k = 0; k < std::min(nh, hist_cap); k++) hist[k] = hh[k];
+    res->iters = converged ? total : p.outer * m;
+    res->converged = converged;
+    res->restarts = cycles;
+    res->implicit_relres = nh > 0 ? hh[nh - 1] : 0;
+    res->t_solve_ms = ms;
+    res->status = breakdown ? SENK_ERR_BREAKDOWN : SENK_OK;
+    if (p.verbose) {
+        if (converged) {
+            std::printf("I iter %d\n", res->iters);
+            std::printf("# res %e\n", res->implicit_relres);
+        } else {
+            std::printf("# iter %d\n", p.outer * m);
+            std::printf("# res : Check by using senk::test\n");
+        }
+        std::fflush(stdout);
+    }
+    return SENK_OK;
+}
+
+// ================================================================= GCR / FGCR kernels
+namespace {
+// dot_Ap[j] = (Ap_j, Ap_j); alpha = (Ap_j, r) / dot_Ap[j]     (senk_gcr.hpp:119-120)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gcr_alpha(int N, const T *__restrict__ Apj, const T *__restrict__ r,
+                                                      double *dotAp, double *alpha, const int *stop,
+                                                      double *partials, unsigned *counter) {
+    if (*stop) return;
+    double a0 = 0, a1 = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const double ap = (double)Apj[i];
+        a0 += ap * ap;
+        a1 += ap * (double)r[i];
+    }
+    double v[2] = {a0, a1}, t[2];
+    if (grid_reduce<2>(v, partials, counter, t) && threadIdx.x == 0) {
+        *dotAp = t[0];
+        *alpha = t[1] / t[0];
+    }
+}
+// x += alpha p_j; r -= alpha Ap_j; optional ||r|| stop test (check_every).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gcr_xr(int N, const T *__restrict__ pj, const T *__restrict__ Apj,
+                                                   const double *alpha, T *x, T *r, int check, double tol,
+                                                   double nrm_b, int hidx, int jstep, int *state, double *hist,
+                                                   int hcap, double *partials, unsigned *counter) {
+    if (state[0]) return;
+    const T a = (T)*alpha;
+    double ss = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        x[i] += a * pj[i];
+        const T ri = r[i] + (-a) * Apj[i];
+        r[i] = ri;
+        ss += (double)ri * (double)ri;
+    }
+    if (!check) return;
+    double v[1] = {ss}, t[1];
+    if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) {
+        const double nr = sqrt(t[0]);
+        if (hist && hidx < hcap) hist[hidx] = nr / nrm_b;
+        state[1] = jstep + 1;
+        if (!isfinite(nr)) { state[2] = 1; state[0] = 1; }
+        else if (nr < tol) state[0] = 1;
+    }
+}
+// beta_k = -(Ap_k, AKr) / dot_Ap[k], k <= j                  (senk_gcr.hpp:126-133)
+template <typename T, int KMAX>
+__global__ __launch_bounds__(kBlock) void k_gcr_betas(int N, int K, const T *__restrict__ Ap, size_t ld,
+                                                      const T *__restrict__ AKr, const double *dotAp, double *beta,
+                                                      const int *stop, double *partials, unsigned *counter) {
+    if (*stop) return;
+    double acc[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) acc[k] = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const double a = (double)AKr[i];
+#pragma unroll
+        for (int k = 0; k < KMAX; k++)
+            if (k < K) acc[k] += (double)Ap[(size_t)k * ld + i] * a;
+    }
+    __shared__ double tot[KMAX];
+    if (grid_reduce_n<KMAX>(acc, K, partials, counter, tot)) {
+        if (threadIdx.x < K) beta[threadIdx.x] = -tot[threadIdx.x] / dotAp[threadIdx.x];
+    }
+}
+// p_{j+1} = beta_0 p_0 + Kr + sum_{k>=1} beta_k p_k, and the same for Ap  (:127-133)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gcr_newdir(int N, int K, T *p, T *Ap, size_t ld, const T *Kr,
+                                                       const T *AKr, const double *beta, const int *stop) {
+    if (*stop) return;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        T a = (T)beta[0] * p[i] + Kr[i];
+        T b = (T)beta[0] * Ap[i] + AKr[i];
+        for (int k = 1; k < K; k++) {
+            a += (T)beta[k] * p[(size_t)k * ld + i];
+            b += (T)beta[k] * Ap[(size_t)k * ld + i];
+        }
+        p[(size_t)K * ld + i] = a;
+        Ap[(size_t)K * ld + i] = b;
+    }
+}
+__global__ void k_gcr_restart(const double *nrm2, double tol, double nrm_b, int *state, double *hist, int hidx,
+                              int hcap, double *nr_out) {
+    if (threadIdx.x != 0) return;
+    const double nr = sqrt(*nrm2);
+    *nr_out = nr;
+    if (hist && hidx < hcap) hist[hidx] = nr / nrm_b;
+    state[0] = 0;
+    state[1] = 0;
+    state[2] = 0;
+    if (!isfinite(nr)) { state[2] = 1; state[0] = 1; }
+    else if (nr < tol) state[0] = 1;
+}
+}  // namespace
+
+template <typename T>
+static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
+               senk_result *res, double *hist, int hist_cap) {
+    const int N = A.N, m = p.m;
+    if (m > kRedMaxK) return SENK_ERR_UNSUPPORTED;
+    const bool prec = M.kind != 0 || M.learned;
+    const int hcap = p.check_every ? p.outer * (m + 1) + 1 : p.outer + 1;
+    size_t need = 2 * bytes_of<T>((size_t)(m + 1) * N) + 4 * bytes_of<T>(N) + 4 * bytes_of<double>(m + 2) +
+                  bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(8);
+    void *wsp;
+    SENK_TRY(ctx_work(c, need, &wsp));
+    Carver cv{(char *)wsp, 0, need};
+    T *P = cv.take<T>((size_t)(m + 1) * N), *AP = cv.take<T>((size_t)(m + 1) * N);
+    T *r = cv.take<T>(N), *Kr = cv.take<T>(N), *AKr = cv.take<T>(N), *tmp = cv.take<T>(N);
+    double *dotAp = cv.take<double>(m + 2), *beta = cv.take<double>(m + 2), *alpha = cv.take<double>(m + 2);
+    double *nrv = cv.take<double>(m + 2);
+    double *dhist = cv.take<double>(hcap + 1);
+    int *state = cv.take<int>(8);
+    double *dsc = cv.take<double>(8);
+    Poller poll;
+    SENK_TRY(poll.init(c, state));
+    hipEvent_t e0, e1;
+    SENK_HIP(hipEventCreate(&e0));
+    SENK_HIP(hipEventCreate(&e1));
+    SENK_HIP(hipEventRecord(e0, c->stream));
+    const double tol = p.nrm_b * p.eps;
+    int converged = 0, breakdown = 0, last = 0, cycles = 0, nh = 0, hstate[4] = {0, 0, 0, 0};
+    const int grid = red_grid(N, 8);
+    for (int i = 0; i < p.outer; i++) {
+        cycles++;
+        SENK_TRY(residual<T>(c, A, x, b, r, dsc, nullptr));
+        const int hidx = p.check_every ? i * (m + 1) : i;
+        k_gcr_restart<<<1, 64, 0, c->stream>>>(dsc, tol, p.nrm_b, state, dhist, hidx, hcap, nrv);
+        SENK_LAUNCH_CHECK();
+        SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+        SENK_HIP(hipStreamSynchronize(c->stream));
+        nh = hidx + 1;
+        last = i * m;
+        if (hstate[0]) {
+            breakdown = hstate[2];
+            converged = !breakdown;
+            break;
+        }
+        // p_0 = M^-1 r (or r);  Ap_0 = A p_0
+        if (prec) SENK_TRY(prec_apply<T>(c, M, r, nullptr, P, tmp, state));
+        else SENK_TRY(launch_copy<T>(c, N, r, P));
+        SENK_TRY(spmv<T>(c, A, P, AP, state));
+        poll.reset();
+        bool stop = false;
+        for (int j = 0; j < m && !stop; j++) {
+            T *pj = P + (size_t)j * N, *Apj = AP + (size_t)j * N;
+            k_gcr_alpha<T><<<grid, kBlock, 0, c->stream>>>(N, Apj, r, dotAp + j, alpha, state, c->red_partials,
+                                                           c->red_counters + 4);
+            SENK_LAUNCH_CHECK();
+            k_gcr_xr<T><<<grid, kBlock, 0, c->stream>>>(N, pj, Apj, alpha, x, r, p.check_every, tol, p.nrm_b,
+                                                        i * (m + 1) + 1 + j, j, state, dhist, hcap,
+                                                        c->red_partials, c->red_counters + 5);
+            SENK_LAUNCH_CHECK();
+            const T *z = r;
+            if (prec) {
+                SENK_TRY(prec_apply<T>(c, M, r, nullptr, Kr, tmp, state));
+                z = Kr;
+            }
+            SENK_TRY(spmv<T>(c, A, z, AKr, state));
+            k_gcr_betas<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, j + 1, AP, N, AKr, dotAp, beta, state,
+                                                                     c->red_partials, c->red_counters + 6);
+            SENK_LAUNCH_CHECK();
+            k_gcr_newdir<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, j + 1, P, AP, N, z, AKr, beta, state);
+            SENK_LAUNCH_CHECK();
+            if (p.check_every) {
+                SENK_TRY(poll.mark());
+                SENK_TRY(poll.poll(kLag, &stop));
+            }
+        }
+        if (p.check_every) {
+            SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+            SENK_HIP(hipStreamSynchronize(c->stream));
+            nh = i * (m + 1) + 1 + hstate[1];
+            if (hstate[0]) {
+                breakdown = hstate[2];
+                converged = !breakdown;
+                last = i * m + hstate[1];
+                break;
+            }
+        }
+    }
+    SENK_HIP(hipEventRecord(e1, c->stream));
+    SENK_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SENK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<double> hh((size_t)hcap + 1, 0.0);
+    SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * hcap, hipMemcpyDeviceToHost));
+    // history: per-restart values (reference) or every step when check_every
+    int out_n = 0;
+    if (!p.check_every) {
+        out_n = std::min(nh, hcap);
+        if (hist) for (int k = 0; k < std::min(out_n, hist_cap); k++) hist[k] = hh[k];
+    } else {
+        // restart value, then one value per step, cycle after cycle
+        out_n = std::min(nh, hcap);
+        if (hist) for (int k = 0; k < std::min(out_n, hist_cap); k++) hist[k] = hh[k];
+    }
+    res->iters = last;
+    res->converged = converged;
+    res->restarts = cycles;
+    res->implicit_relres = out_n > 0 ? (p.check_every ? hh[std::max(0, nh - 1)] : hh[out_n - 1]) : 0;
+    res->t_solve_ms = ms;
+    res->status = breakdown ? SENK_ERR_BREAKDOWN : SENK_OK;
+    if (p.verbose) {
+        for (int k = 0; k < std::min(cycles, hcap); k++) {
+            const int idx = p.check_every ? k * (m + 1) : k;
+            std::printf("%d %e\n", k * m, hh[idx]);
+        }
+        std::fflush(stdout);
+    }
+    return SENK_OK;
+}
+
+// ================================================================= BiCGStab kernels
+namespace {
+// alpha = rr / (AKp, rstr)                                   (senk_bicgstab.hpp:140)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_alpha(int N, const T *__restrict__ AKp, const T *__restrict__ rs,
+                                                       const double *rr, double *alpha, const int *stop,
+                                                       double *partials, unsigned *counter) {
+    if (*stop) return;
+    double a = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
+        a += (double)AKp[i] * (double)rs[i];
+    double v[1] = {a}, t[1];
+    if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) *alpha = *rr / t[0];
+}
+// s = -alpha AKp + r                                          (:141)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_s(int N, const T *__restrict__ AKp, const T *__restrict__ r,
+                                                   const double *alpha, T *__restrict__ s, const int *stop) {
+    if (*stop) return;
+    const T a = (T)(-*alpha);
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
+        s[i] = a * AKp[i] + r[i];
+}
+// omega = (AKs, s) / (AKs, AKs)                               (:145)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_omega(int N, const T *__restrict__ AKs, const T *__restrict__ s,
+                                                       double *omega, const int *stop, double *partials,
+                                                       unsigned *counter) {
+    if (*stop) return;
+    double a0 = 0, a1 = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const double v = (double)AKs[i];
+        a0 += v * (double)s[i];
+        a1 += v * v;
+    }
+    double v[2] = {a0, a1}, t[2];
+    if (grid_reduce<2>(v, partials, counter, t) && threadIdx.x == 0) *omega = t[0] / t[1];
+}
+// x += alpha Kp; x += omega Ks; r = -omega AKs + s; ||r||, (r, rstr), beta, stop
+// (:146-159)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_update(int N, const T *__restrict__ Kp, const T *__restrict__ Ks,
+                                                        const T *__restrict__ AKs, const T *__restrict__ s,
+                                                        const T *__restrict__ rs, T *x, T *r, double *sc,
+                                                        double tol, double nrm_b, int iter, int *state,
+                                                        double *hist, int hcap, double *partials,
+                                                        unsigned *counter) {
+    if (state[0]) return;
+    // sc: [0] alpha, [1] omega, [2] rr, [3] beta
+    const T a = (T)sc[0], w = (T)sc[1];
+    double ss = 0, rr = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        T xi = x[i] + a * Kp[i];
+        xi = xi + w * Ks[i];
+        x[i] = xi;
+        const T ri = (-w) * AKs[i] + s[i];
+        r[i] = ri;
+        ss += (double)ri * (double)ri;
+        rr += (double)ri * (double)rs[i];
+    }
+    double v[2] = {ss, rr}, t[2];
+    if (grid_reduce<2>(v, partials, counter, t) && threadIdx.x == 0) {
+        const double nr = sqrt(t[0]);
+        if (hist && iter < hcap) hist[iter] = nr / nrm_b;
+        state[1] = iter + 1;
+        if (!isfinite(nr)) {
+            state[2] = 1;
+            state[0] = 1;
+        } else if (nr < tol) {
+            state[0] = 1;
+        } else {
+            const double prev = sc[2];
+            sc[2] = t[1];
+            sc[3] = sc[0] / sc[1] * t[1] / prev;
+        }
+    }
+}
+// p = beta (-omega AKp + p) + r                                (:160-161)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_p(int N, const T *__restrict__ AKp, const T *__restrict__ r,
+                                                   const double *sc, T *p, const int *stop) {
+    if (*stop) return;
+    const T w = (T)(-sc[1]), be = (T)sc[3];
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const T tmp = w * AKp[i] + p[i];
+        p[i] = be * tmp + r[i];
+    }
+}
+__global__ void k_bicg_init(const double *rr_src, double *sc, int *state) {
+    if (threadIdx.x != 0) return;
+    sc[2] = *rr_src;
+    state[0] = 0;
+    state[1] = 0;
+    state[2] = 0;
+}
+}  // namespace
+
+template <typename T>
+static int bicgstab(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
+                    senk_result *res, double *hist, int hist_cap) {
+    const int N = A.N;
+    const int hcap = p.max_iter + 1;
+    size_t need = 10 * bytes_of<T>(N) + bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(16);
+    void *wsp;
+    SENK_TRY(ctx_work(c, need, &wsp));
+    Carver cv{(char *)wsp, 0, need};
+    T *r = cv.take<T>(N), *rs = cv.take<T>(N), *pp = cv.take<T>(N), *Kp = cv.take<T>(N), *AKp = cv.take<T>(N);
+    T *s = cv.take<T>(N), *Ks = cv.take<T>(N), *AKs = cv.take<T>(N), *tmp = cv.take<T>(N);
+    double *dhist = cv.take<double>(hcap + 1);
+    int *state = cv.take<int>(8);
+    double *sc = cv.take<double>(16);
+    Poller poll;
+    SENK_TRY(poll.init(c, state));
+    hipEvent_t e0, e1;
+    SENK_HIP(hipEventCreate(&e0));
+    SENK_HIP(hipEventCreate(&e1));
+    SENK_HIP(hipEventRecord(e0, c->stream));
+    const double tol = p.eps * p.nrm_b;
+    const int grid = red_grid(N, 8);
+    // r = b - A x; rstr = r; p = r; rr = (r, rstr)
+    SENK_TRY(residual<T>(c, A, x, b, r, sc + 4, nullptr));
+    SENK_TRY(launch_copy<T>(c, N, r, rs));
+    SENK_TRY(launch_copy<T>(c, N, r, pp));
+    SENK_TRY(launch_multidot<T>(c, N, 1, r, N, rs, sc + 5, 7));
+    k_bicg_init<<<1, 64, 0, c->stream>>>(sc + 5, sc, state);
+    SENK_LAUNCH_CHECK();
+    poll.reset();
+    bool stop = false;
+    for (int i = 0; i < p.max_iter && !stop; i++) {
+        SENK_TRY(prec_apply<T>(c, M, pp, nullptr, Kp, tmp, state));
+        SENK_TRY(spmv<T>(c, A, Kp, AKp, state));
+        k_bicg_alpha<T><<<grid, kBlock, 0, c->stream>>>(N, AKp, rs, sc + 2, sc + 0, state, c->red_partials,
+                                                        c->red_counters + 8);
+        SENK_LAUNCH_CHECK();
+        k_bicg_s<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, AKp, r, sc + 0, s, state);
+        SENK_LAUNCH_CHECK();
+        SENK_TRY(prec_apply<T>(c, M, s, nullptr, Ks, tmp, state));
+        SENK_TRY(spmv<T>(c, A, Ks, AKs, state));
+        k_bicg_omega<T><<<grid, kBlock, 0, c->stream>>>(N, AKs, s, sc + 1, state, c->red_partials,
+                                                        c->red_counters + 9);
+        SENK_LAUNCH_CHECK();
+        k_bicg_update<T><<<grid, kBlock, 0, c->stream>>>(N, Kp, Ks, AKs, s, rs, x, r, sc, tol, p.nrm_b, i, state,
+                                                         dhist, hcap, c->red_partials, c->red_counters + 10);
+        SENK_LAUNCH_CHECK();
+        k_bicg_p<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, AKp, r, sc, pp, state);
+        SENK_LAUNCH_CHECK();
+        SENK_TRY(poll.mark());
+        SENK_TRY(poll.poll(kLag, &stop));
+    }
+    int hstate[4];
+    SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipEventRecord(e1, c->stream));
+    SENK_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SENK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int converged = hstate[0] && !hstate[2];
+    const int iters = hstate[0] ? hstate[1] : p.max_iter;
+    std::vector<double> hh((size_t)std::max(iters, 1), 0.0);
+    if (iters > 0) SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * std::min(iters, hcap), hipMemcpyDeviceToHost));
+    if (hist) for (int k = 0; k < std::min(iters, hist_cap); k++) hist[k] = hh[k];
+    res->iters = iters;
+    res->converged = converged;
+    res->restarts = 0;
+    res->implicit_relres = iters > 0 ? hh[iters - 1] : 0;
+    res->t_solve_ms = ms;
+    res->status = hstate[2] ? SENK_ERR_BREAKDOWN : SENK_OK;
+    if (p.verbose) {
+        for (int k = 0; k < iters; k++) std::printf("%d %e\n", k + 1, hh[k]);
+        if (converged) {
+            std::printf("# iter %d\n", iters);
+            std::printf("# res %e\n", res->implicit_relres);
+        } else {
+            std::printf("# iter %d (max)\n", iters);
+            std::printf("# res %e\n", res->implicit_relres);
+        }
+        std::fflush(stdout);
+    }
+    return SENK_OK;
+}
+
+template <typename T>
+int krylov_solve(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
+                 senk_result *res, double *hist, int hist_cap) {
+    std::memset(res, 0, sizeof(*res));
+    if (A.N <= 0 || p.m <= 0 || p.m > 512) return SENK_ERR_ARG;
+    switch (p.solver) {
+        case SENK_GMRES:
+        case SENK_FGMRES: return gmres<T>(c, A, M, b, x, p, res, hist, hist_cap);
+        case SENK_GCR:
+        case SENK_FGCR: return gcr<T>(c, A, M, b, x, p, res, hist, hist_cap);
+        case SENK_BICGSTAB: return bicgstab<T>(c, A, M, b, x, p, res, hist, hist_cap);
+        default: return SENK_ERR_ARG;
+    }
+}
+template int krylov_solve<double>(Ctx *, const MatView &, const PrecView &, const double *, double *,
+                                  const SolveParams &, senk_result *, double *, int);
+template int krylov_solve<float>(Ctx *, const MatView &, const PrecView &, const float *, float *,
+                                 const SolveParams &, senk_result *, double *, int);
+
+// ================================================================= mixed precision
+namespace {
+// x += scale * d  (FP64 x, FP32 d, scale = ||r|| on the device)
+__global__ __launch_bounds__(kBlock) void k_ir_update(int N, const float *__restrict__ d, const double *nrm2,
+                                                      double *x) {
+    const double s = sqrt(*nrm2);
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
+        x[i] += s * (double)d[i];
+}
+}  // namespace
+
+int true_relres(Ctx *c, const MatView &A, const double *b, const double *x, double nrm_b, double *out) {
+    void *wsp;
+    SENK_TRY(ctx_work(c, bytes_of<double>(A.N) + bytes_of<double>(4), &wsp));
+    Carver cv{(char *)wsp, 0, 0};
+    double *r = cv.take<double>(A.N);
+    double *d = cv.take<double>(4);
+    SENK_TRY(residual<double>(c, A, x, b, r, d, nullptr));
+    double h = 0;
+    SENK_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    *out = std::sqrt(h) / nrm_b;
+    return SENK_OK;
+}
+
+// FP64 iterative refinement around an FP32 Krylov solve (GMRES-IR): the FP64 residual
+// r = b - A x drives the stop test; the inner solve gets r/||r|| in FP32 and its
+// correction is added back in FP64.
+int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x, const senk_solve_opts &o,
+             double nrm_b, senk_result *res, double *hist, int hist_cap) {
+    std::memset(res, 0, sizeof(*res));
+    const int N = A.N;
+    if (!A.v32) return SENK_ERR_ARG;
+    // Inner solver workspace uses the same arena, so keep the IR vectors separate.
+    double *r64 = nullptr;
+    float *r32 = nullptr, *d32 = nullptr;
+    double *dsc = nullptr;
+    SENK_HIP(hipMalloc(&r64, sizeof(double) * N));
+    SENK_HIP(hipMalloc(&r32, sizeof(float) * N));
+    SENK_HIP(hipMalloc(&d32, sizeof(float) * N));
+    SENK_HIP(hipMalloc(&dsc, sizeof(double) * 4));
+    hipEvent_t e0, e1;
+    SENK_HIP(hipEventCreate(&e0));
+    SENK_HIP(hipEventCreate(&e1));
+    SENK_HIP(hipEventRecord(e0, c->stream));
+    SolveParams ip;
+    ip.solver = o.solver;
+    ip.m = o.m;
+    ip.eps = o.inner_eps;
+    ip.nrm_b = 1.0;
+    ip.ortho = o.ortho;
+    ip.check_every = 1;
+    ip.max_iter = o.inner_max;
+    ip.outer = (o.inner_max + o.m - 1) / o.m;
+    const int max_steps = std::max(1, (o.max_iter + o.inner_max - 1) / std::max(1, o.inner_max)) + 8;
+    int steps = 0, inner_total = 0, converged = 0, status = SENK_OK;
+    double relres = 0;
+    for (;;) {
+        SENK_TRY(residual<double>(c, A, x, b, r64, dsc, dsc + 1));
+        double h2 = 0;
+        SENK_HIP(hipMemcpyAsync(&h2, dsc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SENK_HIP(hipStreamSynchronize(c->stream));
+        relres = std::sqrt(h2) / nrm_b;
+        if (hist && steps < hist_cap) hist[steps] = relres;
+        if (o.verbose) { std::printf("IR %d %e\n", steps, relres); std::fflush(stdout); }
+        if (!std::isfinite(relres)) { status = SENK_ERR_BREAKDOWN; break; }
+        if (relres <= o.eps) { converged = 1; break; }
+        if (steps >= max_steps || inner_total >= o.max_iter) break;
+        SENK_TRY(launch_convert<double, float>(c, N, r64, r32, dsc + 1));
+        SENK_HIP(hipMemsetAsync(d32, 0, sizeof(float) * N, c->stream));
+        senk_result ir;
+        SENK_TRY(krylov_solve<float>(c, A, M, r32, d32, ip, &ir, nullptr, 0));
+        inner_total += ir.iters;
+        k_ir_update<<<ew_grid(N), kBlock, 0, c->stream>>>(N, d32, dsc, x);
+        SENK_LAUNCH_CHECK();
+        steps++;
+    }
+    SENK_HIP(hipEventRecord(e1, c->stream));
+    SENK_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SENK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(r64);
+    (void)hipFree(r32);
+    (void)hipFree(d32);
+    (void)hipFree(dsc);
+    res->iters = inner_total;
+    res->inner_iters = inner_total;
+    res->restarts = steps;
+    res->converged = converged;
+    res->implicit_relres = relres;
+    res->true_relres = relres;
+    res->t_solve_ms = ms;
+    res->status = status;
+    return SENK_OK;
+}
+
+}  // namespace senk_amd

@@ -1,0 +1,72 @@
+// Internal interface of the device-resident Krylov drivers.
+#pragma once
+
+#include "internal.hpp"
+
+namespace senk_amd {
+
+struct MatView {
+    int N = 0, nnz = 0;
+    const int *rptr = nullptr, *cind = nullptr;
+    const double *v64 = nullptr;
+    const float *v32 = nullptr;
+    double avg_row = 0;
+    // optional SELL-C-sigma copy (device)
+    const senk_sellcs *sell = nullptr;
+    // optional BCSR form used for SpMV (device); bnl == 0 when absent
+    int bnl = 0, bnw = 0;
+    const int *brptr = nullptr, *bcind = nullptr;
+    const double *bv64 = nullptr;
+    const float *bv32 = nullptr;
+};
+
+struct LearnedPrec;  // gnn.hip
+
+struct PrecView {
+    int N = 0;
+    int kind = 0;  // 0 none, 1 ILU (CSR factors), 2 ILUB (BCSR factors)
+    Schedule ls, us;
+    // CSR factors: L strict lower, U with the inverted diagonal first
+    const int *lrptr = nullptr, *lcind = nullptr, *urptr = nullptr, *ucind = nullptr;
+    const double *l64 = nullptr, *u64 = nullptr;
+    const float *l32 = nullptr, *u32 = nullptr;
+    // BCSR factors
+    int bnl = 0, bnw = 0;
+    const int *blrptr = nullptr, *blcind = nullptr, *burptr = nullptr, *bucind = nullptr;
+    const double *bl64 = nullptr, *bu64 = nullptr;
+    const float *bl32 = nullptr, *bu32 = nullptr;
+    // learned correction added to the ILU application (flexible solvers only)
+    LearnedPrec *learned = nullptr;
+    int *learned_off = nullptr;  // device flag: 1 = correction disabled by the monitor
+};
+
+template <typename T>
+int spmv(Ctx *c, const MatView &A, const T *x, T *y, const int *stop = nullptr);
+template <typename T>
+int prec_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T *out, T *tmp, const int *stop);
+
+struct SolveParams {
+    int solver = SENK_GMRES;
+    int m = 30;
+    int outer = 1;       // GMRES/GCR cycles
+    int max_iter = 1000; // BiCGStab
+    double eps = 1e-8;
+    double nrm_b = 1;
+    int ortho = SENK_MGS;
+    int check_every = 0;
+    int verbose = 0;
+    int unprec_direct = 0;  // GMRES with no preconditioner: x += sum y_k v_k (senk_gmres.hpp:80-82)
+};
+
+// x (device) is the initial guess and the output.  hist (host) optional.
+template <typename T>
+int krylov_solve(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
+                 senk_result *res, double *hist, int hist_cap);
+
+int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x, const senk_solve_opts &o,
+             double nrm_b, senk_result *res, double *hist, int hist_cap);
+
+// ||b - A x|| / nrm_b in FP64 (synchronises).
+int true_relres(Ctx *c, const MatView &A, const double *b, const double *x, double nrm_b, double *out);
+
+}  // namespace senk_amd
