@@ -334,7 +334,7 @@ def split(val, cind, rptr, N, key="L-DU", invDiag=True):
     """Returns dict with L (and U, diag as the key dictates) as numpy arrays."""
     ptrs = [C.c_void_p() for _ in range(7)]
     val, cind, rptr = _f64(val), _i32(cind), _i32(rptr)
-    k = _SPLIT_KEYS[key]
+    k = _SPLIT_KEYS.get(key, -1)
     call("senk_split", _ptr(val), _ptr(cind), _ptr(rptr), *(C.byref(p) for p in ptrs[:6]), C.byref(ptrs[6]), N,
          key.encode(), int(invDiag))
     out = {}
@@ -349,15 +349,8 @@ def split(val, cind, rptr, N, key="L-DU", invDiag=True):
 
 
 def _pattern_inplace(fn, val, cind, rptr, N, *extra):
-    v, c, r = _f64(val), _i32(cind), _i32(rptr)
-    pv, pc, pr = C.c_void_p(), C.c_void_p(), C.c_void_p()
-    # hand over malloc'd copies (the library frees and replaces them)
-    for src, dst, dt in ((v, pv, np.float64), (c, pc, np.int32), (r, pr, np.int32)):
-        m = C.cdll.LoadLibrary("libc.so.6").malloc
-        m.restype = C.c_void_p
-        m.argtypes = [C.c_size_t]
-        dst.value = m(max(1, src.nbytes))
-        C.memmove(dst.value, src.ctypes.data, src.nbytes)
+    # hand over malloc'd copies (the library frees and replaces them, as the reference does)
+    pv, pc, pr = _malloc_copy(_f64(val)), _malloc_copy(_i32(cind)), _malloc_copy(_i32(rptr))
     call(fn, C.byref(pv), C.byref(pc), C.byref(pr), N, *extra)
     rp = _take(pr, N + 1, np.int32)
     nnz = int(rp[-1])
@@ -408,13 +401,23 @@ def csr2sellcs(val, cind, rptr, N, C_=64, sigma=1024):
     return s
 
 
+def _malloc_copy(a: np.ndarray) -> C.c_void_p:
+    m = C.CDLL(None).malloc
+    m.restype = C.c_void_p
+    m.argtypes = [C.c_size_t]
+    p = C.c_void_p(m(max(1, a.nbytes)))
+    C.memmove(p.value, a.ctypes.data, a.nbytes)
+    return p
+
+
 def padding(val, cind, rptr, size, N):
-    v, c, r = _f64(val), _i32(cind), _i32(rptr)
-    extra = (size - N % size) % size
-    v2 = np.concatenate([v, np.ones(extra)])
-    c2 = np.concatenate([c, np.arange(N, N + extra, dtype=np.int32)])
-    r2 = np.concatenate([r, r[-1] + np.arange(1, extra + 1, dtype=np.int32)])
-    return v2, c2, r2, N + extra
+    """Padding (identity rows up to a multiple of size); returns (val, cind, rptr, N)."""
+    pv, pc, pr = _malloc_copy(_f64(val)), _malloc_copy(_i32(cind)), _malloc_copy(_i32(rptr))
+    n = C.c_int(N)
+    call("senk_padding", C.byref(pv), C.byref(pc), C.byref(pr), size, C.byref(n))
+    r = _take(pr, n.value + 1, np.int32)
+    nnz = int(r[-1])
+    return _take(pv, nnz, np.float64), _take(pc, nnz, np.int32), r, n.value
 
 
 def csr2csc(val, cind, rptr, N, M):

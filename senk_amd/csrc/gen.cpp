@@ -179,8 +179,9 @@ int senk_gen_vardiff7(int n, int seed, double corr_len, double sigma, int *N, in
 // Unstructured nonsymmetric matrix: row i draws round(lognormal(ln 12, 0.6)) clipped to
 // [3, 128] off-diagonal candidates with columns (i + round(N(0, 0.001 n))) mod n;
 // duplicates and the row's own column are dropped.  Off-diagonal values N(0,1) rounded
-// to FP32; the diagonal is 1.1 x the row's absolute off-diagonal sum, rounded to FP32,
-// so every value is exact in FP32.  One mt19937(seed) stream, rows in order.
+// to FP32; the diagonal is 1.1 x the row's absolute off-diagonal sum, rounded to FP32
+// (1 for a row left without off-diagonals, which only small n produces), so every
+// value is exact in FP32.  One mt19937(seed) stream, rows in order.
 int senk_gen_unstructured(int n, int seed, int *N, int **rptr, int **cind, double **val) {
     if (n <= 1 || !N || !rptr || !cind || !val) return SENK_ERR_ARG;
     std::mt19937 eng((unsigned)seed);
@@ -208,7 +209,7 @@ int senk_gen_unstructured(int n, int seed, int *N, int **rptr, int **cind, doubl
                   row.end());
         double s = 0;
         for (auto &e : row) s += std::fabs(e.second);
-        const double d = (double)(float)(1.1 * s);
+        const double d = s > 0 ? (double)(float)(1.1 * s) : 1.0;
         bool placed = false;
         for (auto &e : row) {
             if (!placed && e.first > i) { ci.push_back(i); v.push_back(d); placed = true; }

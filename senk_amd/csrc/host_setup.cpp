@@ -334,8 +334,8 @@ int senk_csr2sell(const double *val, const int *cind, const int *rptr, double **
 }
 
 // SELL-C-sigma packing: stable sort by decreasing row length inside windows of sigma
-// rows, slices of C packed rows, slot-major with stride C; padding slots repeat the
-// row's last column (value 0) so the x gather stays on a line already loaded.
+// rows, slices of C packed rows, slot-major with stride C; padding slots hold value 0
+// at column 0 as in Csr2Sell, so sigma = 1 with N % C == 0 reproduces Csr2Sell's arrays.
 int senk_csr2sellcs(const double *val, const int *cind, const int *rptr, int N, int C, int sigma, senk_sellcs *out) {
     if (!cind || !rptr || !out || C <= 0 || sigma <= 0 || N < 0) return SENK_ERR_ARG;
     std::memset(out, 0, sizeof(*out));
@@ -375,10 +375,9 @@ int senk_csr2sellcs(const double *val, const int *cind, const int *rptr, int N, 
         for (int k = 0; k < C; k++) {
             const int r = order[(size_t)s * C + k];
             const int n = r >= 0 ? rptr[r + 1] - rptr[r] : 0;
-            const int last = n > 0 ? cind[rptr[r] + n - 1] : 0;
             for (int q = 0; q < w; q++) {
                 const size_t o = base + (size_t)q * C + k;
-                out->cind[o] = q < n ? cind[rptr[r] + q] : last;
+                out->cind[o] = q < n ? cind[rptr[r] + q] : 0;
                 out->val64[o] = (q < n && val) ? val[rptr[r] + q] : 0.0;
             }
         }
