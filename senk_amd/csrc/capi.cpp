@@ -175,9 +175,7 @@ int senk_ctx_create(int device, senk_ctx *out) {
     Ctx &c = ctx->c;
     c.device = device;
     SENK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    SENK_HIP(hipMalloc(&c.red_partials, sizeof(double) * kRedMaxK * kRedMaxBlocks));
-    SENK_HIP(hipMalloc(&c.red_counters, sizeof(unsigned) * kRedSlots));
-    SENK_HIP(hipMemset(c.red_counters, 0, sizeof(unsigned) * kRedSlots));
+    SENK_HIP(hipMalloc(&c.red_partials, sizeof(double) * kRedSlots * kRedMaxK * kRedMaxBlocks));
     SENK_HIP(hipMalloc(&c.dscalars, sizeof(double) * 64));
     SENK_HIP(hipHostMalloc((void **)&c.hpinned, sizeof(double) * 64, hipHostMallocDefault));
     SENK_HIP(hipHostMalloc((void **)&c.hpinned_i, sizeof(int) * 64, hipHostMallocDefault));
@@ -194,7 +192,6 @@ int senk_ctx_destroy(senk_ctx ctx) {
     (void)hipStreamSynchronize(c.stream);
     if (c.work.p) (void)hipFree(c.work.p);
     (void)hipFree(c.red_partials);
-    (void)hipFree(c.red_counters);
     (void)hipFree(c.dscalars);
     (void)hipHostFree(c.hpinned);
     (void)hipHostFree(c.hpinned_i);
@@ -517,7 +514,7 @@ int senk_dot_f64_dev(senk_ctx ctx, const double *dx, const double *dy, int N, do
         SENK_HIP(hipMemsetAsync(dres, 0, sizeof(double), ctx->c.stream));
         return SENK_OK;
     }
-    return launch_multidot<double>(&ctx->c, N, 1, dx, (size_t)N, dy, dres, 11);
+    return launch_multidot<double>(&ctx->c, N, 1, dx, (size_t)N, dy, dres);
 }
 int senk_dot_f64(senk_ctx ctx, const double *dx, const double *dy, int N, double *result) {
     CTX_CHECK(ctx);
@@ -561,7 +558,7 @@ int senk_dot_f32(senk_ctx ctx, const float *dx, const float *dy, int N, double *
         *result = 0;
         return SENK_OK;
     }
-    SENK_TRY(launch_multidot<float>(&ctx->c, N, 1, dx, (size_t)N, dy, ctx->c.dscalars, 11));
+    SENK_TRY(launch_multidot<float>(&ctx->c, N, 1, dx, (size_t)N, dy, ctx->c.dscalars));
     SENK_HIP(hipMemcpyAsync(ctx->c.hpinned, ctx->c.dscalars, sizeof(double), hipMemcpyDeviceToHost, ctx->c.stream));
     SENK_HIP(hipStreamSynchronize(ctx->c.stream));
     *result = ctx->c.hpinned[0];

@@ -1,5 +1,5 @@
-// Device-side helpers shared by the gfx950 kernels: wave64 DPP reductions and a
-// deterministic single-pass grid reduction.
+// Device-side helpers shared by the gfx950 kernels: wave64 DPP reductions and
+// deterministic cross-workgroup reductions through per-workgroup partials.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -50,24 +50,21 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     return t;
 }
 
-__device__ __forceinline__ double load_agent(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Cross-workgroup reductions never synchronise inside a kernel: every workgroup
+// stores its partial sums (store_partials), and the consumer -- the next kernel in
+// stream order, or a small finalize kernel -- adds the partials in a fixed order
+// (sum_partials).  Kernel boundaries make the partials visible, so no device-scope
+// fence is needed (on MI355X such a fence writes back the per-XCD L2), and the result
+// does not depend on scheduling.
+constexpr int kPartLd = kRedMaxBlocks;  // stride between the K partial arrays of a slot
 
-// Grid reduction of K per-thread values in one kernel: each workgroup writes its
-// partial sums, the last workgroup to arrive (arrival counter, reset by that same
-// workgroup) adds the partials in block order, so the result is independent of
-// scheduling.  Returns true in every thread of the last workgroup, with tot[] valid
-// in all of its threads.  All threads of the grid must call it.
 template <int K>
-__device__ __forceinline__ bool grid_reduce(const double (&v)[K], double *partials, unsigned *counter,
-                                            double (&tot)[K]) {
+__device__ __forceinline__ void store_partials(const double (&v)[K], double *partials) {
     __shared__ double sh[K][kWavesPerBlock];
-    __shared__ int is_last;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        double s = wave_sum(v[k]);
+        const double s = wave_sum(v[k]);
         if (lane == 0) sh[k][w] = s;
     }
     __syncthreads();
@@ -75,40 +72,18 @@ __device__ __forceinline__ bool grid_reduce(const double (&v)[K], double *partia
         double s = 0;
 #pragma unroll
         for (int i = 0; i < kWavesPerBlock; i++) s += sh[threadIdx.x][i];
-        partials[(size_t)threadIdx.x * kRedMaxBlocks + blockIdx.x] = s;
+        partials[(size_t)threadIdx.x * kPartLd + blockIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    __shared__ double shb[kWavesPerBlock];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double s = 0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-            s += load_agent(partials + (size_t)k * kRedMaxBlocks + b);
-        tot[k] = block_sum(s, shb);
-    }
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-    return true;
 }
-
-// Same for a runtime count K <= KMAX (used by the multi-vector Gram-Schmidt passes).
+// Runtime count K <= KMAX.
 template <int KMAX>
-__device__ __forceinline__ bool grid_reduce_n(const double *v, int K, double *partials, unsigned *counter,
-                                              double *tot_sh /* KMAX doubles in LDS */) {
+__device__ __forceinline__ void store_partials_n(const double *v, int K, double *partials) {
     __shared__ double sh[KMAX][kWavesPerBlock];
-    __shared__ int is_last;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
         if (k < K) {
-            double s = wave_sum(v[k]);
+            const double s = wave_sum(v[k]);
             if (lane == 0) sh[k][w] = s;
         }
     }
@@ -117,36 +92,35 @@ __device__ __forceinline__ bool grid_reduce_n(const double *v, int K, double *pa
         double s = 0;
 #pragma unroll
         for (int i = 0; i < kWavesPerBlock; i++) s += sh[threadIdx.x][i];
-        partials[(size_t)threadIdx.x * kRedMaxBlocks + blockIdx.x] = s;
+        partials[(size_t)threadIdx.x * kPartLd + blockIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    __shared__ double shb[kWavesPerBlock];
-    for (int k = 0; k < K; k++) {
-        double s = 0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-            s += load_agent(partials + (size_t)k * kRedMaxBlocks + b);
-        s = block_sum(s, shb);
-        if (threadIdx.x == 0) tot_sh[k] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-    return true;
+}
+// Sum of p[0..B) in a fixed order; every thread of the workgroup gets the total.  All
+// threads must call it.
+__device__ __forceinline__ double sum_partials(const double *p, int B) {
+    __shared__ double sh[kWavesPerBlock];
+    double s = 0;
+    for (int b = threadIdx.x; b < B; b += kBlock) s += p[b];
+    return block_sum(s, sh);
 }
 
-// Grid size for a grid-stride reduction pass over n elements.
-inline int red_grid(long long n, int per_thread = 4) {
-    long long b = (n + (long long)kBlock * per_thread - 1) / ((long long)kBlock * per_thread);
+// Streaming passes: kUnroll elements per thread per trip, at most kRedMaxBlocks
+// workgroups (so a consumer sums at most that many partials).
+constexpr int kUnroll = 4;
+inline int red_grid(long long n) {
+    long long b = (n + (long long)kBlock * kUnroll - 1) / ((long long)kBlock * kUnroll);
     if (b < 1) b = 1;
     if (b > kRedMaxBlocks) b = kRedMaxBlocks;
     return (int)b;
 }
+// for (i over this thread's elements) with kUnroll independent loads in flight
+#define SENK_STREAM(i, n, ...)                                                                          \
+    for (long long i##0_ = (long long)blockIdx.x * kBlock * kUnroll + threadIdx.x; i##0_ < (n);       \
+         i##0_ += (long long)gridDim.x * kBlock * kUnroll) {                                            \
+        _Pragma("unroll") for (int u_ = 0; u_ < kUnroll; u_++) {                                        \
+            const long long i = i##0_ + (long long)u_ * kBlock;                                         \
+            if (i < (n)) { __VA_ARGS__ }                                                                \
+        }                                                                                               \
+    }
 
 }  // namespace senk_amd

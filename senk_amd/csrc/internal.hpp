@@ -42,15 +42,14 @@ struct DevBuf {
 // Reduction scratch: partial sums per workgroup plus the arrival counter of the
 // "last workgroup finishes the reduction" pattern.  Counters are reset by the last
 // workgroup itself, so they are zero between kernels.
-constexpr int kRedMaxBlocks = 2048;
-constexpr int kRedMaxK = 40;       // simultaneous dot products per pass
-constexpr int kRedSlots = 16;      // independent counters (kernels in flight)
+constexpr int kRedMaxBlocks = 1024;  // workgroups of a reducing pass
+constexpr int kRedMaxK = 40;          // simultaneous dot products per pass
+constexpr int kRedSlots = 8;          // independent partial buffers
 
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    double *red_partials = nullptr;   // kRedMaxK * kRedMaxBlocks
-    unsigned *red_counters = nullptr; // kRedSlots
+    double *red_partials = nullptr;   // kRedSlots * kRedMaxK * kRedMaxBlocks
     double *dscalars = nullptr;       // 64 general device scalars
     double *hpinned = nullptr;        // 64 pinned host doubles
     int *hpinned_i = nullptr;         // 64 pinned host ints
@@ -59,6 +58,7 @@ struct Ctx {
 };
 
 int ctx_work(Ctx *c, size_t bytes, void **out);
+inline double *red_slot(Ctx *c, int slot) { return c->red_partials + (size_t)slot * kRedMaxK * kRedMaxBlocks; }
 
 // ----------------------------------------------------------------- schedules
 // Blocks of `bsize` consecutive rows run in `nlev` launches; launch l covers
@@ -75,10 +75,6 @@ struct Schedule {
 template <typename T>
 int launch_spmv_csr(Ctx *c, int N, const int *rptr, const int *cind, const T *val, const T *x, T *y,
                     double avg_row, const int *stop = nullptr);
-// y = alpha*A*x + beta*z (z may alias y); optional ||y||^2 into dresult slot.
-int launch_spmv_csr_axpby_f64(Ctx *c, int N, const int *rptr, const int *cind, const double *val,
-                              const double *x, double alpha, const double *z, double beta, double *y,
-                              double *dnrm2, double avg_row);
 int launch_spmv_csr_diag_f64(Ctx *c, int N, const int *rptr, const int *cind, const double *val,
                              const double *diag, const double *x, double *y);
 template <typename T>
@@ -118,7 +114,7 @@ int launch_convert(Ctx *c, int N, const TI *in, TO *out, const double *dscale);
 // result[k] = sum_i x_k[i]*y[i] for k < K (x_k = xs + k*ldx), FP64 accumulation,
 // deterministic.  result is a device pointer.
 template <typename T>
-int launch_multidot(Ctx *c, int N, int K, const T *xs, size_t ldx, const T *y, double *result, int slot);
+int launch_multidot(Ctx *c, int N, int K, const T *xs, size_t ldx, const T *y, double *result);
 int launch_scaling_f64(Ctx *c, int N, const int *rptr, const int *cind, double *val, const double *ld,
                        const double *rd);
 

@@ -31,39 +31,24 @@ __global__ __launch_bounds__(kBlock) void k_spmv_csr(int N, const int *__restric
     T acc = 0;
     if (row < N) {
         const int e = rptr[row + 1];
-        for (int j = rptr[row] + q; j < e; j += G) acc += val[j] * x[cind[j]];
+        // chunks of 4 independent gathers per lane, summed in entry order
+        for (int j0 = rptr[row] + q; j0 < e; j0 += 4 * G) {
+            T xv[4], vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = j0 + u * G;
+                if (j < e) {
+                    vv[u] = val[j];
+                    xv[u] = x[cind[j]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (j0 + u * G < e) acc += vv[u] * xv[u];
+        }
     }
     acc = group_sum<T, G>(acc);
     if (q == 0 && row < N) y[row] = acc;
-}
-
-// y = alpha*A*x + beta*z, optional ||y||^2 (grid-stride, bounded grid).
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_spmv_csr_axpby(int N, const int *__restrict__ rptr,
-                                                           const int *__restrict__ cind,
-                                                           const double *__restrict__ val,
-                                                           const double *__restrict__ x, double alpha,
-                                                           const double *z, double beta, double *y,
-                                                           double *partials, unsigned *counter, double *dnrm2) {
-    const int rows_per_pass = (int)((long long)gridDim.x * kBlock / G);
-    const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
-    const int q = (int)(gid % G);
-    double ss = 0;
-    for (int row = (int)(gid / G); row < N; row += rows_per_pass) {
-        double acc = 0;
-        const int e = rptr[row + 1];
-        for (int j = rptr[row] + q; j < e; j += G) acc += val[j] * x[cind[j]];
-        acc = group_sum<double, G>(acc);
-        if (q == 0) {
-            double r = alpha * acc + (beta != 0.0 ? beta * z[row] : 0.0);
-            y[row] = r;
-            ss += r * r;
-        }
-    }
-    if (dnrm2) {
-        double v[1] = {ss}, t[1];
-        if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) *dnrm2 = t[0];
-    }
 }
 
 template <typename T>
@@ -88,21 +73,6 @@ template int launch_spmv_csr<double>(Ctx *, int, const int *, const int *, const
                                      double *, double, const int *);
 template int launch_spmv_csr<float>(Ctx *, int, const int *, const int *, const float *, const float *, float *,
                                     double, const int *);
-
-int launch_spmv_csr_axpby_f64(Ctx *c, int N, const int *rptr, const int *cind, const double *val,
-                              const double *x, double alpha, const double *z, double beta, double *y,
-                              double *dnrm2, double avg_row) {
-    if (N <= 0) return SENK_OK;
-    constexpr int G = 8;
-    (void)avg_row;
-    long long threads = (long long)N * G;
-    int grid = (int)((threads + kBlock - 1) / kBlock);
-    if (grid > kRedMaxBlocks) grid = kRedMaxBlocks;
-    k_spmv_csr_axpby<G><<<grid, kBlock, 0, c->stream>>>(N, rptr, cind, val, x, alpha, z, beta, y, c->red_partials,
-                                                        c->red_counters + 0, dnrm2);
-    SENK_LAUNCH_CHECK();
-    return SENK_OK;
-}
 
 // senk_sparse.hpp:47-57 form: the diagonal product is the first addend.
 __global__ __launch_bounds__(kBlock) void k_spmv_csr_diag(int N, const int *__restrict__ rptr,
@@ -144,10 +114,25 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
     const size_t base = (size_t)wid[s] * C + k;
     const int w = wid[s + 1] - wid[s];
     T acc = 0;
-    if (w > 0) acc = val[base] * x[cind[base]];
-    for (int j = 1; j < w; j++) {
-        const size_t o = base + (size_t)j * h;
-        acc += val[o] * x[cind[o]];
+    // slot 0 starts the sum (as the reference's SpmvSell); later slots in order, with
+    // four independent gathers in flight per lane
+    for (int j0 = 0; j0 < w; j0 += 4) {
+        T xv[4], vv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int j = j0 + u;
+            if (j < w) {
+                const size_t o = base + (size_t)j * h;
+                vv[u] = val[o];
+                xv[u] = x[cind[o]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int j = j0 + u;
+            if (j == 0) acc = vv[u] * xv[u];
+            else if (j < w) acc += vv[u] * xv[u];
+        }
     }
     const int row = perm ? perm[(size_t)s * C + k] : s * C + k;
     if (row >= 0) y[row] = acc;
@@ -324,6 +309,120 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_packed(int nblk, int blk0, co
     if (active) y[row] = t;
 }
 
+// LDS-staged form for launches over a contiguous row range (colors of an ABMC
+// ordering are contiguous): a workgroup of kBlock consecutive rows first copies its
+// rows' CSR slice [rptr[r0], rptr[r0+kBlock]) into LDS with coalesced loads, then runs
+// the same two phases on the staged copy.  Per row the arithmetic and its order are
+// those of k_sptrsv_packed (bit-identical results); only the memory path changes.
+// A slice larger than the LDS buffer falls back to reading global memory.
+template <typename T, int BS, bool UPPER, bool STAGED>
+__device__ __forceinline__ void sptrsv_rows(int row, bool active, const int *__restrict__ rptr,
+                                            const int *__restrict__ cind, const T *__restrict__ val, const int *lc,
+                                            const T *lv, int e0, const T *x, T *y, const double *in_scale) {
+    auto C = [&](int j) { return STAGED ? lc[j - e0] : cind[j]; };
+    auto V = [&](int j) { return STAGED ? lv[j - e0] : val[j]; };
+    constexpr int CH = 8;  // gathers issued together
+    const int r = row % BS, base = row - r;
+    T t = 0, dinv = 0;
+    int jin = 0, jlim = 0;
+    if (active) {
+        const int s = rptr[row], e = rptr[row + 1];
+        t = x[row];
+        if (!UPPER) {
+            if (in_scale) t = (T)((double)t * *in_scale);
+            // external entries are the prefix with col < base; find its end first so the
+            // gathers below do not wait on each other
+            int je = s;
+            while (je < e && C(je) < base) je++;
+            for (int j0 = s; j0 < je; j0 += CH) {
+                T yv[CH], vv[CH];
+#pragma unroll
+                for (int q = 0; q < CH; q++)
+                    if (j0 + q < je) {
+                        vv[q] = V(j0 + q);
+                        yv[q] = y[C(j0 + q)];
+                    }
+#pragma unroll
+                for (int q = 0; q < CH; q++)
+                    if (j0 + q < je) t -= vv[q] * yv[q];
+            }
+            jin = je;
+            jlim = e;
+        } else {
+            dinv = V(s);
+            // external entries are the suffix with col >= base + BS, consumed descending
+            int je = e - 1;
+            while (je > s && C(je) >= base + BS) je--;
+            for (int j0 = e - 1; j0 > je; j0 -= CH) {
+                T yv[CH], vv[CH];
+#pragma unroll
+                for (int q = 0; q < CH; q++)
+                    if (j0 - q > je) {
+                        vv[q] = V(j0 - q);
+                        yv[q] = y[C(j0 - q)];
+                    }
+#pragma unroll
+                for (int q = 0; q < CH; q++)
+                    if (j0 - q > je) t -= vv[q] * yv[q];
+            }
+            jin = je;
+            jlim = s;
+        }
+    }
+    if (BS > 1) {
+        const int lane = threadIdx.x & 63, lbase = lane - r;
+        if (!UPPER) {
+#pragma unroll 4
+            for (int c = 0; c < BS; c++) {
+                const T yc = __shfl(t, lbase + c, 64);
+                if (active && jin < jlim && C(jin) == base + c) {
+                    t -= V(jin) * yc;
+                    jin++;
+                }
+            }
+        } else {
+#pragma unroll 4
+            for (int c = BS - 1; c >= 0; c--) {
+                if (r == c) t = t * dinv;
+                const T yc = __shfl(t, lbase + c, 64);
+                if (active && jin > jlim && C(jin) == base + c) {
+                    t -= V(jin) * yc;
+                    jin--;
+                }
+            }
+        }
+    } else if (UPPER) {
+        t = t * dinv;
+    }
+    if (active) y[row] = t;
+}
+
+template <typename T, int BS, bool UPPER>
+__global__ __launch_bounds__(kBlock) void k_sptrsv_lds(int row0, int nrows, const int *__restrict__ rptr,
+                                                       const int *__restrict__ cind, const T *__restrict__ val,
+                                                       const T *x, T *y, const double *__restrict__ in_scale,
+                                                       const int *__restrict__ stop_flag) {
+    if (stop_flag && *stop_flag) return;
+    constexpr int CAP = (40 * 1024) / (sizeof(T) + sizeof(int));
+    __shared__ T lv[CAP];
+    __shared__ int lc[CAP];
+    const int r_begin = row0 + blockIdx.x * kBlock;
+    const int r_end = min(row0 + nrows, r_begin + kBlock);
+    const int e0 = rptr[r_begin], e1 = rptr[r_end];
+    const int row = r_begin + threadIdx.x;
+    const bool active = row < r_end;
+    if (e1 - e0 <= CAP) {
+        for (int q = threadIdx.x; q < e1 - e0; q += kBlock) {
+            lv[q] = val[e0 + q];
+            lc[q] = cind[e0 + q];
+        }
+        __syncthreads();
+        sptrsv_rows<T, BS, UPPER, true>(row, active, rptr, cind, val, lc, lv, e0, x, y, in_scale);
+    } else {
+        sptrsv_rows<T, BS, UPPER, false>(row, active, rptr, cind, val, lc, lv, e0, x, y, in_scale);
+    }
+}
+
 // Any block size: one wave per block, rows in order, the wave splits each row's
 // entries and reduces them; in-block values come from LDS.  (Summation order differs
 // from the reference's, within rounding.)
@@ -373,7 +472,31 @@ int launch_sptrsv(Ctx *c, const Schedule &s, bool upper, const int *rptr, const 
         if (nb <= 0) continue;
         // an upper solve reads x of rows not yet solved only in its own row, so the
         // scale applies to the lower solve only
-        if (packed) {
+        if (packed && !s.dblk) {
+            // contiguous rows [b0*BS, (b0+nb)*BS): LDS-staged kernel
+            const int row0 = b0 * BS, nrows = nb * BS;
+            const int grid = (nrows + kBlock - 1) / kBlock;
+#define SENK_TRSV_LDS(B)                                                                                        \
+    case B:                                                                                                     \
+        if (upper)                                                                                              \
+            k_sptrsv_lds<T, B, true><<<grid, kBlock, 0, c->stream>>>(row0, nrows, rptr, cind, val, x, y, nullptr, \
+                                                                     stop_flag);                                \
+        else                                                                                                    \
+            k_sptrsv_lds<T, B, false><<<grid, kBlock, 0, c->stream>>>(row0, nrows, rptr, cind, val, x, y,         \
+                                                                      in_scale, stop_flag);                     \
+        break;
+            switch (BS) {
+                SENK_TRSV_LDS(1)
+                SENK_TRSV_LDS(2)
+                SENK_TRSV_LDS(4)
+                SENK_TRSV_LDS(8)
+                SENK_TRSV_LDS(16)
+                SENK_TRSV_LDS(32)
+                SENK_TRSV_LDS(64)
+                default: return SENK_ERR_UNSUPPORTED;
+            }
+#undef SENK_TRSV_LDS
+        } else if (packed) {
             const long long threads = (long long)nb * BS;
             const int grid = (int)((threads + kBlock - 1) / kBlock);
 #define SENK_TRSV_CASE(B)                                                                                        \
@@ -767,42 +890,44 @@ template int launch_convert<double, float>(Ctx *, int, const double *, float *, 
 template int launch_convert<float, double>(Ctx *, int, const float *, double *, const double *);
 template int launch_convert<double, double>(Ctx *, int, const double *, double *, const double *);
 
-// K simultaneous dot products against one vector, FP64 accumulation.
+// K simultaneous dot products against one vector, FP64 accumulation: a streaming
+// pass writes per-workgroup partials, then one workgroup per product adds them in a
+// fixed order.
 template <typename T, int KMAX>
 __global__ __launch_bounds__(kBlock) void k_multidot(int N, int K, const T *__restrict__ xs, size_t ldx,
-                                                     const T *__restrict__ y, double *__restrict__ result,
-                                                     double *partials, unsigned *counter) {
+                                                     const T *__restrict__ y, double *partials) {
     double acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0;
-    SENK_GRID_STRIDE(i, N) {
+    SENK_STREAM(i, N, {
         const double yi = (double)y[i];
-#pragma unroll
-        for (int k = 0; k < KMAX; k++)
-            if (k < K) acc[k] += (double)xs[(size_t)k * ldx + i] * yi;
-    }
-    __shared__ double tot[KMAX];
-    if (grid_reduce_n<KMAX>(acc, K, partials, counter, tot)) {
-        if (threadIdx.x < K) result[threadIdx.x] = tot[threadIdx.x];
-    }
+        _Pragma("unroll") for (int k = 0; k < KMAX; k++) if (k < K) acc[k] += (double)xs[(size_t)k * ldx + i] * yi;
+    })
+    store_partials_n<KMAX>(acc, K, partials);
+}
+__global__ __launch_bounds__(kBlock) void k_partials_fin(const double *partials, int B, double *result) {
+    const double s = sum_partials(partials + (size_t)blockIdx.x * kPartLd, B);
+    if (threadIdx.x == 0) result[blockIdx.x] = s;
 }
 template <typename T>
-int launch_multidot(Ctx *c, int N, int K, const T *xs, size_t ldx, const T *y, double *result, int slot) {
+int launch_multidot(Ctx *c, int N, int K, const T *xs, size_t ldx, const T *y, double *result) {
     if (K <= 0) return SENK_OK;
     if (K > kRedMaxK) return SENK_ERR_UNSUPPORTED;
-    const int grid = red_grid(N, 8);
-    unsigned *ctr = c->red_counters + slot;
+    const int grid = red_grid(N);
+    double *P = red_slot(c, 6);
     if (K <= 2)
-        k_multidot<T, 2><<<grid, kBlock, 0, c->stream>>>(N, K, xs, ldx, y, result, c->red_partials, ctr);
+        k_multidot<T, 2><<<grid, kBlock, 0, c->stream>>>(N, K, xs, ldx, y, P);
     else if (K <= 8)
-        k_multidot<T, 8><<<grid, kBlock, 0, c->stream>>>(N, K, xs, ldx, y, result, c->red_partials, ctr);
+        k_multidot<T, 8><<<grid, kBlock, 0, c->stream>>>(N, K, xs, ldx, y, P);
     else
-        k_multidot<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, K, xs, ldx, y, result, c->red_partials, ctr);
+        k_multidot<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, K, xs, ldx, y, P);
+    SENK_LAUNCH_CHECK();
+    k_partials_fin<<<K, kBlock, 0, c->stream>>>(P, grid, result);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
-template int launch_multidot<double>(Ctx *, int, int, const double *, size_t, const double *, double *, int);
-template int launch_multidot<float>(Ctx *, int, int, const float *, size_t, const float *, double *, int);
+template int launch_multidot<double>(Ctx *, int, int, const double *, size_t, const double *, double *);
+template int launch_multidot<float>(Ctx *, int, int, const float *, size_t, const float *, double *);
 
 __global__ __launch_bounds__(kBlock) void k_scaling(int N, const int *__restrict__ rptr, const int *__restrict__ cind,
                                                     double *__restrict__ val, const double *__restrict__ ld,

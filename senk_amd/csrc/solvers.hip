@@ -2,11 +2,14 @@
 // (senk_gcr.hpp), BiCGStab (senk_bicgstab.hpp) and FP32-inner / FP64-refinement
 // mixed precision.
 //
-// Vectors never leave the device.  Scalars (Hessenberg, Givens, alpha/beta/omega)
-// live in device memory and are produced by the last workgroup of the reduction that
-// needs them.  The stop test writes a device flag that every later kernel of the
-// iteration checks on entry; the host polls that flag through pinned memory a few
-// iterations behind the device, so the queue never drains while it waits.
+// Vectors never leave the device.  Scalars (Hessenberg, Givens, alpha/beta/omega) live
+// in device memory.  Every inner product is a streaming pass that leaves one partial
+// sum per workgroup; the kernel that needs the scalar adds those partials in a fixed
+// order in its prologue (or a one-workgroup finalize kernel does), so results are
+// deterministic and no kernel waits on another workgroup.  The stop test writes a
+// device flag that every later kernel of the iteration checks on entry; the host polls
+// that flag through pinned memory a few iterations behind the device, so the queue
+// never drains while it waits.
 //
 // GMRES keeps V_k unnormalised and stores sig_k = 1/||.||: every consumer multiplies
 // by sig_k on the fly (the preconditioner input, the Gram-Schmidt coefficients and the
@@ -35,6 +38,16 @@ template <typename T> const T *bvals(const MatView &A);
 template <> const double *bvals<double>(const MatView &A) { return A.bv64; }
 template <> const float *bvals<float>(const MatView &A) { return A.bv32; }
 
+// Partial-sum slots of the context (each kRedMaxK x kRedMaxBlocks doubles).
+enum { kSlotMgsA = 0, kSlotMgsB = 1, kSlotA = 2, kSlotB = 3, kSlotC = 4, kSlotRes = 5 };
+
+inline int ew_grid(long long n) {
+    long long b = (n + kBlock * 4LL - 1) / (kBlock * 4LL);
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (int)b;
+}
+
 // ----------------------------------------------------------------- small kernels
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scale_copy(int N, const T *__restrict__ in, T *__restrict__ out,
@@ -44,27 +57,14 @@ __global__ __launch_bounds__(kBlock) void k_scale_copy(int N, const T *__restric
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
         out[i] = dscale ? (T)((double)in[i] * sc) : in[i];
 }
-inline int ew_grid(long long n) {
-    long long b = (n + kBlock * 4LL - 1) / (kBlock * 4LL);
-    if (b < 1) b = 1;
-    if (b > 8192) b = 8192;
-    return (int)b;
-}
 
-__global__ void k_state_set(int *st, int a, int b, int c) {
-    st[0] = a;
-    st[1] = b;
-    st[2] = c;
-}
-
-// Residual y = b - A x (x == nullptr means x = 0) with ||y||^2 in FP64; the last
-// workgroup stores nrm2 = ||y||^2 and, optionally, inv = 1/||y||.
+// Residual y = b - A x (x == nullptr means x = 0), G lanes per row, with the
+// workgroup partials of ||y||^2.
 template <typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_residual(int N, const int *__restrict__ rptr,
                                                      const int *__restrict__ cind, const T *__restrict__ val,
                                                      const T *__restrict__ x, const T *__restrict__ b,
-                                                     T *__restrict__ y, double *partials, unsigned *counter,
-                                                     double *nrm2, double *inv) {
+                                                     T *__restrict__ y, double *partials) {
     const int rows_per_pass = (int)((long long)gridDim.x * kBlock / G);
     const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
     const int q = (int)(gid % G);
@@ -84,24 +84,30 @@ __global__ __launch_bounds__(kBlock) void k_residual(int N, const int *__restric
             ss += (double)r * (double)r;
         }
     }
-    double v[1] = {ss}, t[1];
-    if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) {
-        *nrm2 = t[0];
-        if (inv) *inv = 1.0 / sqrt(t[0]);
+    const double v[1] = {ss};
+    store_partials<1>(v, partials);
+}
+
+// nrm2 = sum of the partials, inv = 1/sqrt(nrm2)
+__global__ __launch_bounds__(kBlock) void k_norm_fin(const double *P, int B, double *nrm2, double *inv) {
+    const double s = sum_partials(P, B);
+    if (threadIdx.x == 0) {
+        *nrm2 = s;
+        if (inv) *inv = 1.0 / sqrt(s);
     }
 }
 
 template <typename T>
-int residual(Ctx *c, const MatView &A, const T *x, const T *b, T *y, double *nrm2, double *inv) {
+int residual(Ctx *c, const MatView &A, const T *x, const T *b, T *y, int *grid_out) {
     constexpr int G = 8;
     const T *v = mvals<T>(A);
     if (!v) return SENK_ERR_ARG;
     long long threads = (long long)A.N * G;
     int grid = (int)((threads + kBlock - 1) / kBlock);
     if (grid > kRedMaxBlocks) grid = kRedMaxBlocks;
-    k_residual<T, G><<<grid, kBlock, 0, c->stream>>>(A.N, A.rptr, A.cind, v, x, b, y, c->red_partials,
-                                                    c->red_counters + 1, nrm2, inv);
+    k_residual<T, G><<<grid, kBlock, 0, c->stream>>>(A.N, A.rptr, A.cind, v, x, b, y, red_slot(c, kSlotRes));
     SENK_LAUNCH_CHECK();
+    *grid_out = grid;
     return SENK_OK;
 }
 
@@ -165,59 +171,87 @@ size_t bytes_of(size_t n) {
     return ((sizeof(T) * n + 255) & ~(size_t)255) + 256;
 }
 
+struct SolveTimer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int start(Ctx *c) {
+        SENK_HIP(hipEventCreate(&e0));
+        SENK_HIP(hipEventCreate(&e1));
+        SENK_HIP(hipEventRecord(e0, c->stream));
+        return SENK_OK;
+    }
+    int stop(Ctx *c, float *ms) {
+        SENK_HIP(hipEventRecord(e1, c->stream));
+        SENK_HIP(hipEventSynchronize(e1));
+        SENK_HIP(hipEventElapsedTime(ms, e0, e1));
+        return SENK_OK;
+    }
+    ~SolveTimer() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 // ================================================================= GMRES kernels
-// One modified Gram-Schmidt step fused with the next inner product:
-//   w -= (H[k-1] * sig[k-1]) V_{k-1}      (skipped for k == 0)
-//   H[k] = sig[k] * (V_k, w)               (k <= j)
-//   H[j+1] = ||w||, sig[j+1] = 1/||w||     (k == j+1)
+// e[0] = ||r0||, sig[0] = 1/||r0||, state reset (cycle start).
+__global__ __launch_bounds__(kBlock) void k_gmres_start(const double *P, int B, double *e, double *sig, int *state) {
+    const double s = sum_partials(P, B);
+    if (threadIdx.x == 0) {
+        const double n = sqrt(s);
+        e[0] = n;
+        sig[0] = 1.0 / n;
+        state[0] = 0;
+        state[1] = 0;
+        state[2] = 0;
+    }
+}
+
+// One modified Gram-Schmidt step fused with the next inner product (k = 0 .. j+1):
+//   prologue (k > 0): H[k-1] = sig[k-1] * sum(Pprev);  w -= (H[k-1]*sig[k-1]) V_{k-1}
+//   then partials of (V_k, w) (k <= j) or of (w, w) (k == j+1) into Pout.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mgs(int N, const T *__restrict__ Vprev, const double *hprev,
-                                                const double *sprev, const T *__restrict__ Vk, const double *sk,
-                                                T *w, double *hout, double *sig_out, const int *stop,
-                                                double *partials, unsigned *counter) {
+__global__ __launch_bounds__(kBlock) void k_mgs(int N, int k, int j, const T *__restrict__ V, size_t ld,
+                                                const double *sig, double *hc, T *w, const double *Pprev, int B,
+                                                double *Pout, const int *stop) {
     if (*stop) return;
-    const T coef = Vprev ? (T)(*hprev * *sprev) : (T)0;
+    T coef = 0;
+    if (k > 0) {
+        const double h = sig[k - 1] * sum_partials(Pprev, B);
+        if (blockIdx.x == 0 && threadIdx.x == 0) hc[k - 1] = h;
+        coef = (T)(h * sig[k - 1]);
+    }
+    const T *Vprev = k > 0 ? V + (size_t)(k - 1) * ld : nullptr;
+    const T *Vk = k <= j ? V + (size_t)k * ld : nullptr;
     double acc = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+    SENK_STREAM(i, N, {
         T wi = w[i];
         if (Vprev) {
             wi = wi - coef * Vprev[i];
             w[i] = wi;
         }
         acc += Vk ? (double)Vk[i] * (double)wi : (double)wi * (double)wi;
-    }
-    double v[1] = {acc}, t[1];
-    if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) {
-        if (Vk) {
-            *hout = *sk * t[0];
-        } else {
-            const double h = sqrt(t[0]);
-            *hout = h;
-            *sig_out = 1.0 / h;
-        }
-    }
+    })
+    const double v[1] = {acc};
+    store_partials<1>(v, Pout);
 }
 
 // Classical Gram-Schmidt with re-orthogonalisation (CGS2), one pass over V_0..V_{K-1}
 // (K = j+1 <= kRedMaxK):
-//   coef_in != nullptr:  w -= sum_k coef_in[k]*sig[k] V_k  first
-//   mode 0: hout[k] = sig[k]*(V_k, w)
-//   mode 1: hout[k] += add_from[k] (k < K), hout[K] = ||w||, sig_out = 1/||w||
+//   coef != nullptr:  w -= sum_k coef[k]*sig[k] V_k  first
+//   mode 0: partials of (V_k, w), k < K;   mode 1: partials of (w, w)
 template <typename T, int KMAX>
 __global__ __launch_bounds__(kBlock) void k_cgs(int N, int K, const T *__restrict__ V, size_t ld,
-                                                const double *sig, const double *coef_in, T *w, double *hout,
-                                                int mode, const double *add_from, double *sig_out, const int *stop,
-                                                double *partials, unsigned *counter) {
+                                                const double *sig, const double *coef, T *w, int mode,
+                                                double *Pout, const int *stop) {
     if (*stop) return;
     __shared__ double cf[KMAX];
-    if (coef_in && threadIdx.x < K) cf[threadIdx.x] = coef_in[threadIdx.x] * sig[threadIdx.x];
+    if (coef && threadIdx.x < K) cf[threadIdx.x] = coef[threadIdx.x] * sig[threadIdx.x];
     __syncthreads();
     double acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0;
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
         T wi = w[i];
-        if (coef_in) {
+        if (coef) {
             double s = (double)wi;
 #pragma unroll
             for (int k = 0; k < KMAX; k++)
@@ -233,28 +267,33 @@ __global__ __launch_bounds__(kBlock) void k_cgs(int N, int K, const T *__restric
             acc[0] += (double)wi * (double)wi;
         }
     }
-    __shared__ double tot[KMAX];
-    const int nred = mode == 0 ? K : 1;
-    if (grid_reduce_n<KMAX>(acc, nred, partials, counter, tot)) {
-        if (mode == 0) {
-            if (threadIdx.x < K) hout[threadIdx.x] = sig[threadIdx.x] * tot[threadIdx.x];
-        } else {
-            if (add_from && threadIdx.x < K) hout[threadIdx.x] += add_from[threadIdx.x];
-            if (threadIdx.x == 0) {
-                const double h = sqrt(tot[0]);
-                hout[K] = h;
-                *sig_out = 1.0 / h;
-            }
-        }
-    }
+    store_partials_n<KMAX>(acc, mode == 0 ? K : 1, Pout);
+}
+// out[k] = sig[k] * sum(P_k), one workgroup per k
+__global__ __launch_bounds__(kBlock) void k_cgs_fin(const double *P, int B, const double *sig, double *out,
+                                                    const int *stop) {
+    if (*stop) return;
+    const int k = blockIdx.x;
+    const double s = sum_partials(P + (size_t)k * kPartLd, B);
+    if (threadIdx.x == 0) out[k] = sig[k] * s;
 }
 
-// Givens update of Hessenberg column j and the stop test (senk_gmres.hpp:230-246),
-// FP64, one thread.
-__global__ void k_gmres_tail(int j, int m, double *H, double *cs, double *sn, double *e, double tol, double nrm_b,
-                             int *state, double *hist, int hidx, int hcap) {
-    if (threadIdx.x != 0 || state[0]) return;
+// Givens update of Hessenberg column j and the stop test (senk_gmres.hpp:228-246),
+// FP64.  H[j+1,j] = ||w|| comes from the partials of the last Gram-Schmidt pass;
+// with CGS2 the second projection's coefficients are added to the column first.
+__global__ __launch_bounds__(kBlock) void k_gmres_tail(int j, int m, double *H, double *cs, double *sn, double *e,
+                                                       double *sig, const double *Pn, int B, const double *add_h,
+                                                       double tol, double nrm_b, int *state, double *hist, int hidx,
+                                                       int hcap) {
+    if (state[0]) return;
+    const double nn = sum_partials(Pn, B);
+    if (threadIdx.x != 0) return;
     double *hc = H + (size_t)j * (m + 1);
+    if (add_h)
+        for (int k = 0; k <= j; k++) hc[k] += add_h[k];
+    const double hn = sqrt(nn);
+    hc[j + 1] = hn;
+    sig[j + 1] = 1.0 / hn;
     for (int k = 0; k < j; k++) {
         const double a = hc[k], b = hc[k + 1];
         hc[k] = cs[k] * a - sn[k] * b;
@@ -310,16 +349,6 @@ __global__ __launch_bounds__(kBlock) void k_combine(int N, const T *__restrict__
         }
         out[i] = (T)acc;
     }
-}
-
-__global__ void k_gmres_start(const double *nrm2, double *e, double *sig, int *state) {
-    if (threadIdx.x != 0) return;
-    const double n = sqrt(*nrm2);
-    e[0] = n;
-    sig[0] = 1.0 / n;
-    state[0] = 0;
-    state[1] = 0;
-    state[2] = 0;
 }
 
 }  // namespace
@@ -379,6 +408,7 @@ int prec_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T
         if (!lv || !uv) return SENK_ERR_ARG;
         const T *src = in;
         if (in_scale) {
+            if (!tmp) return SENK_ERR_ARG;
             k_scale_copy<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, tmp, in_scale, stop);
             SENK_LAUNCH_CHECK();
             src = tmp;
@@ -405,7 +435,7 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
     const int hcap = p.outer * m;
     size_t need = bytes_of<T>((size_t)(m + 1) * N) + (flexible ? bytes_of<T>((size_t)m * N) : 0) +
                   3 * bytes_of<T>(N) + bytes_of<double>((size_t)(m + 1) * m) + 7 * bytes_of<double>(m + 2) +
-                  bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(8);
+                  bytes_of<double>(hcap + 1) + bytes_of<int>(8);
     void *wsp;
     SENK_TRY(ctx_work(c, need, &wsp));
     Carver cv{(char *)wsp, 0, need};
@@ -418,24 +448,23 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
     double *h2 = cv.take<double>(m + 2);
     double *dhist = cv.take<double>(hcap + 1);
     int *state = cv.take<int>(8);
-    double *dsc = cv.take<double>(8);
 
     Poller poll;
     SENK_TRY(poll.init(c, state));
-    hipEvent_t e0, e1;
-    SENK_HIP(hipEventCreate(&e0));
-    SENK_HIP(hipEventCreate(&e1));
-    SENK_HIP(hipEventRecord(e0, c->stream));
+    SolveTimer tm;
+    SENK_TRY(tm.start(c));
 
     const double tol = p.nrm_b * p.eps;
     const bool prec = M.kind != 0 || M.learned;
     const bool use_cgs = p.ortho == SENK_CGS2;
+    const int B = red_grid(N);
     int total = p.outer * m, converged = 0, breakdown = 0, cycles = 0;
     int hstate[4] = {0, 0, 0, 0};
     for (int i = 0; i < p.outer; i++) {
         cycles++;
-        SENK_TRY(residual<T>(c, A, x, b, V, dsc, nullptr));
-        k_gmres_start<<<1, 64, 0, c->stream>>>(dsc, e, sig, state);
+        int rg = 0;
+        SENK_TRY(residual<T>(c, A, x, b, V, &rg));
+        k_gmres_start<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, e, sig, state);
         SENK_LAUNCH_CHECK();
         poll.reset();
         bool stop = false;
@@ -446,32 +475,37 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
             SENK_TRY(prec_apply<T>(c, M, vj, sig + j, z, tmp, state));
             SENK_TRY(spmv<T>(c, A, z, w, state));
             double *hc = H + (size_t)j * (m + 1);
+            const double *Pnorm;
+            const double *add_h = nullptr;
             if (use_cgs && j + 1 <= kRedMaxK) {
-                const int K = j + 1, grid = red_grid(N, 8);
-                // h1 = V^T w;  w -= V h1, h2 = V^T w;  w -= V h2, H[:,j] = h1 + h2, ||w||
-                k_cgs<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, K, V, N, sig, nullptr, w, hc, 0, nullptr,
-                                                                   nullptr, state, c->red_partials,
-                                                                   c->red_counters + 2);
+                const int K = j + 1;
+                // h1 = V^T w;  w -= V h1, h2 = V^T w;  w -= V h2, ||w||;  H[:,j] = h1 + h2
+                k_cgs<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, K, V, N, sig, nullptr, w, 0, red_slot(c, kSlotA),
+                                                                state);
                 SENK_LAUNCH_CHECK();
-                k_cgs<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, K, V, N, sig, hc, w, h2, 0, nullptr, nullptr,
-                                                                   state, c->red_partials, c->red_counters + 2);
+                k_cgs_fin<<<K, kBlock, 0, c->stream>>>(red_slot(c, kSlotA), B, sig, hc, state);
                 SENK_LAUNCH_CHECK();
-                k_cgs<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, K, V, N, sig, h2, w, hc, 1, h2, sig + j + 1,
-                                                                   state, c->red_partials, c->red_counters + 2);
+                k_cgs<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, K, V, N, sig, hc, w, 0, red_slot(c, kSlotB),
+                                                                state);
                 SENK_LAUNCH_CHECK();
+                k_cgs_fin<<<K, kBlock, 0, c->stream>>>(red_slot(c, kSlotB), B, sig, h2, state);
+                SENK_LAUNCH_CHECK();
+                k_cgs<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, K, V, N, sig, h2, w, 1, red_slot(c, kSlotC),
+                                                                state);
+                SENK_LAUNCH_CHECK();
+                Pnorm = red_slot(c, kSlotC);
+                add_h = h2;
             } else {
-                const int grid = red_grid(N, 8);
                 for (int k = 0; k <= j + 1; k++) {
-                    const T *Vprev = k > 0 ? V + (size_t)(k - 1) * N : nullptr;
-                    const T *Vk = k <= j ? V + (size_t)k * N : nullptr;
-                    k_mgs<T><<<grid, kBlock, 0, c->stream>>>(N, Vprev, k > 0 ? hc + k - 1 : nullptr,
-                                                             k > 0 ? sig + k - 1 : nullptr, Vk, sig + k, w, hc + k,
-                                                             sig + j + 1, state, c->red_partials,
-                                                             c->red_counters + 2);
+                    const double *Pin = red_slot(c, (k + 1) % 2 ? kSlotMgsB : kSlotMgsA);
+                    double *Pout = red_slot(c, k % 2 ? kSlotMgsB : kSlotMgsA);
+                    k_mgs<T><<<B, kBlock, 0, c->stream>>>(N, k, j, V, N, sig, hc, w, Pin, B, Pout, state);
                     SENK_LAUNCH_CHECK();
                 }
+                Pnorm = red_slot(c, (j + 1) % 2 ? kSlotMgsB : kSlotMgsA);
             }
-            k_gmres_tail<<<1, 64, 0, c->stream>>>(j, m, H, cs, sn, e, tol, p.nrm_b, state, dhist, i * m + j, hcap);
+            k_gmres_tail<<<1, kBlock, 0, c->stream>>>(j, m, H, cs, sn, e, sig, Pnorm, B, add_h, tol, p.nrm_b, state,
+                                                      dhist, i * m + j, hcap);
             SENK_LAUNCH_CHECK();
             SENK_TRY(poll.mark());
             SENK_TRY(poll.poll(kLag, &stop));
@@ -502,16 +536,13 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
             break;
         }
     }
-    SENK_HIP(hipEventRecord(e1, c->stream));
-    SENK_HIP(hipEventSynchronize(e1));
     float ms = 0;
-    SENK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    SENK_TRY(tm.stop(c, &ms));
     const int nh = std::min(hcap, converged ? total : cycles * m);
     std::vector<double> hh((size_t)std::max(nh, 1));
     if (nh > 0) SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * nh, hipMemcpyDeviceToHost));
-    if (hist) for (int k = 0; k < std::min(nh, hist_cap); k++) hist[k] = hh[k];
+    if (hist)
+        for (int k = 0; k < std::min(nh, hist_cap); k++) hist[k] = hh[k];
     res->iters = converged ? total : p.outer * m;
     res->converged = converged;
     res->restarts = cycles;
@@ -533,54 +564,58 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
 
 // ================================================================= GCR / FGCR kernels
 namespace {
-// dot_Ap[j] = (Ap_j, Ap_j); alpha = (Ap_j, r) / dot_Ap[j]     (senk_gcr.hpp:119-120)
+// partials of (Ap_j, Ap_j) and (Ap_j, r)                        (senk_gcr.hpp:119-120)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gcr_alpha(int N, const T *__restrict__ Apj, const T *__restrict__ r,
-                                                      double *dotAp, double *alpha, const int *stop,
-                                                      double *partials, unsigned *counter) {
+                                                      double *Pout, const int *stop) {
     if (*stop) return;
     double a0 = 0, a1 = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+    SENK_STREAM(i, N, {
         const double ap = (double)Apj[i];
         a0 += ap * ap;
         a1 += ap * (double)r[i];
-    }
-    double v[2] = {a0, a1}, t[2];
-    if (grid_reduce<2>(v, partials, counter, t) && threadIdx.x == 0) {
-        *dotAp = t[0];
-        *alpha = t[1] / t[0];
-    }
+    })
+    const double v[2] = {a0, a1};
+    store_partials<2>(v, Pout);
 }
-// x += alpha p_j; r -= alpha Ap_j; optional ||r|| stop test (check_every).
+// alpha = (Ap_j, r)/(Ap_j, Ap_j); x += alpha p_j; r -= alpha Ap_j; partials of ||r||^2.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gcr_xr(int N, const T *__restrict__ pj, const T *__restrict__ Apj,
-                                                   const double *alpha, T *x, T *r, int check, double tol,
-                                                   double nrm_b, int hidx, int jstep, int *state, double *hist,
-                                                   int hcap, double *partials, unsigned *counter) {
-    if (state[0]) return;
-    const T a = (T)*alpha;
+                                                   const double *Pin, int B, double *dotApj, T *x, T *r, double *Pout,
+                                                   const int *stop) {
+    if (*stop) return;
+    const double d0 = sum_partials(Pin, B), d1 = sum_partials(Pin + kPartLd, B);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dotApj = d0;
+    const T a = (T)(d1 / d0);
     double ss = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+    SENK_STREAM(i, N, {
         x[i] += a * pj[i];
         const T ri = r[i] + (-a) * Apj[i];
         r[i] = ri;
         ss += (double)ri * (double)ri;
-    }
-    if (!check) return;
-    double v[1] = {ss}, t[1];
-    if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) {
-        const double nr = sqrt(t[0]);
-        if (hist && hidx < hcap) hist[hidx] = nr / nrm_b;
-        state[1] = jstep + 1;
-        if (!isfinite(nr)) { state[2] = 1; state[0] = 1; }
-        else if (nr < tol) state[0] = 1;
+    })
+    const double v[1] = {ss};
+    store_partials<1>(v, Pout);
+}
+// per-step stop test on ||r|| (check_every mode)
+__global__ __launch_bounds__(kBlock) void k_gcr_check(const double *Pin, int B, double tol, double nrm_b, int hidx,
+                                                      int jstep, int *state, double *hist, int hcap) {
+    if (state[0]) return;
+    const double nr = sqrt(sum_partials(Pin, B));
+    if (threadIdx.x != 0) return;
+    if (hist && hidx < hcap) hist[hidx] = nr / nrm_b;
+    state[1] = jstep + 1;
+    if (!isfinite(nr)) {
+        state[2] = 1;
+        state[0] = 1;
+    } else if (nr < tol) {
+        state[0] = 1;
     }
 }
-// beta_k = -(Ap_k, AKr) / dot_Ap[k], k <= j                  (senk_gcr.hpp:126-133)
+// partials of (Ap_k, AKr), k <= j                                 (senk_gcr.hpp:126-133)
 template <typename T, int KMAX>
 __global__ __launch_bounds__(kBlock) void k_gcr_betas(int N, int K, const T *__restrict__ Ap, size_t ld,
-                                                      const T *__restrict__ AKr, const double *dotAp, double *beta,
-                                                      const int *stop, double *partials, unsigned *counter) {
+                                                      const T *__restrict__ AKr, double *Pout, const int *stop) {
     if (*stop) return;
     double acc[KMAX];
 #pragma unroll
@@ -591,10 +626,15 @@ __global__ __launch_bounds__(kBlock) void k_gcr_betas(int N, int K, const T *__r
         for (int k = 0; k < KMAX; k++)
             if (k < K) acc[k] += (double)Ap[(size_t)k * ld + i] * a;
     }
-    __shared__ double tot[KMAX];
-    if (grid_reduce_n<KMAX>(acc, K, partials, counter, tot)) {
-        if (threadIdx.x < K) beta[threadIdx.x] = -tot[threadIdx.x] / dotAp[threadIdx.x];
-    }
+    store_partials_n<KMAX>(acc, K, Pout);
+}
+// beta_k = -(Ap_k, AKr) / dot_Ap[k], one workgroup per k
+__global__ __launch_bounds__(kBlock) void k_gcr_beta_fin(const double *P, int B, const double *dotAp, double *beta,
+                                                         const int *stop) {
+    if (*stop) return;
+    const int k = blockIdx.x;
+    const double s = sum_partials(P + (size_t)k * kPartLd, B);
+    if (threadIdx.x == 0) beta[k] = -s / dotAp[k];
 }
 // p_{j+1} = beta_0 p_0 + Kr + sum_{k>=1} beta_k p_k, and the same for Ap  (:127-133)
 template <typename T>
@@ -612,17 +652,20 @@ __global__ __launch_bounds__(kBlock) void k_gcr_newdir(int N, int K, T *p, T *Ap
         Ap[(size_t)K * ld + i] = b;
     }
 }
-__global__ void k_gcr_restart(const double *nrm2, double tol, double nrm_b, int *state, double *hist, int hidx,
-                              int hcap, double *nr_out) {
+__global__ __launch_bounds__(kBlock) void k_gcr_restart(const double *P, int B, double tol, double nrm_b, int *state,
+                                                        double *hist, int hidx, int hcap) {
+    const double nr = sqrt(sum_partials(P, B));
     if (threadIdx.x != 0) return;
-    const double nr = sqrt(*nrm2);
-    *nr_out = nr;
     if (hist && hidx < hcap) hist[hidx] = nr / nrm_b;
     state[0] = 0;
     state[1] = 0;
     state[2] = 0;
-    if (!isfinite(nr)) { state[2] = 1; state[0] = 1; }
-    else if (nr < tol) state[0] = 1;
+    if (!isfinite(nr)) {
+        state[2] = 1;
+        state[0] = 1;
+    } else if (nr < tol) {
+        state[0] = 1;
+    }
 }
 }  // namespace
 
@@ -633,32 +676,29 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
     if (m > kRedMaxK) return SENK_ERR_UNSUPPORTED;
     const bool prec = M.kind != 0 || M.learned;
     const int hcap = p.check_every ? p.outer * (m + 1) + 1 : p.outer + 1;
-    size_t need = 2 * bytes_of<T>((size_t)(m + 1) * N) + 4 * bytes_of<T>(N) + 4 * bytes_of<double>(m + 2) +
-                  bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(8);
+    size_t need = 2 * bytes_of<T>((size_t)(m + 1) * N) + 4 * bytes_of<T>(N) + 3 * bytes_of<double>(m + 2) +
+                  bytes_of<double>(hcap + 1) + bytes_of<int>(8);
     void *wsp;
     SENK_TRY(ctx_work(c, need, &wsp));
     Carver cv{(char *)wsp, 0, need};
     T *P = cv.take<T>((size_t)(m + 1) * N), *AP = cv.take<T>((size_t)(m + 1) * N);
     T *r = cv.take<T>(N), *Kr = cv.take<T>(N), *AKr = cv.take<T>(N), *tmp = cv.take<T>(N);
-    double *dotAp = cv.take<double>(m + 2), *beta = cv.take<double>(m + 2), *alpha = cv.take<double>(m + 2);
-    double *nrv = cv.take<double>(m + 2);
+    double *dotAp = cv.take<double>(m + 2), *beta = cv.take<double>(m + 2);
     double *dhist = cv.take<double>(hcap + 1);
     int *state = cv.take<int>(8);
-    double *dsc = cv.take<double>(8);
     Poller poll;
     SENK_TRY(poll.init(c, state));
-    hipEvent_t e0, e1;
-    SENK_HIP(hipEventCreate(&e0));
-    SENK_HIP(hipEventCreate(&e1));
-    SENK_HIP(hipEventRecord(e0, c->stream));
+    SolveTimer tm;
+    SENK_TRY(tm.start(c));
     const double tol = p.nrm_b * p.eps;
     int converged = 0, breakdown = 0, last = 0, cycles = 0, nh = 0, hstate[4] = {0, 0, 0, 0};
-    const int grid = red_grid(N, 8);
+    const int B = red_grid(N);
     for (int i = 0; i < p.outer; i++) {
         cycles++;
-        SENK_TRY(residual<T>(c, A, x, b, r, dsc, nullptr));
+        int rg = 0;
+        SENK_TRY(residual<T>(c, A, x, b, r, &rg));
         const int hidx = p.check_every ? i * (m + 1) : i;
-        k_gcr_restart<<<1, 64, 0, c->stream>>>(dsc, tol, p.nrm_b, state, dhist, hidx, hcap, nrv);
+        k_gcr_restart<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, tol, p.nrm_b, state, dhist, hidx, hcap);
         SENK_LAUNCH_CHECK();
         SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
         SENK_HIP(hipStreamSynchronize(c->stream));
@@ -677,21 +717,25 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
         bool stop = false;
         for (int j = 0; j < m && !stop; j++) {
             T *pj = P + (size_t)j * N, *Apj = AP + (size_t)j * N;
-            k_gcr_alpha<T><<<grid, kBlock, 0, c->stream>>>(N, Apj, r, dotAp + j, alpha, state, c->red_partials,
-                                                           c->red_counters + 4);
+            k_gcr_alpha<T><<<B, kBlock, 0, c->stream>>>(N, Apj, r, red_slot(c, kSlotA), state);
             SENK_LAUNCH_CHECK();
-            k_gcr_xr<T><<<grid, kBlock, 0, c->stream>>>(N, pj, Apj, alpha, x, r, p.check_every, tol, p.nrm_b,
-                                                        i * (m + 1) + 1 + j, j, state, dhist, hcap,
-                                                        c->red_partials, c->red_counters + 5);
+            k_gcr_xr<T><<<B, kBlock, 0, c->stream>>>(N, pj, Apj, red_slot(c, kSlotA), B, dotAp + j, x, r,
+                                                     red_slot(c, kSlotC), state);
             SENK_LAUNCH_CHECK();
+            if (p.check_every) {
+                k_gcr_check<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotC), B, tol, p.nrm_b, i * (m + 1) + 1 + j,
+                                                         j, state, dhist, hcap);
+                SENK_LAUNCH_CHECK();
+            }
             const T *z = r;
             if (prec) {
                 SENK_TRY(prec_apply<T>(c, M, r, nullptr, Kr, tmp, state));
                 z = Kr;
             }
             SENK_TRY(spmv<T>(c, A, z, AKr, state));
-            k_gcr_betas<T, kRedMaxK><<<grid, kBlock, 0, c->stream>>>(N, j + 1, AP, N, AKr, dotAp, beta, state,
-                                                                     c->red_partials, c->red_counters + 6);
+            k_gcr_betas<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, j + 1, AP, N, AKr, red_slot(c, kSlotB), state);
+            SENK_LAUNCH_CHECK();
+            k_gcr_beta_fin<<<j + 1, kBlock, 0, c->stream>>>(red_slot(c, kSlotB), B, dotAp, beta, state);
             SENK_LAUNCH_CHECK();
             k_gcr_newdir<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, j + 1, P, AP, N, z, AKr, beta, state);
             SENK_LAUNCH_CHECK();
@@ -712,28 +756,19 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
             }
         }
     }
-    SENK_HIP(hipEventRecord(e1, c->stream));
-    SENK_HIP(hipEventSynchronize(e1));
     float ms = 0;
-    SENK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    SENK_TRY(tm.stop(c, &ms));
     std::vector<double> hh((size_t)hcap + 1, 0.0);
     SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * hcap, hipMemcpyDeviceToHost));
-    // history: per-restart values (reference) or every step when check_every
-    int out_n = 0;
-    if (!p.check_every) {
-        out_n = std::min(nh, hcap);
-        if (hist) for (int k = 0; k < std::min(out_n, hist_cap); k++) hist[k] = hh[k];
-    } else {
-        // restart value, then one value per step, cycle after cycle
-        out_n = std::min(nh, hcap);
-        if (hist) for (int k = 0; k < std::min(out_n, hist_cap); k++) hist[k] = hh[k];
-    }
+    // history: one value per restart (reference), or restart value then one value per
+    // step, cycle after cycle (check_every)
+    const int out_n = std::min(nh, hcap);
+    if (hist)
+        for (int k = 0; k < std::min(out_n, hist_cap); k++) hist[k] = hh[k];
     res->iters = last;
     res->converged = converged;
     res->restarts = cycles;
-    res->implicit_relres = out_n > 0 ? (p.check_every ? hh[std::max(0, nh - 1)] : hh[out_n - 1]) : 0;
+    res->implicit_relres = out_n > 0 ? hh[out_n - 1] : 0;
     res->t_solve_ms = ms;
     res->status = breakdown ? SENK_ERR_BREAKDOWN : SENK_OK;
     if (p.verbose) {
@@ -747,57 +782,57 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
 }
 
 // ================================================================= BiCGStab kernels
+// Scalars in sc[]: [0] alpha, [1] omega, [2]/[3] rr (double-buffered by iteration
+// parity so that a kernel never overwrites a value other workgroups still read).
 namespace {
-// alpha = rr / (AKp, rstr)                                   (senk_bicgstab.hpp:140)
+// partials of (AKp, rstr)                                       (senk_bicgstab.hpp:140)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_alpha(int N, const T *__restrict__ AKp, const T *__restrict__ rs,
-                                                       const double *rr, double *alpha, const int *stop,
-                                                       double *partials, unsigned *counter) {
+                                                       double *Pout, const int *stop) {
     if (*stop) return;
     double a = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
-        a += (double)AKp[i] * (double)rs[i];
-    double v[1] = {a}, t[1];
-    if (grid_reduce<1>(v, partials, counter, t) && threadIdx.x == 0) *alpha = *rr / t[0];
+    SENK_STREAM(i, N, { a += (double)AKp[i] * (double)rs[i]; })
+    const double v[1] = {a};
+    store_partials<1>(v, Pout);
 }
-// s = -alpha AKp + r                                          (:141)
+// alpha = rr / (AKp, rstr);  s = -alpha AKp + r                    (:140-141)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_s(int N, const T *__restrict__ AKp, const T *__restrict__ r,
-                                                   const double *alpha, T *__restrict__ s, const int *stop) {
+                                                   const double *Pin, int B, const double *rr, double *alpha_out,
+                                                   T *__restrict__ s, const int *stop) {
     if (*stop) return;
-    const T a = (T)(-*alpha);
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
-        s[i] = a * AKp[i] + r[i];
+    const double alpha = *rr / sum_partials(Pin, B);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = alpha;
+    const T a = (T)(-alpha);
+    SENK_STREAM(i, N, { s[i] = a * AKp[i] + r[i]; })
 }
-// omega = (AKs, s) / (AKs, AKs)                               (:145)
+// partials of (AKs, s) and (AKs, AKs)                            (:145)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_omega(int N, const T *__restrict__ AKs, const T *__restrict__ s,
-                                                       double *omega, const int *stop, double *partials,
-                                                       unsigned *counter) {
+                                                       double *Pout, const int *stop) {
     if (*stop) return;
     double a0 = 0, a1 = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+    SENK_STREAM(i, N, {
         const double v = (double)AKs[i];
         a0 += v * (double)s[i];
         a1 += v * v;
-    }
-    double v[2] = {a0, a1}, t[2];
-    if (grid_reduce<2>(v, partials, counter, t) && threadIdx.x == 0) *omega = t[0] / t[1];
+    })
+    const double v[2] = {a0, a1};
+    store_partials<2>(v, Pout);
 }
-// x += alpha Kp; x += omega Ks; r = -omega AKs + s; ||r||, (r, rstr), beta, stop
-// (:146-159)
+// omega = (AKs,s)/(AKs,AKs); x += alpha Kp; x += omega Ks; r = -omega AKs + s;
+// partials of ||r||^2 and (r, rstr)                              (:145-158)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_update(int N, const T *__restrict__ Kp, const T *__restrict__ Ks,
                                                         const T *__restrict__ AKs, const T *__restrict__ s,
                                                         const T *__restrict__ rs, T *x, T *r, double *sc,
-                                                        double tol, double nrm_b, int iter, int *state,
-                                                        double *hist, int hcap, double *partials,
-                                                        unsigned *counter) {
-    if (state[0]) return;
-    // sc: [0] alpha, [1] omega, [2] rr, [3] beta
-    const T a = (T)sc[0], w = (T)sc[1];
+                                                        const double *Pin, int B, double *Pout, const int *stop) {
+    if (*stop) return;
+    const double omega = sum_partials(Pin, B) / sum_partials(Pin + kPartLd, B);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[1] = omega;
+    const T a = (T)sc[0], w = (T)omega;
     double ss = 0, rr = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+    SENK_STREAM(i, N, {
         T xi = x[i] + a * Kp[i];
         xi = xi + w * Ks[i];
         x[i] = xi;
@@ -805,38 +840,53 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(int N, const T *__restri
         r[i] = ri;
         ss += (double)ri * (double)ri;
         rr += (double)ri * (double)rs[i];
-    }
-    double v[2] = {ss, rr}, t[2];
-    if (grid_reduce<2>(v, partials, counter, t) && threadIdx.x == 0) {
-        const double nr = sqrt(t[0]);
-        if (hist && iter < hcap) hist[iter] = nr / nrm_b;
-        state[1] = iter + 1;
-        if (!isfinite(nr)) {
-            state[2] = 1;
-            state[0] = 1;
-        } else if (nr < tol) {
-            state[0] = 1;
-        } else {
-            const double prev = sc[2];
-            sc[2] = t[1];
-            sc[3] = sc[0] / sc[1] * t[1] / prev;
-        }
-    }
+    })
+    const double v[2] = {ss, rr};
+    store_partials<2>(v, Pout);
 }
-// p = beta (-omega AKp + p) + r                                (:160-161)
+// stop test on ||r||; beta = alpha/omega * rr_new/rr_old; p = beta(-omega AKp + p) + r
+// (:149-161)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_p(int N, const T *__restrict__ AKp, const T *__restrict__ r,
-                                                   const double *sc, T *p, const int *stop) {
-    if (*stop) return;
-    const T w = (T)(-sc[1]), be = (T)sc[3];
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+                                                   double *sc, int parity, const double *Pin, int B, double tol,
+                                                   double nrm_b, int iter, int *state, double *hist, int hcap,
+                                                   T *p) {
+    if (state[0]) return;
+    const double nr = sqrt(sum_partials(Pin, B));
+    const double rr_new = sum_partials(Pin + kPartLd, B);
+    const bool fin = !isfinite(nr), conv = nr < tol;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (hist && iter < hcap) hist[iter] = nr / nrm_b;
+        state[1] = iter + 1;
+        if (fin) {
+            state[2] = 1;
+            state[0] = 1;
+        } else if (conv) {
+            state[0] = 1;
+        } else {
+            sc[2 + (parity ^ 1)] = rr_new;
+        }
+    }
+    if (fin || conv) return;
+    const double rr_old = sc[2 + parity];
+    const double beta = sc[0] / sc[1] * rr_new / rr_old;
+    const T w = (T)(-sc[1]), be = (T)beta;
+    SENK_STREAM(i, N, {
         const T tmp = w * AKp[i] + p[i];
         p[i] = be * tmp + r[i];
+    })
+}
+__global__ void k_state_reset(int *state) {
+    if (threadIdx.x == 0) {
+        state[0] = 0;
+        state[1] = 0;
+        state[2] = 0;
     }
 }
-__global__ void k_bicg_init(const double *rr_src, double *sc, int *state) {
+__global__ __launch_bounds__(kBlock) void k_bicg_init(const double *P, int B, double *sc, int *state) {
+    const double rr = sum_partials(P, B);
     if (threadIdx.x != 0) return;
-    sc[2] = *rr_src;
+    sc[2] = rr;
     state[0] = 0;
     state[1] = 0;
     state[2] = 0;
@@ -859,55 +909,57 @@ static int bicgstab(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *
     double *sc = cv.take<double>(16);
     Poller poll;
     SENK_TRY(poll.init(c, state));
-    hipEvent_t e0, e1;
-    SENK_HIP(hipEventCreate(&e0));
-    SENK_HIP(hipEventCreate(&e1));
-    SENK_HIP(hipEventRecord(e0, c->stream));
+    SolveTimer tm;
+    SENK_TRY(tm.start(c));
     const double tol = p.eps * p.nrm_b;
-    const int grid = red_grid(N, 8);
+    const int B = red_grid(N);
     // r = b - A x; rstr = r; p = r; rr = (r, rstr)
-    SENK_TRY(residual<T>(c, A, x, b, r, sc + 4, nullptr));
+    int rg = 0;
+    SENK_TRY(residual<T>(c, A, x, b, r, &rg));
     SENK_TRY(launch_copy<T>(c, N, r, rs));
     SENK_TRY(launch_copy<T>(c, N, r, pp));
-    SENK_TRY(launch_multidot<T>(c, N, 1, r, N, rs, sc + 5, 7));
-    k_bicg_init<<<1, 64, 0, c->stream>>>(sc + 5, sc, state);
+    // the state words come from the shared arena: clear the stop flag before the first
+    // flag-checking kernel
+    k_state_reset<<<1, 64, 0, c->stream>>>(state);
+    SENK_LAUNCH_CHECK();
+    k_bicg_alpha<T><<<B, kBlock, 0, c->stream>>>(N, r, rs, red_slot(c, kSlotA), state);
+    SENK_LAUNCH_CHECK();
+    k_bicg_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotA), B, sc, state);
     SENK_LAUNCH_CHECK();
     poll.reset();
     bool stop = false;
     for (int i = 0; i < p.max_iter && !stop; i++) {
+        const int par = i & 1;
         SENK_TRY(prec_apply<T>(c, M, pp, nullptr, Kp, tmp, state));
         SENK_TRY(spmv<T>(c, A, Kp, AKp, state));
-        k_bicg_alpha<T><<<grid, kBlock, 0, c->stream>>>(N, AKp, rs, sc + 2, sc + 0, state, c->red_partials,
-                                                        c->red_counters + 8);
+        k_bicg_alpha<T><<<B, kBlock, 0, c->stream>>>(N, AKp, rs, red_slot(c, kSlotA), state);
         SENK_LAUNCH_CHECK();
-        k_bicg_s<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, AKp, r, sc + 0, s, state);
+        k_bicg_s<T><<<B, kBlock, 0, c->stream>>>(N, AKp, r, red_slot(c, kSlotA), B, sc + 2 + par, sc + 0, s, state);
         SENK_LAUNCH_CHECK();
         SENK_TRY(prec_apply<T>(c, M, s, nullptr, Ks, tmp, state));
         SENK_TRY(spmv<T>(c, A, Ks, AKs, state));
-        k_bicg_omega<T><<<grid, kBlock, 0, c->stream>>>(N, AKs, s, sc + 1, state, c->red_partials,
-                                                        c->red_counters + 9);
+        k_bicg_omega<T><<<B, kBlock, 0, c->stream>>>(N, AKs, s, red_slot(c, kSlotB), state);
         SENK_LAUNCH_CHECK();
-        k_bicg_update<T><<<grid, kBlock, 0, c->stream>>>(N, Kp, Ks, AKs, s, rs, x, r, sc, tol, p.nrm_b, i, state,
-                                                         dhist, hcap, c->red_partials, c->red_counters + 10);
+        k_bicg_update<T><<<B, kBlock, 0, c->stream>>>(N, Kp, Ks, AKs, s, rs, x, r, sc, red_slot(c, kSlotB), B,
+                                                      red_slot(c, kSlotC), state);
         SENK_LAUNCH_CHECK();
-        k_bicg_p<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, AKp, r, sc, pp, state);
+        k_bicg_p<T><<<B, kBlock, 0, c->stream>>>(N, AKp, r, sc, par, red_slot(c, kSlotC), B, tol, p.nrm_b, i, state,
+                                                 dhist, hcap, pp);
         SENK_LAUNCH_CHECK();
         SENK_TRY(poll.mark());
         SENK_TRY(poll.poll(kLag, &stop));
     }
     int hstate[4];
     SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
-    SENK_HIP(hipEventRecord(e1, c->stream));
-    SENK_HIP(hipEventSynchronize(e1));
     float ms = 0;
-    SENK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    SENK_TRY(tm.stop(c, &ms));
     const int converged = hstate[0] && !hstate[2];
     const int iters = hstate[0] ? hstate[1] : p.max_iter;
     std::vector<double> hh((size_t)std::max(iters, 1), 0.0);
-    if (iters > 0) SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * std::min(iters, hcap), hipMemcpyDeviceToHost));
-    if (hist) for (int k = 0; k < std::min(iters, hist_cap); k++) hist[k] = hh[k];
+    if (iters > 0)
+        SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * std::min(iters, hcap), hipMemcpyDeviceToHost));
+    if (hist)
+        for (int k = 0; k < std::min(iters, hist_cap); k++) hist[k] = hh[k];
     res->iters = iters;
     res->converged = converged;
     res->restarts = 0;
@@ -949,7 +1001,7 @@ template int krylov_solve<float>(Ctx *, const MatView &, const PrecView &, const
 
 // ================================================================= mixed precision
 namespace {
-// x += scale * d  (FP64 x, FP32 d, scale = ||r|| on the device)
+// x += ||r|| * d  (FP64 x, FP32 d; ||r||^2 on the device)
 __global__ __launch_bounds__(kBlock) void k_ir_update(int N, const float *__restrict__ d, const double *nrm2,
                                                       double *x) {
     const double s = sqrt(*nrm2);
@@ -964,7 +1016,10 @@ int true_relres(Ctx *c, const MatView &A, const double *b, const double *x, doub
     Carver cv{(char *)wsp, 0, 0};
     double *r = cv.take<double>(A.N);
     double *d = cv.take<double>(4);
-    SENK_TRY(residual<double>(c, A, x, b, r, d, nullptr));
+    int rg = 0;
+    SENK_TRY(residual<double>(c, A, x, b, r, &rg));
+    k_norm_fin<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, d, nullptr);
+    SENK_LAUNCH_CHECK();
     double h = 0;
     SENK_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SENK_HIP(hipStreamSynchronize(c->stream));
@@ -980,7 +1035,8 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
     std::memset(res, 0, sizeof(*res));
     const int N = A.N;
     if (!A.v32) return SENK_ERR_ARG;
-    // Inner solver workspace uses the same arena, so keep the IR vectors separate.
+    // The inner solver carves its workspace from the context arena, so the refinement
+    // vectors are allocated separately.
     double *r64 = nullptr;
     float *r32 = nullptr, *d32 = nullptr;
     double *dsc = nullptr;
@@ -988,10 +1044,8 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
     SENK_HIP(hipMalloc(&r32, sizeof(float) * N));
     SENK_HIP(hipMalloc(&d32, sizeof(float) * N));
     SENK_HIP(hipMalloc(&dsc, sizeof(double) * 4));
-    hipEvent_t e0, e1;
-    SENK_HIP(hipEventCreate(&e0));
-    SENK_HIP(hipEventCreate(&e1));
-    SENK_HIP(hipEventRecord(e0, c->stream));
+    SolveTimer tm;
+    SENK_TRY(tm.start(c));
     SolveParams ip;
     ip.solver = o.solver;
     ip.m = o.m;
@@ -1005,15 +1059,27 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
     int steps = 0, inner_total = 0, converged = 0, status = SENK_OK;
     double relres = 0;
     for (;;) {
-        SENK_TRY(residual<double>(c, A, x, b, r64, dsc, dsc + 1));
+        int rg = 0;
+        SENK_TRY(residual<double>(c, A, x, b, r64, &rg));
+        k_norm_fin<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, dsc, dsc + 1);
+        SENK_LAUNCH_CHECK();
         double h2 = 0;
         SENK_HIP(hipMemcpyAsync(&h2, dsc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         SENK_HIP(hipStreamSynchronize(c->stream));
         relres = std::sqrt(h2) / nrm_b;
         if (hist && steps < hist_cap) hist[steps] = relres;
-        if (o.verbose) { std::printf("IR %d %e\n", steps, relres); std::fflush(stdout); }
-        if (!std::isfinite(relres)) { status = SENK_ERR_BREAKDOWN; break; }
-        if (relres <= o.eps) { converged = 1; break; }
+        if (o.verbose) {
+            std::printf("IR %d %e\n", steps, relres);
+            std::fflush(stdout);
+        }
+        if (!std::isfinite(relres)) {
+            status = SENK_ERR_BREAKDOWN;
+            break;
+        }
+        if (relres <= o.eps) {
+            converged = 1;
+            break;
+        }
         if (steps >= max_steps || inner_total >= o.max_iter) break;
         SENK_TRY(launch_convert<double, float>(c, N, r64, r32, dsc + 1));
         SENK_HIP(hipMemsetAsync(d32, 0, sizeof(float) * N, c->stream));
@@ -1024,12 +1090,8 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
         SENK_LAUNCH_CHECK();
         steps++;
     }
-    SENK_HIP(hipEventRecord(e1, c->stream));
-    SENK_HIP(hipEventSynchronize(e1));
     float ms = 0;
-    SENK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    SENK_TRY(tm.stop(c, &ms));
     (void)hipFree(r64);
     (void)hipFree(r32);
     (void)hipFree(d32);
