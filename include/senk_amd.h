@@ -328,6 +328,9 @@ typedef struct {
     int p;          /* ILU(p) level                                                    */
     int bnl, bnw;   /* ILUB block shape                                                */
     int precision;  /* bitmask of senk_factor_precision copies kept: 1<<F64 | 1<<F32 ... */
+    int csr_only;   /* 0: also keep the factors in the sliced solve layout (block sizes
+                       dividing 64) and solve with it; 1: CSR factors and kernels only.
+                       Both give the same bits.                                         */
 } senk_ilu_opts;
 /* Factorize A (already ABMC-reordered with colors cptr/cnum/bsize) on the device:
  * host symbolic pattern (ILU(p)/ILUB), device numeric Ilu0 per color, device split

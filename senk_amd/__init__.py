@@ -55,7 +55,8 @@ class senk_sellcs(C.Structure):
 
 
 class senk_ilu_opts(C.Structure):
-    _fields_ = [("kind", C.c_int), ("p", C.c_int), ("bnl", C.c_int), ("bnw", C.c_int), ("precision", C.c_int)]
+    _fields_ = [("kind", C.c_int), ("p", C.c_int), ("bnl", C.c_int), ("bnw", C.c_int), ("precision", C.c_int),
+                ("csr_only", C.c_int)]
 
 
 class senk_solve_opts(C.Structure):

@@ -33,6 +33,7 @@ struct senk_mat_s {
 struct senk_prec_s {
     senk_ctx ctx = nullptr;
     senk_amd::PrecView view;
+    senk_amd::TriLayout lt, ut;
     int lnnz = 0, unnz = 0;
     std::vector<void *> owned;
     std::vector<int> hlrptr, hurptr;
@@ -1018,6 +1019,21 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
             v.l32 = l32;
             v.u32 = u32;
         }
+        // solve layouts; a pattern the layout cannot hold keeps the CSR kernels
+        if (!opts->csr_only && tri_layout_build(c, v.ls, false, N, lr, lc, &M->lt) == SENK_OK &&
+            tri_layout_build(c, v.us, true, N, ur, uc, &M->ut) == SENK_OK) {
+            if ((st = tri_layout_fill<double>(c, v.ls, &M->lt, lr, lc, l64))) break;
+            if ((st = tri_layout_fill<double>(c, v.us, &M->ut, ur, uc, u64))) break;
+            if (v.l32) {
+                if ((st = tri_layout_fill<float>(c, v.ls, &M->lt, lr, lc, v.l32))) break;
+                if ((st = tri_layout_fill<float>(c, v.us, &M->ut, ur, uc, v.u32))) break;
+            }
+            v.lt = &M->lt;
+            v.ut = &M->ut;
+        } else {
+            tri_layout_free(&M->lt);
+            tri_layout_free(&M->ut);
+        }
         if (hipStreamSynchronize(c->stream) != hipSuccess) st = SENK_ERR_HIP;
     } while (0);
     (void)hipFree(dfr);
@@ -1037,6 +1053,8 @@ int senk_prec_destroy(senk_prec M) {
     if (!M) return SENK_ERR_ARG;
     (void)hipStreamSynchronize(M->ctx->c.stream);
     for (void *p : M->owned) (void)hipFree(p);
+    senk_amd::tri_layout_free(&M->lt);
+    senk_amd::tri_layout_free(&M->ut);
     delete M;
     return SENK_OK;
 }

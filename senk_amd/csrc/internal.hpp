@@ -71,6 +71,33 @@ struct Schedule {
     int *dblk = nullptr;       // device, optional
 };
 
+// A triangular factor rearranged for the scheduled solve (trsv_sell.hip): per level,
+// slices of 64 rows; per row the external entries (value + column) and the in-block
+// entries (value; column = bit of `mask`), each group stored packed and in the order
+// the solve consumes it.  Built once per pattern, filled per precision.
+struct TriLayout {
+    int bsize = 0;
+    bool upper = false;
+    std::vector<int> lev_slice;  // host, nlev + 1: first slice of each level
+    int nslices = 0;
+    long long n_ext = 0, n_int = 0;
+    int2 *soff = nullptr;        // per slice: offset of its external / in-block entries
+    int *next = nullptr;         // per slot (slice*64 + lane): external entries of the row
+    void *mask = nullptr;        // per slot: in-block columns (u32, or u64 when bsize == 64)
+    int *scol = nullptr;         // n_ext external columns
+    bool cols_filled = false;
+    double *ve64 = nullptr, *vi64 = nullptr, *d64 = nullptr;  // values; d = inverted diagonal (U)
+    float *ve32 = nullptr, *vi32 = nullptr, *d32 = nullptr;
+};
+int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rptr, const int *cind,
+                     TriLayout *L);
+template <typename T>
+int tri_layout_fill(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, const int *cind, const T *val);
+void tri_layout_free(TriLayout *L);
+template <typename T>
+int launch_sptrsv_layout(Ctx *c, const Schedule &s, const TriLayout &L, const T *x, T *y, const double *in_scale,
+                         const int *stop);
+
 // ----------------------------------------------------------------- launchers (kernels.hip)
 template <typename T>
 int launch_spmv_csr(Ctx *c, int N, const int *rptr, const int *cind, const T *val, const T *x, T *y,

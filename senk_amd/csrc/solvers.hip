@@ -391,6 +391,12 @@ int learned_apply<double>(Ctx *c, const PrecView &M, const double *in, const dou
 }
 
 template <typename T>
+static bool layout_has(const TriLayout *L) {
+    if (!L) return false;
+    return std::is_same<T, double>::value ? L->ve64 != nullptr : L->ve32 != nullptr;
+}
+
+template <typename T>
 int prec_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T *out, T *tmp, const int *stop) {
     const int N = M.N;
     if (M.kind == 0) {
@@ -400,8 +406,13 @@ int prec_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T
         const T *lv = std::is_same<T, double>::value ? (const T *)M.l64 : (const T *)M.l32;
         const T *uv = std::is_same<T, double>::value ? (const T *)M.u64 : (const T *)M.u32;
         if (!lv || !uv) return SENK_ERR_ARG;
-        SENK_TRY(launch_sptrsv<T>(c, M.ls, false, M.lrptr, M.lcind, lv, in, out, in_scale, stop));
-        SENK_TRY(launch_sptrsv<T>(c, M.us, true, M.urptr, M.ucind, uv, out, out, nullptr, stop));
+        if (layout_has<T>(M.lt) && layout_has<T>(M.ut)) {
+            SENK_TRY(launch_sptrsv_layout<T>(c, M.ls, *M.lt, in, out, in_scale, stop));
+            SENK_TRY(launch_sptrsv_layout<T>(c, M.us, *M.ut, out, out, nullptr, stop));
+        } else {
+            SENK_TRY(launch_sptrsv<T>(c, M.ls, false, M.lrptr, M.lcind, lv, in, out, in_scale, stop));
+            SENK_TRY(launch_sptrsv<T>(c, M.us, true, M.urptr, M.ucind, uv, out, out, nullptr, stop));
+        }
     } else if (M.kind == 2) {
         const T *lv = std::is_same<T, double>::value ? (const T *)M.bl64 : (const T *)M.bl32;
         const T *uv = std::is_same<T, double>::value ? (const T *)M.bu64 : (const T *)M.bu32;

@@ -30,6 +30,8 @@ struct PrecView {
     const int *lrptr = nullptr, *lcind = nullptr, *urptr = nullptr, *ucind = nullptr;
     const double *l64 = nullptr, *u64 = nullptr;
     const float *l32 = nullptr, *u32 = nullptr;
+    // optional solve layouts of the CSR factors (used when they hold the precision)
+    const TriLayout *lt = nullptr, *ut = nullptr;
     // BCSR factors
     int bnl = 0, bnw = 0;
     const int *blrptr = nullptr, *blcind = nullptr, *burptr = nullptr, *bucind = nullptr;

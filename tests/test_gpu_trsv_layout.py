@@ -1,0 +1,124 @@
+"""The sliced solve layout of the ILU factors against the CSR kernels and the oracle.
+
+A preconditioner handle keeps its factors in the sliced layout by default and in CSR
+only when asked (csr_only = 1).  Both paths must give the oracle's sequential
+L-then-U solve bit for bit in FP64 (same operations in the same order per row), and
+the same bits as each other in FP32, over every block size the layout takes, for
+color schedules and for dependency-level schedules (ILU(p) fill, natural ordering).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import senk_amd as S
+from oracle import oracle as O
+from tests.helpers import abmc_system, oracle_ilu0
+
+pytestmark = pytest.mark.gpu
+
+
+def _expected(sy, kind, p):
+    N = sy["N"]
+    if kind == S.SENK_ILUP and p > 0:
+        fv, fc, fr = S.alloc_level_zero(sy["val"], sy["cind"], sy["rptr"], N, p)
+    else:
+        fv, fc, fr = sy["val"], sy["cind"], sy["rptr"]
+    f = oracle_ilu0(fv, fc, fr)
+    return S.split(f, fc, fr, N, "L-DU", True)
+
+
+def _oracle_apply(sp, x):
+    N = len(x)
+    yl, yu = np.zeros(N), np.zeros(N)
+    O.orc().or_sptrsv_l_d(O.P(sp["L"][0]), O.P(sp["L"][1]), O.P(sp["L"][2]), O.P(x), O.P(yl), N)
+    O.orc().or_sptrsv_u_d(O.P(sp["U"][0]), O.P(sp["U"][1]), O.P(sp["U"][2]), O.P(yl), O.P(yu), N)
+    return yu
+
+
+def _handles(ctx, sy, kind, p, bsize, colors):
+    N = sy["N"]
+    A = C.c_void_p()
+    S.call("senk_mat_create", ctx.h, N, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]), 0, C.byref(A))
+    out = []
+    for csr_only in (0, 1):
+        opts = S.senk_ilu_opts(kind, p, 0, 0, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32), csr_only)
+        M = C.c_void_p()
+        cp = sy["cptr"].ctypes.data_as(C.c_void_p) if colors else None
+        S.call("senk_prec_create_ilu", ctx.h, A, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]), cp,
+               sy["nc"] if colors else 0, bsize, C.byref(opts), C.byref(M))
+        out.append(M)
+    return A, out
+
+
+def _check(ctx, sy, kind, p, bsize, colors, seed=3):
+    N = sy["N"]
+    A, (Ml, Mc) = _handles(ctx, sy, kind, p, bsize, colors)
+    try:
+        sp = _expected(sy, kind, p)
+        x = np.random.default_rng(seed).uniform(-1, 1, N)
+        want = _oracle_apply(sp, x)
+        dx = ctx.array(x)
+        got = []
+        for M in (Ml, Mc):
+            dy = ctx.zeros(N)
+            S.call("senk_prec_apply_f64", M, dx.p, dy.p)
+            got.append(dy.download())
+        assert np.array_equal(got[0], want), np.max(np.abs(got[0] - want))
+        assert np.array_equal(got[1], want)
+        x32 = x.astype(np.float32)
+        dx32 = ctx.array(x32)
+        got32 = []
+        for M in (Ml, Mc):
+            dy = ctx.array(np.zeros(N, np.float32))
+            S.call("senk_prec_apply_f32", M, dx32.p, dy.p)
+            got32.append(dy.download())
+        assert np.array_equal(got32[0], got32[1])
+        assert np.max(np.abs(got32[0] - want)) <= 1e-4 * max(1.0, np.max(np.abs(want)))
+        # applying twice gives the same bits (no state carried between applications)
+        dy = ctx.zeros(N)
+        S.call("senk_prec_apply_f64", Ml, dx.p, dy.p)
+        assert np.array_equal(dy.download(), want)
+    finally:
+        for M in (Ml, Mc):
+            S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
+
+
+@pytest.mark.parametrize("bsize", [1, 2, 4, 8, 16, 32, 64])
+def test_layout_abmc_poisson(ctx, bsize):
+    rptr, cind, val = S.gen_poisson7(16)
+    sy = abmc_system(rptr, cind, val, bsize)
+    _check(ctx, sy, S.SENK_ILU0, 0, bsize, colors=True)
+
+
+@pytest.mark.parametrize("bsize", [4, 16])
+def test_layout_abmc_convdiff27(ctx, bsize):
+    rptr, cind, val = S.gen_convdiff27(12, 10.0)
+    sy = abmc_system(rptr, cind, val, bsize)
+    _check(ctx, sy, S.SENK_ILU0, 0, bsize, colors=True)
+
+
+@pytest.mark.parametrize("p,bsize", [(1, 8), (2, 4)])
+def test_layout_ilup_fill(ctx, p, bsize):
+    # ILU(p) fill can couple blocks of one color: the handle falls back to the
+    # dependency levels of the factor (block lists), which the layout also takes
+    rptr, cind, val = S.gen_convdiff27(10, 10.0)
+    sy = abmc_system(rptr, cind, val, bsize)
+    _check(ctx, sy, S.SENK_ILUP, p, bsize, colors=True)
+
+
+@pytest.mark.parametrize("bsize", [1, 4])
+def test_layout_natural_order_levels(ctx, bsize):
+    rptr, cind, val = S.gen_unstructured(2048, 5)
+    N = len(rptr) - 1
+    sy = dict(N=N, nc=0, cptr=np.zeros(1, np.int32), val=val, cind=cind, rptr=rptr)
+    _check(ctx, sy, S.SENK_ILU0, 0, bsize, colors=False)
+
+
+def test_layout_partial_last_slice(ctx):
+    # colors whose row counts are not multiples of 64 leave partial slices
+    rptr, cind, val = S.gen_poisson7(10, 9, 7)
+    sy = abmc_system(rptr, cind, val, 2)
+    assert any((np.diff(sy["cptr"]) * 2) % 64)
+    _check(ctx, sy, S.SENK_ILU0, 0, 2, colors=True)
