@@ -48,8 +48,19 @@ def spmv_bytes(n, nnz, sv, si=4):
 
 
 def build_system(n, bsize):
+    return build_system_kind("convdiff", n, bsize)
+
+
+def build_system_kind(kind, n, bsize, seed=1):
+    """kind "convdiff": 27-point convection-diffusion, Pe 10; "vardiff": 7-point
+    -div(k grad u) with k = exp(Gaussian random field, correlation length 8, sigma 1.5)."""
     t0 = time.perf_counter()
-    rptr, cind, val = S.gen_convdiff27(n, 10.0)
+    if kind == "convdiff":
+        rptr, cind, val = S.gen_convdiff27(n, 10.0)
+    elif kind == "vardiff":
+        rptr, cind, val = S.gen_vardiff7(n, seed, 8.0, 1.5)
+    else:
+        raise ValueError(kind)
     N = len(rptr) - 1
     xs = S.gen_uniform(1, N)
     t1 = time.perf_counter()

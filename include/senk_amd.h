@@ -306,9 +306,56 @@ int senk_abmc_ilu_bicgstab_f64(senk_ctx ctx, const double *dval, const int *dcin
                                const double *db, double *dx, double nrm_b, int max_iter, int N, double epsilon,
                                int verbose, senk_result *res, double *hist, int hist_cap); /* :107-176 */
 
+/* Sequential, AMC and block-Jacobi forms.  lplan/uplan are senk_trsv_plan_create plans
+ * of the factors' patterns (their dependency levels); the triangular solves give the
+ * sequential result bit for bit.  ILUB forms take BCSR factors (bnl x bnw blocks) and
+ * schedule their block rows by dependency levels. */
+int senk_gmresm_bcsr_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
+                         const double *db, double *dx, double nrm_b, int outer, int m, int N, double epsilon,
+                         int bnl, int bnw, int verbose, senk_result *res, double *hist, int hist_cap); /* senk_gmres.hpp:113-175 */
+int senk_amc_ilu_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                            const double *dlval, const int *dlcind, const int *dlrptr,
+                            const double *duval, const int *ducind, const int *durptr,
+                            const int *cptr, int cnum,
+                            const double *db, double *dx, double nrm_b, int outer, int m, int N, double epsilon,
+                            int verbose, senk_result *res, double *hist, int hist_cap);  /* senk_gmres.hpp:293-367 */
+int senk_bj_ilu_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                           const double *dlval, const int *dlcind, const int *dlrptr,
+                           const double *duval, const int *ducind, const int *durptr,
+                           senk_trsv_plan lplan, senk_trsv_plan uplan, int bnum,
+                           const double *db, double *dx, double nrm_b, int outer, int m, int N, double epsilon,
+                           int verbose, senk_result *res, double *hist, int hist_cap);   /* senk_gmres.hpp:487-561 */
+int senk_ilub_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                         const double *dblval, const int *dblcind, const int *dblrptr,
+                         const double *dbuval, const int *dbucind, const int *dburptr,
+                         const double *db, double *dx, double nrm_b, int outer, int m, int N, double epsilon,
+                         int bnl, int bnw, int verbose, senk_result *res, double *hist, int hist_cap); /* senk_gmres.hpp:582-655 */
+int senk_ilu_gcrm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                      const double *dlval, const int *dlcind, const int *dlrptr,
+                      const double *duval, const int *ducind, const int *durptr,
+                      senk_trsv_plan lplan, senk_trsv_plan uplan,
+                      const double *db, double *dx, double nrm_b, int outer, int m, int N, double epsilon,
+                      int verbose, senk_result *res, double *hist, int hist_cap);        /* senk_gcr.hpp:93-142 */
+int senk_ilub_gcrm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                       const double *dblval, const int *dblcind, const int *dblrptr,
+                       const double *dbuval, const int *dbucind, const int *dburptr,
+                       const double *db, double *dx, double nrm_b, int outer, int m, int N, double epsilon,
+                       int bnl, int bnw, int verbose, senk_result *res, double *hist, int hist_cap); /* senk_gcr.hpp:163-212 */
+int senk_ilu_bicgstab_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                          const double *dlval, const int *dlcind, const int *dlrptr,
+                          const double *duval, const int *ducind, const int *durptr,
+                          senk_trsv_plan lplan, senk_trsv_plan uplan,
+                          const double *db, double *dx, double nrm_b, int max_iter, int N, double epsilon,
+                          int verbose, senk_result *res, double *hist, int hist_cap);    /* senk_bicgstab.hpp:107-176 */
+int senk_ilub_bicgstab_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                           const double *dblval, const int *dblcind, const int *dblrptr,
+                           const double *dbuval, const int *dbucind, const int *dburptr,
+                           const double *db, double *dx, double nrm_b, int max_iter, int N, double epsilon,
+                           int bnl, int bnw, int verbose, senk_result *res, double *hist, int hist_cap); /* senk_bicgstab.hpp:198-267 */
+
 /* ------------------------------------------------------------------ handle API
  * Owned device matrix + preconditioner for repeated solves, mixed precision and the
- * learned correction (no reference counterparts; n1/n2/n4/n5 extensions). */
+ * learned correction (no reference counterparts). */
 typedef struct senk_mat_s *senk_mat;
 typedef struct senk_prec_s *senk_prec;
 
@@ -352,13 +399,16 @@ typedef struct {
     int max_iter;        /* iteration budget (GMRES/GCR: rounded up to whole cycles)     */
     double eps;          /* relative residual target (relative to ||b||)                  */
     int ortho;           /* senk_ortho (GMRES family)                                    */
-    int mixed;           /* 1: FP32 matrix/factors/inner Krylov + FP64 iterative refinement */
+    int mixed;           /* 1: FP32 matrix/factors/inner Krylov + FP64 iterative refinement;
+                            2: FP32 inner Krylov as the preconditioner of an FP64 flexible
+                               GCR(outer_m) outer iteration (nested)                      */
     int factor_prec;     /* senk_factor_precision used by the inner solve                */
     double inner_eps;    /* mixed: inner relative tolerance per refinement step          */
     int inner_max;       /* mixed: inner iteration budget per refinement step            */
     int check_every;     /* GCR: 1 = test ||r|| every step (reference: restarts only)    */
     int use_learned;     /* 1: add the learned correction (flexible solvers only)         */
     int verbose;         /* print the reference's lines                                  */
+    int outer_m;         /* mixed == 2: restart length of the FP64 outer GCR               */
 } senk_solve_opts;
 void senk_solve_opts_default(senk_solve_opts *o);
 int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *dx, const senk_solve_opts *opts,

@@ -62,7 +62,8 @@ class senk_ilu_opts(C.Structure):
 class senk_solve_opts(C.Structure):
     _fields_ = [("solver", C.c_int), ("m", C.c_int), ("max_iter", C.c_int), ("eps", C.c_double),
                 ("ortho", C.c_int), ("mixed", C.c_int), ("factor_prec", C.c_int), ("inner_eps", C.c_double),
-                ("inner_max", C.c_int), ("check_every", C.c_int), ("use_learned", C.c_int), ("verbose", C.c_int)]
+                ("inner_max", C.c_int), ("check_every", C.c_int), ("use_learned", C.c_int), ("verbose", C.c_int),
+                ("outer_m", C.c_int)]
 
 
 # ----------------------------------------------------------------- header-driven prototypes

@@ -50,17 +50,19 @@ void record_hip_error(hipError_t e, const char *what, const char *file, int line
 }
 void record_error(const std::string &msg) { g_last_error = msg; }
 
-int ctx_work(Ctx *c, size_t bytes, void **out) {
-    if (c->work.bytes < bytes) {
-        if (c->work.p) SENK_HIP(hipFree(c->work.p));
-        c->work.p = nullptr;
-        c->work.bytes = 0;
-        SENK_HIP(hipMalloc(&c->work.p, bytes));
-        c->work.bytes = bytes;
+static int arena_get(DevBuf &a, size_t bytes, void **out) {
+    if (a.bytes < bytes) {
+        if (a.p) SENK_HIP(hipFree(a.p));
+        a.p = nullptr;
+        a.bytes = 0;
+        SENK_HIP(hipMalloc(&a.p, bytes));
+        a.bytes = bytes;
     }
-    *out = c->work.p;
+    *out = a.p;
     return SENK_OK;
 }
+int ctx_work(Ctx *c, size_t bytes, void **out) { return arena_get(c->work, bytes, out); }
+int ctx_outer(Ctx *c, size_t bytes, void **out) { return arena_get(c->outer, bytes, out); }
 
 namespace {
 int nnz_of(senk_ctx ctx, const int *drptr, int N, int *nnz) {
@@ -192,6 +194,7 @@ int senk_ctx_destroy(senk_ctx ctx) {
     (void)hipSetDevice(c.device);
     (void)hipStreamSynchronize(c.stream);
     if (c.work.p) (void)hipFree(c.work.p);
+    if (c.outer.p) (void)hipFree(c.outer.p);
     (void)hipFree(c.red_partials);
     (void)hipFree(c.dscalars);
     (void)hipHostFree(c.hpinned);
@@ -454,11 +457,8 @@ int senk_sptrsv_bcsr_u_abmc_f64(senk_ctx ctx, const double *dbval, const int *db
 }
 // Sequential BCSR forms: the group-level dependency levels are computed from a host
 // copy of the block pattern on every call (the reference runs these serially too).
-static int bcsr_seq(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
-                    double *dy, int N, int bnl, int bnw, bool upper) {
-    CTX_CHECK(ctx);
-    if (bnl <= 0 || bnw <= 0 || N < 0) return SENK_ERR_ARG;
-    if (N % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
+static int bcsr_seq_schedule(senk_ctx ctx, const int *dbcind, const int *dbrptr, int N, int bnl, int bnw, bool upper,
+                             Schedule *s) {
     const int ng = N / bnl;
     std::vector<int> brptr((size_t)ng + 1);
     SENK_HIP(hipMemcpyAsync(brptr.data(), dbrptr, sizeof(int) * (ng + 1), hipMemcpyDeviceToHost, ctx->c.stream));
@@ -467,12 +467,19 @@ static int bcsr_seq(senk_ctx ctx, const double *dbval, const int *dbcind, const 
     if (brptr[ng])
         SENK_HIP(hipMemcpyAsync(bcind.data(), dbcind, sizeof(int) * brptr[ng], hipMemcpyDeviceToHost, ctx->c.stream));
     SENK_HIP(hipStreamSynchronize(ctx->c.stream));
-    Schedule s;
-    s.bsize = bnl;
+    s->bsize = bnl;
     std::vector<int> blk;
-    bcsr_levels(brptr, bcind, ng, bnl, bnw, upper, s.lev_ptr, blk);
-    s.nlev = (int)s.lev_ptr.size() - 1;
-    SENK_TRY(upload(ctx, &s.dblk, blk.data(), blk.size()));
+    bcsr_levels(brptr, bcind, ng, bnl, bnw, upper, s->lev_ptr, blk);
+    s->nlev = (int)s->lev_ptr.size() - 1;
+    return upload(ctx, &s->dblk, blk.data(), blk.size());
+}
+static int bcsr_seq(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
+                    double *dy, int N, int bnl, int bnw, bool upper) {
+    CTX_CHECK(ctx);
+    if (bnl <= 0 || bnw <= 0 || N < 0) return SENK_ERR_ARG;
+    if (N % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
+    Schedule s;
+    SENK_TRY(bcsr_seq_schedule(ctx, dbcind, dbrptr, N, bnl, bnw, upper, &s));
     const int st = launch_sptrsv_bcsr<double>(&ctx->c, s, upper, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
     SENK_HIP(hipStreamSynchronize(ctx->c.stream));
     (void)hipFree(s.dblk);
@@ -844,6 +851,146 @@ int senk_abmc_ilu_bicgstab_f64(senk_ctx ctx, const double *dval, const int *dcin
     return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
 }
 
+// ---------------------------------------------------------------- further driver twins
+// The reference's sequential and block-Jacobi preconditioned drivers run here on level
+// schedules of the factors (plans, or BCSR block-row levels computed from the pattern);
+// the triangular solves give the sequential result bit for bit.
+static SolveParams twin_params(int solver, int outer, int m, int max_iter, double nrm_b, double eps, int verbose) {
+    SolveParams p;
+    p.solver = solver;
+    if (m > 0) p.m = m;  // BiCGStab has no restart length
+    p.outer = outer;
+    p.max_iter = max_iter;
+    p.eps = eps;
+    p.nrm_b = nrm_b;
+    p.verbose = verbose;
+    return p;
+}
+static int plan_twin(senk_ctx ctx, int solver, const double *dval, const int *dcind, const int *drptr,
+                     const double *dlval, const int *dlcind, const int *dlrptr, const double *duval,
+                     const int *ducind, const int *durptr, senk_trsv_plan lplan, senk_trsv_plan uplan,
+                     const double *db, double *dx, double nrm_b, int outer, int m, int max_iter, int N,
+                     double epsilon, int verbose, senk_result *res, double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || !lplan || !uplan || (solver != SENK_BICGSTAB && m <= 0)) return SENK_ERR_ARG;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    ilu_view(M, N, dlval, dlcind, dlrptr, duval, ducind, durptr);
+    M.ls = lplan->s;
+    M.us = uplan->s;
+    SolveParams p = twin_params(solver, outer, m, max_iter, nrm_b, epsilon, verbose);
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+static int ilub_seq_twin(senk_ctx ctx, int solver, const double *dval, const int *dcind, const int *drptr,
+                         const double *dblval, const int *dblcind, const int *dblrptr, const double *dbuval,
+                         const int *dbucind, const int *dburptr, const double *db, double *dx, double nrm_b,
+                         int outer, int m, int max_iter, int N, double epsilon, int bnl, int bnw, int verbose,
+                         senk_result *res, double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || bnl <= 0 || bnw <= 0 || (solver != SENK_BICGSTAB && m <= 0)) return SENK_ERR_ARG;
+    if (N % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    PrecView M;
+    M.N = N;
+    M.kind = 2;
+    M.bnl = bnl;
+    M.bnw = bnw;
+    M.bl64 = dblval;
+    M.blcind = dblcind;
+    M.blrptr = dblrptr;
+    M.bu64 = dbuval;
+    M.bucind = dbucind;
+    M.burptr = dburptr;
+    int st = bcsr_seq_schedule(ctx, dblcind, dblrptr, N, bnl, bnw, false, &M.ls);
+    if (st == SENK_OK) st = bcsr_seq_schedule(ctx, dbucind, dburptr, N, bnl, bnw, true, &M.us);
+    if (st == SENK_OK) {
+        SolveParams p = twin_params(solver, outer, m, max_iter, nrm_b, epsilon, verbose);
+        st = run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+    }
+    (void)hipStreamSynchronize(ctx->c.stream);
+    (void)hipFree(M.ls.dblk);
+    (void)hipFree(M.us.dblk);
+    return st;
+}
+
+int senk_gmresm_bcsr_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *db,
+                         double *dx, double nrm_b, int outer, int m, int N, double epsilon, int bnl, int bnw,
+                         int verbose, senk_result *res, double *hist, int hist_cap) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || m <= 0 || outer < 0 || bnl <= 0 || bnw <= 0 || !dbval || !dbcind || !dbrptr) return SENK_ERR_ARG;
+    if (N % bnl || N % bnw) return SENK_ERR_DIVISIBILITY;
+    MatView A;
+    A.N = N;
+    A.bnl = bnl;
+    A.bnw = bnw;
+    A.brptr = dbrptr;
+    A.bcind = dbcind;
+    A.bv64 = dbval;
+    PrecView M;
+    M.N = N;
+    SolveParams p = twin_params(SENK_GMRES, outer, m, 0, nrm_b, epsilon, verbose);
+    return run_twin(ctx, A, M, p, db, dx, res, hist, hist_cap);
+}
+int senk_amc_ilu_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                            const double *dlval, const int *dlcind, const int *dlrptr, const double *duval,
+                            const int *ducind, const int *durptr, const int *cptr, int cnum, const double *db,
+                            double *dx, double nrm_b, int outer, int m, int N, double epsilon, int verbose,
+                            senk_result *res, double *hist, int hist_cap) {
+    return senk_abmc_ilu_gmresm_f64(ctx, dval, dcind, drptr, dlval, dlcind, dlrptr, duval, ducind, durptr, cptr, cnum,
+                                    1, db, dx, nrm_b, outer, m, N, epsilon, verbose, res, hist, hist_cap);
+}
+int senk_bj_ilu_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dlval,
+                           const int *dlcind, const int *dlrptr, const double *duval, const int *ducind,
+                           const int *durptr, senk_trsv_plan lplan, senk_trsv_plan uplan, int bnum, const double *db,
+                           double *dx, double nrm_b, int outer, int m, int N, double epsilon, int verbose,
+                           senk_result *res, double *hist, int hist_cap) {
+    if (bnum <= 0) return SENK_ERR_ARG;
+    return plan_twin(ctx, SENK_GMRES, dval, dcind, drptr, dlval, dlcind, dlrptr, duval, ducind, durptr, lplan, uplan,
+                     db, dx, nrm_b, outer, m, 0, N, epsilon, verbose, res, hist, hist_cap);
+}
+int senk_ilub_gmresm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dblval,
+                         const int *dblcind, const int *dblrptr, const double *dbuval, const int *dbucind,
+                         const int *dburptr, const double *db, double *dx, double nrm_b, int outer, int m, int N,
+                         double epsilon, int bnl, int bnw, int verbose, senk_result *res, double *hist, int hist_cap) {
+    return ilub_seq_twin(ctx, SENK_GMRES, dval, dcind, drptr, dblval, dblcind, dblrptr, dbuval, dbucind, dburptr, db,
+                         dx, nrm_b, outer, m, 0, N, epsilon, bnl, bnw, verbose, res, hist, hist_cap);
+}
+int senk_ilu_gcrm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dlval,
+                      const int *dlcind, const int *dlrptr, const double *duval, const int *ducind, const int *durptr,
+                      senk_trsv_plan lplan, senk_trsv_plan uplan, const double *db, double *dx, double nrm_b,
+                      int outer, int m, int N, double epsilon, int verbose, senk_result *res, double *hist,
+                      int hist_cap) {
+    return plan_twin(ctx, SENK_GCR, dval, dcind, drptr, dlval, dlcind, dlrptr, duval, ducind, durptr, lplan, uplan,
+                     db, dx, nrm_b, outer, m, 0, N, epsilon, verbose, res, hist, hist_cap);
+}
+int senk_ilub_gcrm_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dblval,
+                       const int *dblcind, const int *dblrptr, const double *dbuval, const int *dbucind,
+                       const int *dburptr, const double *db, double *dx, double nrm_b, int outer, int m, int N,
+                       double epsilon, int bnl, int bnw, int verbose, senk_result *res, double *hist, int hist_cap) {
+    return ilub_seq_twin(ctx, SENK_GCR, dval, dcind, drptr, dblval, dblcind, dblrptr, dbuval, dbucind, dburptr, db, dx,
+                         nrm_b, outer, m, 0, N, epsilon, bnl, bnw, verbose, res, hist, hist_cap);
+}
+int senk_ilu_bicgstab_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dlval,
+                          const int *dlcind, const int *dlrptr, const double *duval, const int *ducind,
+                          const int *durptr, senk_trsv_plan lplan, senk_trsv_plan uplan, const double *db, double *dx,
+                          double nrm_b, int max_iter, int N, double epsilon, int verbose, senk_result *res,
+                          double *hist, int hist_cap) {
+    if (max_iter < 0) return SENK_ERR_ARG;
+    return plan_twin(ctx, SENK_BICGSTAB, dval, dcind, drptr, dlval, dlcind, dlrptr, duval, ducind, durptr, lplan,
+                     uplan, db, dx, nrm_b, 0, 0, max_iter, N, epsilon, verbose, res, hist, hist_cap);
+}
+int senk_ilub_bicgstab_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, const double *dblval,
+                           const int *dblcind, const int *dblrptr, const double *dbuval, const int *dbucind,
+                           const int *dburptr, const double *db, double *dx, double nrm_b, int max_iter, int N,
+                           double epsilon, int bnl, int bnw, int verbose, senk_result *res, double *hist,
+                           int hist_cap) {
+    if (max_iter < 0) return SENK_ERR_ARG;
+    return ilub_seq_twin(ctx, SENK_BICGSTAB, dval, dcind, drptr, dblval, dblcind, dblrptr, dbuval, dbucind, dburptr,
+                         db, dx, nrm_b, 0, 0, max_iter, N, epsilon, bnl, bnw, verbose, res, hist, hist_cap);
+}
+
 // ---------------------------------------------------------------- handle API
 int senk_mat_create(senk_ctx ctx, int N, const int *rptr, const int *cind, const double *val, int flags,
                     senk_mat *out) {
@@ -1101,6 +1248,7 @@ void senk_solve_opts_default(senk_solve_opts *o) {
     o->inner_eps = 1e-5;
     o->inner_max = 600;
     o->check_every = 0;
+    o->outer_m = 20;
 }
 int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *dx, const senk_solve_opts *o,
                senk_result *res, double *hist, int hist_cap) {
@@ -1113,7 +1261,10 @@ int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *
     const PrecView &pv = M ? M->view : none;
     if (o->mixed) {
         if (!A->v32 || (M && !M->view.l32)) return SENK_ERR_ARG;
-        SENK_TRY(ir_solve(&ctx->c, A->view, pv, db, dx, *o, nrm_b, res, hist, hist_cap));
+        if (o->mixed == 2)
+            SENK_TRY(nested_solve(&ctx->c, A->view, pv, db, dx, *o, nrm_b, res, hist, hist_cap));
+        else
+            SENK_TRY(ir_solve(&ctx->c, A->view, pv, db, dx, *o, nrm_b, res, hist, hist_cap));
     } else {
         SolveParams p;
         p.solver = o->solver;

@@ -39,9 +39,8 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
-// Reduction scratch: partial sums per workgroup plus the arrival counter of the
-// "last workgroup finishes the reduction" pattern.  Counters are reset by the last
-// workgroup itself, so they are zero between kernels.
+// Reduction scratch: one partial sum per workgroup and dot product; the consuming
+// kernel adds them in a fixed order.
 constexpr int kRedMaxBlocks = 1024;  // workgroups of a reducing pass
 constexpr int kRedMaxK = 40;          // simultaneous dot products per pass
 constexpr int kRedSlots = 8;          // independent partial buffers
@@ -54,10 +53,12 @@ struct Ctx {
     double *hpinned = nullptr;        // 64 pinned host doubles
     int *hpinned_i = nullptr;         // 64 pinned host ints
     DevBuf work;                      // solver arena
+    DevBuf outer;                     // arena of the FP64 outer loop around an inner solve
     int num_cus = 256;
 };
 
 int ctx_work(Ctx *c, size_t bytes, void **out);
+int ctx_outer(Ctx *c, size_t bytes, void **out);
 inline double *red_slot(Ctx *c, int slot) { return c->red_partials + (size_t)slot * kRedMaxK * kRedMaxBlocks; }
 
 // ----------------------------------------------------------------- schedules

@@ -97,10 +97,33 @@ __global__ __launch_bounds__(kBlock) void k_norm_fin(const double *P, int B, dou
     }
 }
 
+// y = 1*b + (-1)*y with the partial sums of y^2 (residual of a matrix that has only its
+// BCSR form: y holds A x on entry)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub_norm(int N, const T *__restrict__ b, T *y, double *partials) {
+    double ss = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const T r = (T)1 * b[i] + (T)(-1) * y[i];
+        y[i] = r;
+        ss += (double)r * (double)r;
+    }
+    const double v[1] = {ss};
+    store_partials<1>(v, partials);
+}
+
 template <typename T>
 int residual(Ctx *c, const MatView &A, const T *x, const T *b, T *y, int *grid_out) {
     constexpr int G = 8;
     const T *v = mvals<T>(A);
+    if (!v && A.bnl && bvals<T>(A)) {
+        if (!x) return SENK_ERR_ARG;
+        SENK_TRY(spmv<T>(c, A, x, y, nullptr));
+        const int grid = red_grid(A.N);
+        k_sub_norm<T><<<grid, kBlock, 0, c->stream>>>(A.N, b, y, red_slot(c, kSlotRes));
+        SENK_LAUNCH_CHECK();
+        *grid_out = grid;
+        return SENK_OK;
+    }
     if (!v) return SENK_ERR_ARG;
     long long threads = (long long)A.N * G;
     int grid = (int)((threads + kBlock - 1) / kBlock);
@@ -755,6 +778,8 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
                 SENK_TRY(poll.poll(kLag, &stop));
             }
         }
+        // a cycle that ran to its end did m steps
+        last = (i + 1) * m;
         if (p.check_every) {
             SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
             SENK_HIP(hipStreamSynchronize(c->stream));
@@ -1041,31 +1066,78 @@ int true_relres(Ctx *c, const MatView &A, const double *b, const double *x, doub
 // FP64 iterative refinement around an FP32 Krylov solve (GMRES-IR): the FP64 residual
 // r = b - A x drives the stop test; the inner solve gets r/||r|| in FP32 and its
 // correction is added back in FP64.
+namespace {
+// q -= sum_k coef[k] Q_k and z -= sum_k coef[k] Z_k (vectors k at stride ld)
+__global__ __launch_bounds__(kBlock) void k_sub_lincomb2(int N, int K, const double *__restrict__ coef,
+                                                         const double *__restrict__ Q, const double *__restrict__ Z,
+                                                         size_t ld, double *q, double *z) {
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        double sq = q[i], sz = z[i];
+        for (int k = 0; k < K; k++) {
+            const double a = coef[k];
+            sq -= a * Q[(size_t)k * ld + i];
+            sz -= a * Z[(size_t)k * ld + i];
+        }
+        q[i] = sq;
+        z[i] = sz;
+    }
+}
+// s = {(q,q), (r,q)}: alpha = s1/s0, x += alpha z, r -= alpha q, then q and z are scaled
+// by 1/||q|| so that the stored directions satisfy A z = q with ||q|| = 1.
+__global__ __launch_bounds__(kBlock) void k_gcr_outer_step(int N, const double *__restrict__ s, double *q,
+                                                           double *z, double *x, double *r) {
+    const double alpha = s[1] / s[0], inv = 1.0 / sqrt(s[0]);
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const double qi = q[i], zi = z[i];
+        x[i] += alpha * zi;
+        r[i] -= alpha * qi;
+        q[i] = qi * inv;
+        z[i] = zi * inv;
+    }
+}
+__global__ void k_inv_sqrt(const double *s, double *inv) {
+    if (threadIdx.x == 0) *inv = 1.0 / sqrt(*s);
+}
+
+SolveParams inner_params(const senk_solve_opts &o) {
+    SolveParams ip;
+    ip.solver = o.solver;
+    ip.m = o.m;
+    ip.eps = o.inner_eps;
+    ip.nrm_b = 1.0;  // the inner right-hand side is normalised
+    ip.ortho = o.ortho;
+    ip.check_every = 1;
+    ip.max_iter = o.inner_max;
+    ip.outer = (o.inner_max + o.m - 1) / o.m;
+    return ip;
+}
+
+int read_scalar(Ctx *c, const double *d, double *h) {
+    SENK_HIP(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    return SENK_OK;
+}
+}  // namespace
+
+// Iterative refinement: r = b - A x in FP64, d = inner FP32 solve of A d = r/||r||,
+// x += ||r|| d, until ||r|| <= eps ||b||.
 int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x, const senk_solve_opts &o,
              double nrm_b, senk_result *res, double *hist, int hist_cap) {
     std::memset(res, 0, sizeof(*res));
     const int N = A.N;
     if (!A.v32) return SENK_ERR_ARG;
-    // The inner solver carves its workspace from the context arena, so the refinement
-    // vectors are allocated separately.
-    double *r64 = nullptr;
-    float *r32 = nullptr, *d32 = nullptr;
-    double *dsc = nullptr;
-    SENK_HIP(hipMalloc(&r64, sizeof(double) * N));
-    SENK_HIP(hipMalloc(&r32, sizeof(float) * N));
-    SENK_HIP(hipMalloc(&d32, sizeof(float) * N));
-    SENK_HIP(hipMalloc(&dsc, sizeof(double) * 4));
+    // the inner solver carves its workspace from the context arena; the refinement
+    // vectors live in the outer arena
+    const size_t need = bytes_of<double>(N) + 2 * bytes_of<float>(N) + bytes_of<double>(4);
+    void *wsp;
+    SENK_TRY(ctx_outer(c, need, &wsp));
+    Carver cv{(char *)wsp, 0, need};
+    double *r64 = cv.take<double>(N);
+    float *r32 = cv.take<float>(N), *d32 = cv.take<float>(N);
+    double *dsc = cv.take<double>(4);
     SolveTimer tm;
     SENK_TRY(tm.start(c));
-    SolveParams ip;
-    ip.solver = o.solver;
-    ip.m = o.m;
-    ip.eps = o.inner_eps;
-    ip.nrm_b = 1.0;
-    ip.ortho = o.ortho;
-    ip.check_every = 1;
-    ip.max_iter = o.inner_max;
-    ip.outer = (o.inner_max + o.m - 1) / o.m;
+    const SolveParams ip = inner_params(o);
     const int max_steps = std::max(1, (o.max_iter + o.inner_max - 1) / std::max(1, o.inner_max)) + 8;
     int steps = 0, inner_total = 0, converged = 0, status = SENK_OK;
     double relres = 0;
@@ -1075,8 +1147,7 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
         k_norm_fin<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, dsc, dsc + 1);
         SENK_LAUNCH_CHECK();
         double h2 = 0;
-        SENK_HIP(hipMemcpyAsync(&h2, dsc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        SENK_HIP(hipStreamSynchronize(c->stream));
+        SENK_TRY(read_scalar(c, dsc, &h2));
         relres = std::sqrt(h2) / nrm_b;
         if (hist && steps < hist_cap) hist[steps] = relres;
         if (o.verbose) {
@@ -1103,10 +1174,6 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
     }
     float ms = 0;
     SENK_TRY(tm.stop(c, &ms));
-    (void)hipFree(r64);
-    (void)hipFree(r32);
-    (void)hipFree(d32);
-    (void)hipFree(dsc);
     res->iters = inner_total;
     res->inner_iters = inner_total;
     res->restarts = steps;
@@ -1115,6 +1182,105 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
     res->true_relres = relres;
     res->t_solve_ms = ms;
     res->status = status;
+    return SENK_OK;
+}
+
+// Nested Krylov: FP64 flexible GCR(outer_m) on A x = b whose preconditioner is an
+// inner FP32 Krylov solve (o.solver, restart o.m, o.inner_max iterations or relative
+// tolerance o.inner_eps) with the FP32 matrix and factors.  Outer step j:
+//   z_j = inner(r / ||r||),  q_j = A z_j (FP64),
+//   q_j, z_j orthogonalised against q_0..q_{j-1} (classical Gram-Schmidt, twice),
+//   alpha = (r, q_j)/(q_j, q_j),  x += alpha z_j,  r -= alpha q_j.
+// The outer basis keeps what every inner solve found, so unlike refinement nothing is
+// discarded between inner solves.  The FP64 residual is recomputed at every restart
+// and before convergence is reported.
+int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x,
+                 const senk_solve_opts &o, double nrm_b, senk_result *res, double *hist, int hist_cap) {
+    std::memset(res, 0, sizeof(*res));
+    const int N = A.N;
+    const int mo = o.outer_m > 0 ? o.outer_m : 20;
+    if (!A.v32 || mo > kRedMaxK) return SENK_ERR_ARG;
+    const size_t need = 2 * bytes_of<double>((size_t)mo * N) + bytes_of<double>(N) + 2 * bytes_of<float>(N) +
+                        bytes_of<double>(kRedMaxK) + bytes_of<double>(8);
+    void *wsp;
+    SENK_TRY(ctx_outer(c, need, &wsp));
+    Carver cv{(char *)wsp, 0, need};
+    double *Q = cv.take<double>((size_t)mo * N), *Z = cv.take<double>((size_t)mo * N);
+    double *r = cv.take<double>(N);
+    float *r32 = cv.take<float>(N), *d32 = cv.take<float>(N);
+    double *beta = cv.take<double>(kRedMaxK), *sc = cv.take<double>(8);
+    SolveTimer tm;
+    SENK_TRY(tm.start(c));
+    const SolveParams ip = inner_params(o);
+    const double tol = o.eps * nrm_b;
+    int inner_total = 0, outer_its = 0, cycles = 0, converged = 0, status = SENK_OK, hn = 0;
+    double rho = 0;
+    for (;;) {
+        int rg = 0;
+        SENK_TRY(residual<double>(c, A, x, b, r, &rg));
+        k_norm_fin<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, sc, sc + 1);
+        SENK_LAUNCH_CHECK();
+        double h2 = 0;
+        SENK_TRY(read_scalar(c, sc, &h2));
+        rho = std::sqrt(h2);
+        if (hist && hn < hist_cap) hist[hn++] = rho / nrm_b;
+        if (!std::isfinite(rho)) {
+            status = SENK_ERR_BREAKDOWN;
+            break;
+        }
+        if (rho <= tol) {
+            converged = 1;
+            break;
+        }
+        if (inner_total >= o.max_iter) break;
+        cycles++;
+        for (int j = 0; j < mo; j++) {
+            double *q = Q + (size_t)j * N, *z = Z + (size_t)j * N;
+            SENK_TRY(launch_convert<double, float>(c, N, r, r32, sc + 1));
+            SENK_HIP(hipMemsetAsync(d32, 0, sizeof(float) * N, c->stream));
+            senk_result ir;
+            SENK_TRY(krylov_solve<float>(c, A, M, r32, d32, ip, &ir, nullptr, 0));
+            inner_total += ir.iters;
+            SENK_TRY(launch_convert<float, double>(c, N, d32, z, nullptr));
+            SENK_TRY(spmv<double>(c, A, z, q));
+            for (int pass = 0; pass < 2 && j > 0; pass++) {
+                SENK_TRY(launch_multidot<double>(c, N, j, Q, (size_t)N, q, beta));
+                k_sub_lincomb2<<<ew_grid(N), kBlock, 0, c->stream>>>(N, j, beta, Q, Z, (size_t)N, q, z);
+                SENK_LAUNCH_CHECK();
+            }
+            SENK_TRY(launch_multidot<double>(c, N, 1, q, (size_t)N, q, sc + 2));
+            SENK_TRY(launch_multidot<double>(c, N, 1, q, (size_t)N, r, sc + 3));
+            k_gcr_outer_step<<<ew_grid(N), kBlock, 0, c->stream>>>(N, sc + 2, q, z, x, r);
+            SENK_LAUNCH_CHECK();
+            SENK_TRY(launch_multidot<double>(c, N, 1, r, (size_t)N, r, sc));
+            k_inv_sqrt<<<1, 64, 0, c->stream>>>(sc, sc + 1);
+            SENK_LAUNCH_CHECK();
+            SENK_TRY(read_scalar(c, sc, &h2));
+            rho = std::sqrt(h2);
+            outer_its++;
+            if (hist && hn < hist_cap) hist[hn++] = rho / nrm_b;
+            if (o.verbose) {
+                std::printf("outer %d inner %d %e\n", outer_its, inner_total, rho / nrm_b);
+                std::fflush(stdout);
+            }
+            if (!std::isfinite(rho) || rho <= tol || inner_total >= o.max_iter) break;
+        }
+        if (!std::isfinite(rho)) {
+            status = SENK_ERR_BREAKDOWN;
+            break;
+        }
+    }
+    float ms = 0;
+    SENK_TRY(tm.stop(c, &ms));
+    res->iters = inner_total;
+    res->inner_iters = inner_total;
+    res->restarts = outer_its;
+    res->converged = converged;
+    res->implicit_relres = rho / nrm_b;
+    res->true_relres = rho / nrm_b;
+    res->t_solve_ms = ms;
+    res->status = status;
+    (void)cycles;
     return SENK_OK;
 }
 

@@ -68,6 +68,10 @@ int krylov_solve(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
 int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x, const senk_solve_opts &o,
              double nrm_b, senk_result *res, double *hist, int hist_cap);
 
+// FP64 flexible GCR(o.outer_m) whose preconditioner is an FP32 inner Krylov solve.
+int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x,
+                 const senk_solve_opts &o, double nrm_b, senk_result *res, double *hist, int hist_cap);
+
 // ||b - A x|| / nrm_b in FP64 (synchronises).
 int true_relres(Ctx *c, const MatView &A, const double *b, const double *x, double nrm_b, double *out);
 

@@ -307,3 +307,41 @@ def test_reference_lines_printed(ctx, capfd):
     out = capfd.readouterr().out
     assert f"I iter {res.iters}\n" in out
     assert "# res " in out
+
+
+@pytest.mark.parametrize("solver", [S.SENK_GMRES, S.SENK_FGCR])
+def test_nested_mixed_precision(ctx, solver):
+    # FP64 flexible GCR outer iteration around FP32 inner Krylov solves
+    sy = _problem("vardiff", 20, 16, seed=3)
+    N = sy["N"]
+    A = C.c_void_p()
+    S.call("senk_mat_create", ctx.h, N, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]), 1 | 2, C.byref(A))
+    opts = S.senk_ilu_opts(S.SENK_ILU0, 0, 0, 0, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32))
+    M = C.c_void_p()
+    cp = sy["cptr"].ctypes.data_as(C.c_void_p)
+    S.call("senk_prec_create_ilu", ctx.h, A, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]), cp, sy["nc"], 16,
+           C.byref(opts), C.byref(M))
+    db = ctx.array(sy["b"])
+    dx64 = ctx.zeros(N)
+    r64 = S.senk_result()
+    o64 = S.solve_opts(solver=S.SENK_GMRES, m=30, eps=1e-14, max_iter=6000)
+    S.call("senk_solve", ctx.h, A, M, db.p, dx64.p, C.byref(o64), C.byref(r64), None, 0)
+    dx = ctx.zeros(N)
+    o = S.solve_opts(solver=solver, m=30, eps=1e-10, mixed=2, inner_eps=1e-3, inner_max=30, outer_m=20,
+                     max_iter=6000)
+    res = S.senk_result()
+    hist = np.zeros(256)
+    S.call("senk_solve", ctx.h, A, M, db.p, dx.p, C.byref(o), C.byref(res), O.P(hist), 256)
+    assert res.converged == 1 and res.status == 0
+    assert res.true_relres <= 1e-10
+    assert res.inner_iters == res.iters > 0
+    # each outer GCR step minimises ||r|| over the new direction, so the FP64 residual
+    # does not grow (recorded: start of each cycle, then every outer step; a restart
+    # records the true residual, which differs from the recursive one by rounding of
+    # order 1e-16 ||b||, i.e. up to ~1e-6 relative near 1e-10 ||b||)
+    h = hist[hist > 0]
+    assert len(h) >= res.restarts
+    assert np.all(h[1:] <= h[:-1] * (1 + 1e-4)), h
+    assert relerr(dx.download(), dx64.download()) <= 1e-7
+    S.lib().senk_prec_destroy(M)
+    S.lib().senk_mat_destroy(A)
