@@ -62,6 +62,15 @@ static int arena_get(DevBuf &a, size_t bytes, void **out) {
     return SENK_OK;
 }
 int ctx_work(Ctx *c, size_t bytes, void **out) { return arena_get(c->work, bytes, out); }
+int ctx_events(Ctx *c) {
+    if (c->ring) return SENK_OK;
+    for (int i = 0; i < 12; i++) {
+        const unsigned flags = i < 8 ? hipEventDisableTiming : hipEventDefault;
+        SENK_HIP(hipEventCreateWithFlags(&c->ev[i], flags));
+    }
+    SENK_HIP(hipHostMalloc((void **)&c->ring, sizeof(int) * 4 * 8, hipHostMallocDefault));
+    return SENK_OK;
+}
 int ctx_outer(Ctx *c, size_t bytes, void **out) { return arena_get(c->outer, bytes, out); }
 
 namespace {
@@ -195,6 +204,9 @@ int senk_ctx_destroy(senk_ctx ctx) {
     (void)hipStreamSynchronize(c.stream);
     if (c.work.p) (void)hipFree(c.work.p);
     if (c.outer.p) (void)hipFree(c.outer.p);
+    for (hipEvent_t e : c.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c.ring) (void)hipHostFree(c.ring);
     (void)hipFree(c.red_partials);
     (void)hipFree(c.dscalars);
     (void)hipHostFree(c.hpinned);
@@ -353,17 +365,20 @@ int senk_sellcs_free_device(senk_ctx ctx, senk_sellcs *d) {
     std::memset(d, 0, sizeof(*d));
     return SENK_OK;
 }
+static double sell_avg_width(const senk_sellcs &d) {
+    return d.nslices > 0 ? (double)d.nnz_padded / ((double)d.nslices * d.C) : 0.0;
+}
 int senk_spmv_sellcs_f64(senk_ctx ctx, const senk_sellcs *d, const double *dx, double *dy) {
     CTX_CHECK(ctx);
     if (!d || !d->val64) return SENK_ERR_ARG;
     return launch_spmv_sell<double>(&ctx->c, d->N, d->C, d->nslices, d->slice_ptr, d->cind, d->val64, d->perm, dx,
-                                    dy, false);
+                                    dy, false, nullptr, sell_avg_width(*d));
 }
 int senk_spmv_sellcs_f32(senk_ctx ctx, const senk_sellcs *d, const float *dx, float *dy) {
     CTX_CHECK(ctx);
     if (!d || !d->val32) return SENK_ERR_ARG;
     return launch_spmv_sell<float>(&ctx->c, d->N, d->C, d->nslices, d->slice_ptr, d->cind, d->val32, d->perm, dx,
-                                   dy, false);
+                                   dy, false, nullptr, sell_avg_width(*d));
 }
 
 // ---------------------------------------------------------------- SpTRSV

@@ -104,6 +104,53 @@ __device__ __forceinline__ double sum_partials(const double *p, int B) {
     return block_sum(s, sh);
 }
 
+// ----------------------------------------------------------------- 16-byte streaming
+// Streaming passes move 16 bytes per lane per access (a float4, or two double2): on
+// MI355X that streams at ~6.4 TB/s where 4- and 8-byte accesses reach 2.5-4.3 TB/s.
+// Each thread takes groups of four consecutive elements (SENK_FOR4); a group that runs
+// past n is loaded element by element with zero fill and stored masked, and so is every
+// group of a vector whose base is not 16-byte aligned (work vectors are aligned, and
+// the columns of a basis are stored at a stride that is a multiple of 4 elements, pad4).
+__host__ __device__ inline size_t pad4(size_t n) { return (n + 3) & ~(size_t)3; }
+__device__ __forceinline__ void ld4(const float *p, float (&o)[4]) {
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+__device__ __forceinline__ void ld4(const double *p, double (&o)[4]) {
+    const double2 a = *reinterpret_cast<const double2 *>(p), b = *reinterpret_cast<const double2 *>(p + 2);
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void st4(float *p, const float (&o)[4]) {
+    *reinterpret_cast<float4 *>(p) = make_float4(o[0], o[1], o[2], o[3]);
+}
+__device__ __forceinline__ void st4(double *p, const double (&o)[4]) {
+    *reinterpret_cast<double2 *>(p) = make_double2(o[0], o[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(o[2], o[3]);
+}
+__device__ __forceinline__ bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+template <typename T>
+__device__ __forceinline__ void get4(const T *p, long long i0, long long n, T (&o)[4]) {
+    if (i0 + 4 <= n && aligned16(p)) {
+        ld4(p + i0, o);
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) o[u] = i0 + u < n ? p[i0 + u] : T(0);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void put4(T *p, long long i0, long long n, const T (&o)[4]) {
+    if (i0 + 4 <= n && aligned16(p)) {
+        st4(p + i0, o);
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (i0 + u < n) p[i0 + u] = o[u];
+    }
+}
+#define SENK_FOR4(i0, n)                                                                        \
+    for (long long i0 = ((long long)blockIdx.x * kBlock + threadIdx.x) * 4; i0 < (long long)(n); \
+         i0 += (long long)gridDim.x * kBlock * 4)
+
 // Streaming passes: kUnroll elements per thread per trip, at most kRedMaxBlocks
 // workgroups (so a consumer sums at most that many partials).
 constexpr int kUnroll = 4;

@@ -55,7 +55,13 @@ struct Ctx {
     DevBuf work;                      // solver arena
     DevBuf outer;                     // arena of the FP64 outer loop around an inner solve
     int num_cus = 256;
+    // events and a pinned ring that solves reuse (created on first use): [0, 8) stop
+    // polling, [8, 10) inner solve timer, [10, 12) outer solve timer
+    hipEvent_t ev[12] = {};
+    int *ring = nullptr;
 };
+// Creates the context's reusable events and pinned ring on first use.
+int ctx_events(Ctx *c);
 
 int ctx_work(Ctx *c, size_t bytes, void **out);
 int ctx_outer(Ctx *c, size_t bytes, void **out);
@@ -110,7 +116,8 @@ int launch_spmv_bcsr(Ctx *c, int N, int bnl, int bnw, const int *brptr, const in
                      const T *x, T *y);
 template <typename T>
 int launch_spmv_sell(Ctx *c, int N, int C, int nslices, const int *wid, const int *cind, const T *val,
-                     const int *perm, const T *x, T *y, bool last_slice_short, const int *stop = nullptr);
+                     const int *perm, const T *x, T *y, bool last_slice_short, const int *stop = nullptr,
+                     double avg_width = 0);
 
 template <typename T>
 int launch_sptrsv(Ctx *c, const Schedule &s, bool upper, const int *rptr, const int *cind, const T *val,

@@ -99,7 +99,7 @@ int launch_spmv_csr_diag_f64(Ctx *c, int N, const int *rptr, const int *cind, co
 // a contiguous 64-element access.  Slice s has rows h = C (or N % C for a short last
 // slice in the reference's Csr2Sell layout) and slot j of row k at
 // wid[s]*C + j*h + k.  perm (optional) maps a packed row to its output row.
-template <typename T>
+template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices, const int *__restrict__ wid,
                                                       const int *__restrict__ cind, const T *__restrict__ val,
                                                       const int *__restrict__ perm, const T *__restrict__ x,
@@ -114,21 +114,26 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
     const size_t base = (size_t)wid[s] * C + k;
     const int w = wid[s + 1] - wid[s];
     T acc = 0;
-    // slot 0 starts the sum (as the reference's SpmvSell); later slots in order, with
-    // four independent gathers in flight per lane
-    for (int j0 = 0; j0 < w; j0 += 4) {
-        T xv[4], vv[4];
+    // slot 0 starts the sum (as the reference's SpmvSell), later slots in order.  The
+    // matrix streams through once: its loads are nontemporal so that the cache keeps
+    // x; CH slots of indices and values are loaded before their gathers are issued.
+    for (int j0 = 0; j0 < w; j0 += CH) {
+        int cc[CH];
+        T vv[CH], xv[CH];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < CH; u++) {
             const int j = j0 + u;
             if (j < w) {
                 const size_t o = base + (size_t)j * h;
-                vv[u] = val[o];
-                xv[u] = x[cind[o]];
+                cc[u] = __builtin_nontemporal_load(cind + o);
+                vv[u] = __builtin_nontemporal_load(val + o);
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < CH; u++)
+            if (j0 + u < w) xv[u] = x[cc[u]];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
             const int j = j0 + u;
             if (j == 0) acc = vv[u] * xv[u];
             else if (j < w) acc += vv[u] * xv[u];
@@ -139,18 +144,25 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
 }
 template <typename T>
 int launch_spmv_sell(Ctx *c, int N, int C, int nslices, const int *wid, const int *cind, const T *val,
-                     const int *perm, const T *x, T *y, bool last_short, const int *stop) {
+                     const int *perm, const T *x, T *y, bool last_short, const int *stop, double avg_width) {
     if (N <= 0 || nslices <= 0) return SENK_OK;
     long long threads = (long long)nslices * C;
     int grid = (int)((threads + kBlock - 1) / kBlock);
-    k_spmv_sell<T><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y, last_short ? 1 : 0, stop);
+    // short rows (7-point stencils) keep all their slots in one chunk of 8; longer rows
+    // take chunks of 16
+    if (avg_width > 0 && avg_width <= 8.5)
+        k_spmv_sell<T, 8><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y,
+                                                          last_short ? 1 : 0, stop);
+    else
+        k_spmv_sell<T, 16><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y,
+                                                           last_short ? 1 : 0, stop);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
 template int launch_spmv_sell<double>(Ctx *, int, int, int, const int *, const int *, const double *, const int *,
-                                      const double *, double *, bool, const int *);
+                                      const double *, double *, bool, const int *, double);
 template int launch_spmv_sell<float>(Ctx *, int, int, int, const int *, const int *, const float *, const int *,
-                                     const float *, float *, bool, const int *);
+                                     const float *, float *, bool, const int *, double);
 
 // =============================================================== SpMV BCSR
 // G lanes per block row; lane q takes blocks brptr[g]+q, +G, ...; each block is
@@ -791,40 +803,94 @@ inline int ew_grid(long long n) {
 #define SENK_GRID_STRIDE(i, n) \
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < (n); i += (long long)gridDim.x * kBlock)
 
+// Elementwise passes: four consecutive elements per thread, 16-byte accesses.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpby(int N, T a, const T *__restrict__ x, T b, T *y) {
-    SENK_GRID_STRIDE(i, N) y[i] = a * x[i] + b * y[i];
+    SENK_FOR4(i0, N) {
+        T xv[4], yv[4];
+        get4(x, i0, N, xv);
+        get4(y, i0, N, yv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) yv[u] = a * xv[u] + b * yv[u];
+        put4(y, i0, N, yv);
+    }
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy(int N, T a, const T *__restrict__ x, T *y) {
-    SENK_GRID_STRIDE(i, N) y[i] += a * x[i];
+    SENK_FOR4(i0, N) {
+        T xv[4], yv[4];
+        get4(x, i0, N, xv);
+        get4(y, i0, N, yv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) yv[u] += a * xv[u];
+        put4(y, i0, N, yv);
+    }
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpyz(int N, T a, const T *x, const T *y, T *z) {
-    SENK_GRID_STRIDE(i, N) z[i] = a * x[i] + y[i];
+    SENK_FOR4(i0, N) {
+        T xv[4], yv[4];
+        get4(x, i0, N, xv);
+        get4(y, i0, N, yv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) yv[u] = a * xv[u] + yv[u];
+        put4(z, i0, N, yv);
+    }
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scal(int N, T a, T *x) {
-    SENK_GRID_STRIDE(i, N) x[i] *= a;
+    SENK_FOR4(i0, N) {
+        T xv[4];
+        get4(x, i0, N, xv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) xv[u] *= a;
+        put4(x, i0, N, xv);
+    }
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_copy(int N, const T *__restrict__ x, T *__restrict__ y) {
-    SENK_GRID_STRIDE(i, N) y[i] = x[i];
+    SENK_FOR4(i0, N) {
+        T xv[4];
+        get4(x, i0, N, xv);
+        put4(y, i0, N, xv);
+    }
 }
 __global__ __launch_bounds__(kBlock) void k_hadprod(int N, const double *__restrict__ x, double *y) {
-    SENK_GRID_STRIDE(i, N) y[i] *= x[i];
+    SENK_FOR4(i0, N) {
+        double xv[4], yv[4];
+        get4(x, i0, N, xv);
+        get4(y, i0, N, yv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) yv[u] *= xv[u];
+        put4(y, i0, N, yv);
+    }
 }
 __global__ __launch_bounds__(kBlock) void k_haddiv(int N, const double *__restrict__ x, double *y) {
-    SENK_GRID_STRIDE(i, N) y[i] /= x[i];
+    SENK_FOR4(i0, N) {
+        double xv[4], yv[4];
+        get4(x, i0, N, xv);
+        get4(y, i0, N, yv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) yv[u] /= xv[u];
+        put4(y, i0, N, yv);
+    }
 }
 template <typename TI, typename TO>
 __global__ __launch_bounds__(kBlock) void k_convert(int N, const TI *__restrict__ in, TO *__restrict__ out,
                                                     const double *__restrict__ dscale) {
     const double sc = dscale ? *dscale : 1.0;
-    if (dscale) {
-        SENK_GRID_STRIDE(i, N) out[i] = (TO)((double)in[i] * sc);
-    } else {
-        SENK_GRID_STRIDE(i, N) out[i] = (TO)in[i];
+    SENK_FOR4(i0, N) {
+        TI a[4];
+        TO b[4];
+        get4(in, i0, N, a);
+        if (dscale) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) b[u] = (TO)((double)a[u] * sc);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; u++) b[u] = (TO)a[u];
+        }
+        put4(out, i0, N, b);
     }
 }
 
@@ -899,10 +965,18 @@ __global__ __launch_bounds__(kBlock) void k_multidot(int N, int K, const T *__re
     double acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0;
-    SENK_STREAM(i, N, {
-        const double yi = (double)y[i];
-        _Pragma("unroll") for (int k = 0; k < KMAX; k++) if (k < K) acc[k] += (double)xs[(size_t)k * ldx + i] * yi;
-    })
+    SENK_FOR4(i0, N) {
+        T yv[4];
+        get4(y, i0, N, yv);
+#pragma unroll
+        for (int k = 0; k < KMAX; k++)
+            if (k < K) {
+                T xv[4];
+                get4(xs + (size_t)k * ldx, i0, N, xv);
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[k] += (double)xv[u] * (double)yv[u];
+            }
+    }
     store_partials_n<KMAX>(acc, K, partials);
 }
 __global__ __launch_bounds__(kBlock) void k_partials_fin(const double *partials, int B, double *result) {

@@ -54,8 +54,14 @@ __global__ __launch_bounds__(kBlock) void k_scale_copy(int N, const T *__restric
                                                        const double *__restrict__ dscale, const int *stop) {
     if (stop && *stop) return;
     const double sc = dscale ? *dscale : 1.0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
-        out[i] = dscale ? (T)((double)in[i] * sc) : in[i];
+    SENK_FOR4(i0, N) {
+        T v[4];
+        get4(in, i0, N, v);
+        if (dscale)
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = (T)((double)v[u] * sc);
+        put4(out, i0, N, v);
+    }
 }
 
 // Residual y = b - A x (x == nullptr means x = 0), G lanes per row, with the
@@ -102,10 +108,16 @@ __global__ __launch_bounds__(kBlock) void k_norm_fin(const double *P, int B, dou
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sub_norm(int N, const T *__restrict__ b, T *y, double *partials) {
     double ss = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
-        const T r = (T)1 * b[i] + (T)(-1) * y[i];
-        y[i] = r;
-        ss += (double)r * (double)r;
+    SENK_FOR4(i0, N) {
+        T bv[4], yv[4];
+        get4(b, i0, N, bv);
+        get4(y, i0, N, yv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            yv[u] = (T)1 * bv[u] + (T)(-1) * yv[u];
+            ss += (double)yv[u] * (double)yv[u];
+        }
+        put4(y, i0, N, yv);
     }
     const double v[1] = {ss};
     store_partials<1>(v, partials);
@@ -139,21 +151,17 @@ struct Poller {
     static constexpr int kRing = 8;
     Ctx *c;
     const int *dstate;
-    int *hring = nullptr;  // pinned, kRing * 4 ints
-    hipEvent_t ev[kRing] = {};
+    int *hring = nullptr;  // the context's pinned ring, kRing * 4 ints
+    hipEvent_t *ev = nullptr;
     long long issued = 0, checked = 0;
     bool stopped = false;
     int init(Ctx *cc, const int *ds) {
         c = cc;
         dstate = ds;
-        SENK_HIP(hipHostMalloc((void **)&hring, sizeof(int) * 4 * kRing, hipHostMallocDefault));
-        for (int i = 0; i < kRing; i++) SENK_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+        SENK_TRY(ctx_events(c));
+        hring = c->ring;
+        ev = c->ev;
         return SENK_OK;
-    }
-    ~Poller() {
-        for (int i = 0; i < kRing; i++)
-            if (ev[i]) (void)hipEventDestroy(ev[i]);
-        if (hring) (void)hipHostFree(hring);
     }
     void reset() { issued = checked = 0; stopped = false; }
     int mark() {
@@ -174,6 +182,7 @@ struct Poller {
         *stop = stopped;
         return SENK_OK;
     }
+    // A ring slot is reused only after its mark was checked: callers poll with lag < kRing.
 };
 constexpr int kLag = 2;
 
@@ -194,11 +203,14 @@ size_t bytes_of(size_t n) {
     return ((sizeof(T) * n + 255) & ~(size_t)255) + 256;
 }
 
+// Device time of a solve from two of the context's events; `outer` selects the pair
+// kept for a loop that runs inner solves (which use the other pair).
 struct SolveTimer {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int start(Ctx *c) {
-        SENK_HIP(hipEventCreate(&e0));
-        SENK_HIP(hipEventCreate(&e1));
+    int start(Ctx *c, bool outer = false) {
+        SENK_TRY(ctx_events(c));
+        e0 = c->ev[outer ? 10 : 8];
+        e1 = c->ev[outer ? 11 : 9];
         SENK_HIP(hipEventRecord(e0, c->stream));
         return SENK_OK;
     }
@@ -207,10 +219,6 @@ struct SolveTimer {
         SENK_HIP(hipEventSynchronize(e1));
         SENK_HIP(hipEventElapsedTime(ms, e0, e1));
         return SENK_OK;
-    }
-    ~SolveTimer() {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
     }
 };
 
@@ -245,14 +253,26 @@ __global__ __launch_bounds__(kBlock) void k_mgs(int N, int k, int j, const T *__
     const T *Vprev = k > 0 ? V + (size_t)(k - 1) * ld : nullptr;
     const T *Vk = k <= j ? V + (size_t)k * ld : nullptr;
     double acc = 0;
-    SENK_STREAM(i, N, {
-        T wi = w[i];
+    SENK_FOR4(i0, N) {
+        T wv[4];
+        get4(w, i0, N, wv);
         if (Vprev) {
-            wi = wi - coef * Vprev[i];
-            w[i] = wi;
+            T pv[4];
+            get4(Vprev, i0, N, pv);
+#pragma unroll
+            for (int u = 0; u < 4; u++) wv[u] = wv[u] - coef * pv[u];
+            put4(w, i0, N, wv);
         }
-        acc += Vk ? (double)Vk[i] * (double)wi : (double)wi * (double)wi;
-    })
+        if (Vk) {
+            T kv[4];
+            get4(Vk, i0, N, kv);
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc += (double)kv[u] * (double)wv[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc += (double)wv[u] * (double)wv[u];
+        }
+    }
     const double v[1] = {acc};
     store_partials<1>(v, Pout);
 }
@@ -272,25 +292,55 @@ __global__ __launch_bounds__(kBlock) void k_cgs(int N, int K, const T *__restric
     double acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
-        T wi = w[i];
+    SENK_FOR4(i0, N) {
+        T wv[4];
+        get4(w, i0, N, wv);
         if (coef) {
-            double s = (double)wi;
+            double sv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) sv[u] = (double)wv[u];
 #pragma unroll
             for (int k = 0; k < KMAX; k++)
-                if (k < K) s -= cf[k] * (double)V[(size_t)k * ld + i];
-            wi = (T)s;
-            w[i] = wi;
+                if (k < K) {
+                    T vk[4];
+                    get4(V + (size_t)k * ld, i0, N, vk);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) sv[u] -= cf[k] * (double)vk[u];
+                }
+#pragma unroll
+            for (int u = 0; u < 4; u++) wv[u] = (T)sv[u];
+            put4(w, i0, N, wv);
         }
         if (mode == 0) {
 #pragma unroll
             for (int k = 0; k < KMAX; k++)
-                if (k < K) acc[k] += (double)V[(size_t)k * ld + i] * (double)wi;
+                if (k < K) {
+                    T vk[4];
+                    get4(V + (size_t)k * ld, i0, N, vk);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) acc[k] += (double)vk[u] * (double)wv[u];
+                }
         } else {
-            acc[0] += (double)wi * (double)wi;
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[0] += (double)wv[u] * (double)wv[u];
         }
     }
     store_partials_n<KMAX>(acc, mode == 0 ? K : 1, Pout);
+}
+// k_cgs with its accumulator count sized to K
+template <typename T>
+void launch_cgs(Ctx *c, int B, int N, int K, const T *V, size_t ld, const double *sig, const double *coef, T *w,
+                int mode, double *Pout, const int *stop) {
+    if (K <= 8)
+        k_cgs<T, 8><<<B, kBlock, 0, c->stream>>>(N, K, V, ld, sig, coef, w, mode, Pout, stop);
+    else if (K <= 16)
+        k_cgs<T, 16><<<B, kBlock, 0, c->stream>>>(N, K, V, ld, sig, coef, w, mode, Pout, stop);
+    else if (K <= 24)
+        k_cgs<T, 24><<<B, kBlock, 0, c->stream>>>(N, K, V, ld, sig, coef, w, mode, Pout, stop);
+    else if (K <= 32)
+        k_cgs<T, 32><<<B, kBlock, 0, c->stream>>>(N, K, V, ld, sig, coef, w, mode, Pout, stop);
+    else
+        k_cgs<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, K, V, ld, sig, coef, w, mode, Pout, stop);
 }
 // out[k] = sig[k] * sum(P_k), one workgroup per k
 __global__ __launch_bounds__(kBlock) void k_cgs_fin(const double *P, int B, const double *sig, double *out,
@@ -362,15 +412,30 @@ __global__ __launch_bounds__(kBlock) void k_combine(int N, const T *__restrict__
                                                     const double *__restrict__ coef, const int *state, T *out,
                                                     int accumulate) {
     const int J = state[1];
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
-        double acc = accumulate ? (double)out[i] : 0.0;
+    SENK_FOR4(i0, N) {
+        double acc[4];
+        T ov[4];
+        get4(out, i0, N, ov);
+        int k0 = 0;
         if (!accumulate && J > 0) {
-            acc = coef[0] * (double)V[i];
-            for (int k = 1; k < J; k++) acc += coef[k] * (double)V[(size_t)k * ld + i];
+            T vk[4];
+            get4(V, i0, N, vk);
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[u] = coef[0] * (double)vk[u];
+            k0 = 1;
         } else {
-            for (int k = 0; k < J; k++) acc += coef[k] * (double)V[(size_t)k * ld + i];
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[u] = accumulate ? (double)ov[u] : 0.0;
         }
-        out[i] = (T)acc;
+        for (int k = k0; k < J; k++) {
+            T vk[4];
+            get4(V + (size_t)k * ld, i0, N, vk);
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[u] += coef[k] * (double)vk[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) ov[u] = (T)acc[u];
+        put4(out, i0, N, ov);
     }
 }
 
@@ -383,7 +448,8 @@ int spmv(Ctx *c, const MatView &A, const T *x, T *y, const int *stop) {
         const T *sv = svals<T>(*A.sell);
         if (sv)
             return launch_spmv_sell<T>(c, A.N, A.sell->C, A.sell->nslices, A.sell->slice_ptr, A.sell->cind, sv,
-                                       A.sell->perm, x, y, false, stop);
+                                       A.sell->perm, x, y, false, stop,
+                                       (double)A.sell->nnz_padded / ((double)A.sell->nslices * A.sell->C));
     }
     if (A.bnl) {
         const T *bv = bvals<T>(A);
@@ -465,16 +531,17 @@ template <typename T>
 static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
                  senk_result *res, double *hist, int hist_cap) {
     const int N = A.N, m = p.m;
+    const size_t ld = pad4(N);  // basis stride: every column 16-byte aligned
     const bool flexible = p.solver == SENK_FGMRES;
     const int hcap = p.outer * m;
-    size_t need = bytes_of<T>((size_t)(m + 1) * N) + (flexible ? bytes_of<T>((size_t)m * N) : 0) +
+    size_t need = bytes_of<T>((size_t)(m + 1) * ld) + (flexible ? bytes_of<T>((size_t)m * ld) : 0) +
                   3 * bytes_of<T>(N) + bytes_of<double>((size_t)(m + 1) * m) + 7 * bytes_of<double>(m + 2) +
                   bytes_of<double>(hcap + 1) + bytes_of<int>(8);
     void *wsp;
     SENK_TRY(ctx_work(c, need, &wsp));
     Carver cv{(char *)wsp, 0, need};
-    T *V = cv.take<T>((size_t)(m + 1) * N);
-    T *Z = flexible ? cv.take<T>((size_t)m * N) : nullptr;
+    T *V = cv.take<T>((size_t)(m + 1) * ld);
+    T *Z = flexible ? cv.take<T>((size_t)m * ld) : nullptr;
     T *t = cv.take<T>(N), *u = cv.take<T>(N), *tmp = cv.take<T>(N);
     double *H = cv.take<double>((size_t)(m + 1) * m);
     double *cs = cv.take<double>(m + 2), *sn = cv.take<double>(m + 2), *e = cv.take<double>(m + 2);
@@ -503,8 +570,8 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
         poll.reset();
         bool stop = false;
         for (int j = 0; j < m && !stop; j++) {
-            T *vj = V + (size_t)j * N, *w = V + (size_t)(j + 1) * N;
-            T *z = flexible ? Z + (size_t)j * N : t;
+            T *vj = V + (size_t)j * ld, *w = V + (size_t)(j + 1) * ld;
+            T *z = flexible ? Z + (size_t)j * ld : t;
             // z = M^-1 (sig_j V_j);  w = A z
             SENK_TRY(prec_apply<T>(c, M, vj, sig + j, z, tmp, state));
             SENK_TRY(spmv<T>(c, A, z, w, state));
@@ -514,18 +581,15 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
             if (use_cgs && j + 1 <= kRedMaxK) {
                 const int K = j + 1;
                 // h1 = V^T w;  w -= V h1, h2 = V^T w;  w -= V h2, ||w||;  H[:,j] = h1 + h2
-                k_cgs<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, K, V, N, sig, nullptr, w, 0, red_slot(c, kSlotA),
-                                                                state);
+                launch_cgs<T>(c, B, N, K, V, ld, sig, nullptr, w, 0, red_slot(c, kSlotA), state);
                 SENK_LAUNCH_CHECK();
                 k_cgs_fin<<<K, kBlock, 0, c->stream>>>(red_slot(c, kSlotA), B, sig, hc, state);
                 SENK_LAUNCH_CHECK();
-                k_cgs<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, K, V, N, sig, hc, w, 0, red_slot(c, kSlotB),
-                                                                state);
+                launch_cgs<T>(c, B, N, K, V, ld, sig, hc, w, 0, red_slot(c, kSlotB), state);
                 SENK_LAUNCH_CHECK();
                 k_cgs_fin<<<K, kBlock, 0, c->stream>>>(red_slot(c, kSlotB), B, sig, h2, state);
                 SENK_LAUNCH_CHECK();
-                k_cgs<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, K, V, N, sig, h2, w, 1, red_slot(c, kSlotC),
-                                                                state);
+                launch_cgs<T>(c, B, N, K, V, ld, sig, h2, w, 1, red_slot(c, kSlotC), state);
                 SENK_LAUNCH_CHECK();
                 Pnorm = red_slot(c, kSlotC);
                 add_h = h2;
@@ -533,7 +597,7 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
                 for (int k = 0; k <= j + 1; k++) {
                     const double *Pin = red_slot(c, (k + 1) % 2 ? kSlotMgsB : kSlotMgsA);
                     double *Pout = red_slot(c, k % 2 ? kSlotMgsB : kSlotMgsA);
-                    k_mgs<T><<<B, kBlock, 0, c->stream>>>(N, k, j, V, N, sig, hc, w, Pin, B, Pout, state);
+                    k_mgs<T><<<B, kBlock, 0, c->stream>>>(N, k, j, V, ld, sig, hc, w, Pin, B, Pout, state);
                     SENK_LAUNCH_CHECK();
                 }
                 Pnorm = red_slot(c, (j + 1) % 2 ? kSlotMgsB : kSlotMgsA);
@@ -548,13 +612,13 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
         k_gmres_y<<<1, 64, 0, c->stream>>>(m, H, e, sig, y, coef, state, flexible ? 0 : 1);
         SENK_LAUNCH_CHECK();
         if (flexible) {
-            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, Z, N, coef, state, x, 1);
+            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, Z, ld, coef, state, x, 1);
             SENK_LAUNCH_CHECK();
         } else if (!prec) {
-            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, V, N, coef, state, x, 1);
+            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, V, ld, coef, state, x, 1);
             SENK_LAUNCH_CHECK();
         } else {
-            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, V, N, coef, state, u, 0);
+            k_combine<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, V, ld, coef, state, u, 0);
             SENK_LAUNCH_CHECK();
             SENK_TRY(prec_apply<T>(c, M, u, nullptr, t, tmp, nullptr));
             SENK_TRY(launch_axpby<T>(c, N, 1.0, t, 1.0, x));
@@ -604,11 +668,16 @@ __global__ __launch_bounds__(kBlock) void k_gcr_alpha(int N, const T *__restrict
                                                       double *Pout, const int *stop) {
     if (*stop) return;
     double a0 = 0, a1 = 0;
-    SENK_STREAM(i, N, {
-        const double ap = (double)Apj[i];
-        a0 += ap * ap;
-        a1 += ap * (double)r[i];
-    })
+    SENK_FOR4(i0, N) {
+        T av[4], rv[4];
+        get4(Apj, i0, N, av);
+        get4(r, i0, N, rv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a0 += (double)av[u] * (double)av[u];
+            a1 += (double)av[u] * (double)rv[u];
+        }
+    }
     const double v[2] = {a0, a1};
     store_partials<2>(v, Pout);
 }
@@ -622,12 +691,21 @@ __global__ __launch_bounds__(kBlock) void k_gcr_xr(int N, const T *__restrict__ 
     if (blockIdx.x == 0 && threadIdx.x == 0) *dotApj = d0;
     const T a = (T)(d1 / d0);
     double ss = 0;
-    SENK_STREAM(i, N, {
-        x[i] += a * pj[i];
-        const T ri = r[i] + (-a) * Apj[i];
-        r[i] = ri;
-        ss += (double)ri * (double)ri;
-    })
+    SENK_FOR4(i0, N) {
+        T xv[4], pv[4], rv[4], av[4];
+        get4(x, i0, N, xv);
+        get4(pj, i0, N, pv);
+        get4(r, i0, N, rv);
+        get4(Apj, i0, N, av);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            xv[u] += a * pv[u];
+            rv[u] = rv[u] + (-a) * av[u];
+            ss += (double)rv[u] * (double)rv[u];
+        }
+        put4(x, i0, N, xv);
+        put4(r, i0, N, rv);
+    }
     const double v[1] = {ss};
     store_partials<1>(v, Pout);
 }
@@ -654,11 +732,17 @@ __global__ __launch_bounds__(kBlock) void k_gcr_betas(int N, int K, const T *__r
     double acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
-        const double a = (double)AKr[i];
+    SENK_FOR4(i0, N) {
+        T a[4];
+        get4(AKr, i0, N, a);
 #pragma unroll
         for (int k = 0; k < KMAX; k++)
-            if (k < K) acc[k] += (double)Ap[(size_t)k * ld + i] * a;
+            if (k < K) {
+                T v[4];
+                get4(Ap + (size_t)k * ld, i0, N, v);
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[k] += (double)v[u] * (double)a[u];
+            }
     }
     store_partials_n<KMAX>(acc, K, Pout);
 }
@@ -670,21 +754,46 @@ __global__ __launch_bounds__(kBlock) void k_gcr_beta_fin(const double *P, int B,
     const double s = sum_partials(P + (size_t)k * kPartLd, B);
     if (threadIdx.x == 0) beta[k] = -s / dotAp[k];
 }
-// p_{j+1} = beta_0 p_0 + Kr + sum_{k>=1} beta_k p_k, and the same for Ap  (:127-133)
+// p_{j+1} = beta_0 p_0 + Kr + sum_{k>=1} beta_k p_k, and the same for Ap  (:127-133).
+// Also stores the partials of (Ap_{j+1}, Ap_{j+1}) and (Ap_{j+1}, r) that the next
+// step's alpha needs (k_gcr_alpha's output), so that pass is not repeated.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gcr_newdir(int N, int K, T *p, T *Ap, size_t ld, const T *Kr,
-                                                       const T *AKr, const double *beta, const int *stop) {
+                                                       const T *AKr, const double *beta, const T *__restrict__ r,
+                                                       double *Palpha, const int *stop) {
     if (*stop) return;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
-        T a = (T)beta[0] * p[i] + Kr[i];
-        T b = (T)beta[0] * Ap[i] + AKr[i];
+    double a0 = 0, a1 = 0;
+    SENK_FOR4(i0, N) {
+        T a[4], b[4], t[4];
+        get4(p, i0, N, a);
+        get4(Kr, i0, N, t);
+        const T b0 = (T)beta[0];
+#pragma unroll
+        for (int u = 0; u < 4; u++) a[u] = b0 * a[u] + t[u];
+        get4(Ap, i0, N, b);
+        get4(AKr, i0, N, t);
+#pragma unroll
+        for (int u = 0; u < 4; u++) b[u] = b0 * b[u] + t[u];
         for (int k = 1; k < K; k++) {
-            a += (T)beta[k] * p[(size_t)k * ld + i];
-            b += (T)beta[k] * Ap[(size_t)k * ld + i];
+            const T bk = (T)beta[k];
+            get4(p + (size_t)k * ld, i0, N, t);
+#pragma unroll
+            for (int u = 0; u < 4; u++) a[u] += bk * t[u];
+            get4(Ap + (size_t)k * ld, i0, N, t);
+#pragma unroll
+            for (int u = 0; u < 4; u++) b[u] += bk * t[u];
         }
-        p[(size_t)K * ld + i] = a;
-        Ap[(size_t)K * ld + i] = b;
+        put4(p + (size_t)K * ld, i0, N, a);
+        put4(Ap + (size_t)K * ld, i0, N, b);
+        get4(r, i0, N, t);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a0 += (double)b[u] * (double)b[u];
+            a1 += (double)b[u] * (double)t[u];
+        }
     }
+    const double v[2] = {a0, a1};
+    store_partials<2>(v, Palpha);
 }
 __global__ __launch_bounds__(kBlock) void k_gcr_restart(const double *P, int B, double tol, double nrm_b, int *state,
                                                         double *hist, int hidx, int hcap) {
@@ -707,15 +816,16 @@ template <typename T>
 static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
                senk_result *res, double *hist, int hist_cap) {
     const int N = A.N, m = p.m;
+    const size_t ld = pad4(N);  // direction stride: every column 16-byte aligned
     if (m > kRedMaxK) return SENK_ERR_UNSUPPORTED;
     const bool prec = M.kind != 0 || M.learned;
     const int hcap = p.check_every ? p.outer * (m + 1) + 1 : p.outer + 1;
-    size_t need = 2 * bytes_of<T>((size_t)(m + 1) * N) + 4 * bytes_of<T>(N) + 3 * bytes_of<double>(m + 2) +
+    size_t need = 2 * bytes_of<T>((size_t)(m + 1) * ld) + 4 * bytes_of<T>(N) + 3 * bytes_of<double>(m + 2) +
                   bytes_of<double>(hcap + 1) + bytes_of<int>(8);
     void *wsp;
     SENK_TRY(ctx_work(c, need, &wsp));
     Carver cv{(char *)wsp, 0, need};
-    T *P = cv.take<T>((size_t)(m + 1) * N), *AP = cv.take<T>((size_t)(m + 1) * N);
+    T *P = cv.take<T>((size_t)(m + 1) * ld), *AP = cv.take<T>((size_t)(m + 1) * ld);
     T *r = cv.take<T>(N), *Kr = cv.take<T>(N), *AKr = cv.take<T>(N), *tmp = cv.take<T>(N);
     double *dotAp = cv.take<double>(m + 2), *beta = cv.take<double>(m + 2);
     double *dhist = cv.take<double>(hcap + 1);
@@ -750,9 +860,11 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
         poll.reset();
         bool stop = false;
         for (int j = 0; j < m && !stop; j++) {
-            T *pj = P + (size_t)j * N, *Apj = AP + (size_t)j * N;
-            k_gcr_alpha<T><<<B, kBlock, 0, c->stream>>>(N, Apj, r, red_slot(c, kSlotA), state);
-            SENK_LAUNCH_CHECK();
+            T *pj = P + (size_t)j * ld, *Apj = AP + (size_t)j * ld;
+            if (j == 0) {  // later steps get these partials from k_gcr_newdir
+                k_gcr_alpha<T><<<B, kBlock, 0, c->stream>>>(N, Apj, r, red_slot(c, kSlotA), state);
+                SENK_LAUNCH_CHECK();
+            }
             k_gcr_xr<T><<<B, kBlock, 0, c->stream>>>(N, pj, Apj, red_slot(c, kSlotA), B, dotAp + j, x, r,
                                                      red_slot(c, kSlotC), state);
             SENK_LAUNCH_CHECK();
@@ -767,11 +879,21 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
                 z = Kr;
             }
             SENK_TRY(spmv<T>(c, A, z, AKr, state));
-            k_gcr_betas<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, j + 1, AP, N, AKr, red_slot(c, kSlotB), state);
+            const int K = j + 1;
+            if (K <= 8)
+                k_gcr_betas<T, 8><<<B, kBlock, 0, c->stream>>>(N, K, AP, ld, AKr, red_slot(c, kSlotB), state);
+            else if (K <= 16)
+                k_gcr_betas<T, 16><<<B, kBlock, 0, c->stream>>>(N, K, AP, ld, AKr, red_slot(c, kSlotB), state);
+            else if (K <= 24)
+                k_gcr_betas<T, 24><<<B, kBlock, 0, c->stream>>>(N, K, AP, ld, AKr, red_slot(c, kSlotB), state);
+            else if (K <= 32)
+                k_gcr_betas<T, 32><<<B, kBlock, 0, c->stream>>>(N, K, AP, ld, AKr, red_slot(c, kSlotB), state);
+            else
+                k_gcr_betas<T, kRedMaxK><<<B, kBlock, 0, c->stream>>>(N, K, AP, ld, AKr, red_slot(c, kSlotB), state);
             SENK_LAUNCH_CHECK();
-            k_gcr_beta_fin<<<j + 1, kBlock, 0, c->stream>>>(red_slot(c, kSlotB), B, dotAp, beta, state);
+            k_gcr_beta_fin<<<K, kBlock, 0, c->stream>>>(red_slot(c, kSlotB), B, dotAp, beta, state);
             SENK_LAUNCH_CHECK();
-            k_gcr_newdir<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, j + 1, P, AP, N, z, AKr, beta, state);
+            k_gcr_newdir<T><<<B, kBlock, 0, c->stream>>>(N, K, P, AP, ld, z, AKr, beta, r, red_slot(c, kSlotA), state);
             SENK_LAUNCH_CHECK();
             if (p.check_every) {
                 SENK_TRY(poll.mark());
@@ -827,7 +949,13 @@ __global__ __launch_bounds__(kBlock) void k_bicg_alpha(int N, const T *__restric
                                                        double *Pout, const int *stop) {
     if (*stop) return;
     double a = 0;
-    SENK_STREAM(i, N, { a += (double)AKp[i] * (double)rs[i]; })
+    SENK_FOR4(i0, N) {
+        T x[4], y[4];
+        get4(AKp, i0, N, x);
+        get4(rs, i0, N, y);
+#pragma unroll
+        for (int u = 0; u < 4; u++) a += (double)x[u] * (double)y[u];
+    }
     const double v[1] = {a};
     store_partials<1>(v, Pout);
 }
@@ -840,7 +968,14 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(int N, const T *__restrict__ 
     const double alpha = *rr / sum_partials(Pin, B);
     if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_out = alpha;
     const T a = (T)(-alpha);
-    SENK_STREAM(i, N, { s[i] = a * AKp[i] + r[i]; })
+    SENK_FOR4(i0, N) {
+        T x[4], y[4];
+        get4(AKp, i0, N, x);
+        get4(r, i0, N, y);
+#pragma unroll
+        for (int u = 0; u < 4; u++) y[u] = a * x[u] + y[u];
+        put4(s, i0, N, y);
+    }
 }
 // partials of (AKs, s) and (AKs, AKs)                            (:145)
 template <typename T>
@@ -848,11 +983,16 @@ __global__ __launch_bounds__(kBlock) void k_bicg_omega(int N, const T *__restric
                                                        double *Pout, const int *stop) {
     if (*stop) return;
     double a0 = 0, a1 = 0;
-    SENK_STREAM(i, N, {
-        const double v = (double)AKs[i];
-        a0 += v * (double)s[i];
-        a1 += v * v;
-    })
+    SENK_FOR4(i0, N) {
+        T x[4], y[4];
+        get4(AKs, i0, N, x);
+        get4(s, i0, N, y);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a0 += (double)x[u] * (double)y[u];
+            a1 += (double)x[u] * (double)x[u];
+        }
+    }
     const double v[2] = {a0, a1};
     store_partials<2>(v, Pout);
 }
@@ -868,15 +1008,28 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(int N, const T *__restri
     if (blockIdx.x == 0 && threadIdx.x == 0) sc[1] = omega;
     const T a = (T)sc[0], w = (T)omega;
     double ss = 0, rr = 0;
-    SENK_STREAM(i, N, {
-        T xi = x[i] + a * Kp[i];
-        xi = xi + w * Ks[i];
-        x[i] = xi;
-        const T ri = (-w) * AKs[i] + s[i];
-        r[i] = ri;
-        ss += (double)ri * (double)ri;
-        rr += (double)ri * (double)rs[i];
-    })
+    SENK_FOR4(i0, N) {
+        T xv[4], t[4], rv[4];
+        get4(x, i0, N, xv);
+        get4(Kp, i0, N, t);
+#pragma unroll
+        for (int u = 0; u < 4; u++) xv[u] = xv[u] + a * t[u];
+        get4(Ks, i0, N, t);
+#pragma unroll
+        for (int u = 0; u < 4; u++) xv[u] = xv[u] + w * t[u];
+        put4(x, i0, N, xv);
+        get4(AKs, i0, N, t);
+        get4(s, i0, N, rv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) rv[u] = (-w) * t[u] + rv[u];
+        put4(r, i0, N, rv);
+        get4(rs, i0, N, t);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            ss += (double)rv[u] * (double)rv[u];
+            rr += (double)rv[u] * (double)t[u];
+        }
+    }
     const double v[2] = {ss, rr};
     store_partials<2>(v, Pout);
 }
@@ -907,10 +1060,18 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(int N, const T *__restrict__ 
     const double rr_old = sc[2 + parity];
     const double beta = sc[0] / sc[1] * rr_new / rr_old;
     const T w = (T)(-sc[1]), be = (T)beta;
-    SENK_STREAM(i, N, {
-        const T tmp = w * AKp[i] + p[i];
-        p[i] = be * tmp + r[i];
-    })
+    SENK_FOR4(i0, N) {
+        T a[4], pv[4], rv[4];
+        get4(AKp, i0, N, a);
+        get4(p, i0, N, pv);
+        get4(r, i0, N, rv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const T tmp = w * a[u] + pv[u];
+            pv[u] = be * tmp + rv[u];
+        }
+        put4(p, i0, N, pv);
+    }
 }
 __global__ void k_state_reset(int *state) {
     if (threadIdx.x == 0) {
@@ -1041,8 +1202,15 @@ namespace {
 __global__ __launch_bounds__(kBlock) void k_ir_update(int N, const float *__restrict__ d, const double *nrm2,
                                                       double *x) {
     const double s = sqrt(*nrm2);
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
-        x[i] += s * (double)d[i];
+    SENK_FOR4(i0, N) {
+        float dv[4];
+        double xv[4];
+        get4(d, i0, N, dv);
+        get4(x, i0, N, xv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) xv[u] += s * (double)dv[u];
+        put4(x, i0, N, xv);
+    }
 }
 }  // namespace
 
@@ -1071,15 +1239,21 @@ namespace {
 __global__ __launch_bounds__(kBlock) void k_sub_lincomb2(int N, int K, const double *__restrict__ coef,
                                                          const double *__restrict__ Q, const double *__restrict__ Z,
                                                          size_t ld, double *q, double *z) {
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
-        double sq = q[i], sz = z[i];
+    SENK_FOR4(i0, N) {
+        double sq[4], sz[4], t[4];
+        get4(q, i0, N, sq);
+        get4(z, i0, N, sz);
         for (int k = 0; k < K; k++) {
             const double a = coef[k];
-            sq -= a * Q[(size_t)k * ld + i];
-            sz -= a * Z[(size_t)k * ld + i];
+            get4(Q + (size_t)k * ld, i0, N, t);
+#pragma unroll
+            for (int u = 0; u < 4; u++) sq[u] -= a * t[u];
+            get4(Z + (size_t)k * ld, i0, N, t);
+#pragma unroll
+            for (int u = 0; u < 4; u++) sz[u] -= a * t[u];
         }
-        q[i] = sq;
-        z[i] = sz;
+        put4(q, i0, N, sq);
+        put4(z, i0, N, sz);
     }
 }
 // s = {(q,q), (r,q)}: alpha = s1/s0, x += alpha z, r -= alpha q, then q and z are scaled
@@ -1087,12 +1261,25 @@ __global__ __launch_bounds__(kBlock) void k_sub_lincomb2(int N, int K, const dou
 __global__ __launch_bounds__(kBlock) void k_gcr_outer_step(int N, const double *__restrict__ s, double *q,
                                                            double *z, double *x, double *r) {
     const double alpha = s[1] / s[0], inv = 1.0 / sqrt(s[0]);
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
-        const double qi = q[i], zi = z[i];
-        x[i] += alpha * zi;
-        r[i] -= alpha * qi;
-        q[i] = qi * inv;
-        z[i] = zi * inv;
+    SENK_FOR4(i0, N) {
+        double qv[4], zv[4], t[4];
+        get4(q, i0, N, qv);
+        get4(z, i0, N, zv);
+        get4(x, i0, N, t);
+#pragma unroll
+        for (int u = 0; u < 4; u++) t[u] += alpha * zv[u];
+        put4(x, i0, N, t);
+        get4(r, i0, N, t);
+#pragma unroll
+        for (int u = 0; u < 4; u++) t[u] -= alpha * qv[u];
+        put4(r, i0, N, t);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            qv[u] *= inv;
+            zv[u] *= inv;
+        }
+        put4(q, i0, N, qv);
+        put4(z, i0, N, zv);
     }
 }
 __global__ void k_inv_sqrt(const double *s, double *inv) {
@@ -1136,7 +1323,7 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
     float *r32 = cv.take<float>(N), *d32 = cv.take<float>(N);
     double *dsc = cv.take<double>(4);
     SolveTimer tm;
-    SENK_TRY(tm.start(c));
+    SENK_TRY(tm.start(c, true));
     const SolveParams ip = inner_params(o);
     const int max_steps = std::max(1, (o.max_iter + o.inner_max - 1) / std::max(1, o.inner_max)) + 8;
     int steps = 0, inner_total = 0, converged = 0, status = SENK_OK;
@@ -1199,18 +1386,19 @@ int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, d
     std::memset(res, 0, sizeof(*res));
     const int N = A.N;
     const int mo = o.outer_m > 0 ? o.outer_m : 20;
+    const size_t ld = pad4(N);
     if (!A.v32 || mo > kRedMaxK) return SENK_ERR_ARG;
-    const size_t need = 2 * bytes_of<double>((size_t)mo * N) + bytes_of<double>(N) + 2 * bytes_of<float>(N) +
+    const size_t need = 2 * bytes_of<double>((size_t)mo * ld) + bytes_of<double>(N) + 2 * bytes_of<float>(N) +
                         bytes_of<double>(kRedMaxK) + bytes_of<double>(8);
     void *wsp;
     SENK_TRY(ctx_outer(c, need, &wsp));
     Carver cv{(char *)wsp, 0, need};
-    double *Q = cv.take<double>((size_t)mo * N), *Z = cv.take<double>((size_t)mo * N);
+    double *Q = cv.take<double>((size_t)mo * ld), *Z = cv.take<double>((size_t)mo * ld);
     double *r = cv.take<double>(N);
     float *r32 = cv.take<float>(N), *d32 = cv.take<float>(N);
     double *beta = cv.take<double>(kRedMaxK), *sc = cv.take<double>(8);
     SolveTimer tm;
-    SENK_TRY(tm.start(c));
+    SENK_TRY(tm.start(c, true));
     const SolveParams ip = inner_params(o);
     const double tol = o.eps * nrm_b;
     int inner_total = 0, outer_its = 0, cycles = 0, converged = 0, status = SENK_OK, hn = 0;
@@ -1235,7 +1423,7 @@ int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, d
         if (inner_total >= o.max_iter) break;
         cycles++;
         for (int j = 0; j < mo; j++) {
-            double *q = Q + (size_t)j * N, *z = Z + (size_t)j * N;
+            double *q = Q + (size_t)j * ld, *z = Z + (size_t)j * ld;
             SENK_TRY(launch_convert<double, float>(c, N, r, r32, sc + 1));
             SENK_HIP(hipMemsetAsync(d32, 0, sizeof(float) * N, c->stream));
             senk_result ir;
@@ -1244,8 +1432,8 @@ int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, d
             SENK_TRY(launch_convert<float, double>(c, N, d32, z, nullptr));
             SENK_TRY(spmv<double>(c, A, z, q));
             for (int pass = 0; pass < 2 && j > 0; pass++) {
-                SENK_TRY(launch_multidot<double>(c, N, j, Q, (size_t)N, q, beta));
-                k_sub_lincomb2<<<ew_grid(N), kBlock, 0, c->stream>>>(N, j, beta, Q, Z, (size_t)N, q, z);
+                SENK_TRY(launch_multidot<double>(c, N, j, Q, ld, q, beta));
+                k_sub_lincomb2<<<ew_grid(N), kBlock, 0, c->stream>>>(N, j, beta, Q, Z, ld, q, z);
                 SENK_LAUNCH_CHECK();
             }
             SENK_TRY(launch_multidot<double>(c, N, 1, q, (size_t)N, q, sc + 2));
