@@ -414,6 +414,44 @@ void senk_solve_opts_default(senk_solve_opts *o);
 int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *dx, const senk_solve_opts *opts,
                senk_result *res, double *hist, int hist_cap);
 
+/* ------------------------------------------------------------------ learned correction
+ * A message-passing network over the matrix graph that adds a correction to the ILU
+ * application in flexible solves (no reference counterpart):
+ *   s = |diag A|^-1/2, Ahat = S A S, x = [s r, z0/s]/rms(s r)   (z0 = ILU output)
+ *   h1 = relu(x W1s + (Ahat x) W1n + b1), h2, h3 likewise (width 32)
+ *   z = z0 + rms(s r) * s * (h3 wo + bo)
+ * Layers 2 and 3 run on the matrix cores with BF16 weights and FP32 accumulation; the
+ * parameter layout is W1s[2][32] W1n[2][32] b1[32] W2s[32][32] W2n[32][32] b2[32]
+ * W3s[32][32] W3n[32][32] b3[32] wo[32] bo (SENK_GNN_PARAMS floats). */
+typedef struct senk_gnn_s *senk_gnn;
+enum { SENK_GNN_PARAMS = 4353 };
+int senk_gnn_create(senk_ctx ctx, senk_gnn *out);   /* all parameters zero: no correction */
+int senk_gnn_destroy(senk_gnn g);
+int senk_gnn_set_params(senk_gnn g, const float *theta);   /* host arrays of SENK_GNN_PARAMS */
+int senk_gnn_get_params(senk_gnn g, float *theta);
+/* Bind the matrix the correction runs on (needs its FP32 values: senk_mat_create flag bit0). */
+int senk_gnn_bind(senk_gnn g, senk_mat A);
+/* dz = dz + correction(dr, dz) on the bound matrix (dz holds the ILU output on entry). */
+int senk_gnn_apply_f32(senk_gnn g, const float *dr, float *dz);
+/* Mean training loss ||A z - r||^2/||r||^2 over nvec residuals (device FP32, nvec rows of N)
+ * with M's FP32 factors, and its gradient (host, SENK_GNN_PARAMS doubles, may be NULL). */
+int senk_gnn_loss_grad(senk_gnn g, senk_mat A, senk_prec M, const float *dR, int nvec, double *loss,
+                       double *grad);
+/* Adam on the device over the residual sets of nmat matrices (dR[k]: nvec[k] FP32 rows of
+ * A[k]'s N); loss_hist (host, epochs) receives the mean loss of each epoch. */
+int senk_gnn_train(senk_gnn g, int nmat, const senk_mat *A, const senk_prec *M, const float *const *dR,
+                   const int *nvec, int epochs, int batch, double lr, unsigned seed, double *loss_hist,
+                   int verbose);
+/* Residual/error pairs from the plain FP64 ILU-GMRES(m) of the handle API: after each of
+ * up to ncycles restart cycles from x0 = 0, row k of dR receives r = b - A x and, when
+ * dxstar is given, row k of dE receives x* - x (device FP64).  *count = rows written. */
+int senk_gnn_collect(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, const double *dxstar, int m,
+                     int ncycles, double *dR, double *dE, int *count);
+/* Use g (bound to M's matrix) as M's learned correction in solves with use_learned = 1;
+ * g == NULL detaches.  A solve whose FP64 residual stagnates switches the correction off
+ * for the rest of that solve (senk_result.fell_back). */
+int senk_prec_attach_gnn(senk_prec M, senk_gnn g);
+
 /* ------------------------------------------------------------------ synthetic matrices
  * Generators for the benchmark systems (no reference counterpart).  Outputs are
  * malloc'd sorted CSR arrays.  Every choice a generator makes is stated in its source. */

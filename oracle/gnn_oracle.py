@@ -131,6 +131,11 @@ def loss(theta, g: Graph, R, Z0, emulate_bf16=False):
     return tot / len(R)
 
 
+def _agg_T(g: Graph, v):
+    """Backward of the aggregation m = Ahat h: dh += Ahat^T dm."""
+    return g.Ah.T @ v
+
+
 def loss_grad(theta, g: Graph, R, Z0, emulate_bf16=False):
     """Returns (L, dL/dtheta) by reverse-mode differentiation of forward()."""
     grads = {name: np.zeros(shp) for name, shp in PARAM_SHAPES}
@@ -156,5 +161,5 @@ def loss_grad(theta, g: Graph, R, Z0, emulate_bf16=False):
             grads[f"W{l}n"] += mag.T @ gl
             grads[f"b{l}"] += gl.sum(axis=0)
             if l > 1:
-                dh = gl @ p[f"W{l}s"].T + g.Ah.T @ (gl @ p[f"W{l}n"].T)
+                dh = gl @ p[f"W{l}s"].T + _agg_T(g, gl @ p[f"W{l}n"].T)
     return tot / B, pack(grads)

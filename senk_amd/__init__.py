@@ -78,7 +78,7 @@ def _ctype(decl: str):
             return C.c_char_p
         return C.c_void_p
     base = decl.split()[0]
-    if base in ("senk_ctx", "senk_trsv_plan", "senk_mat", "senk_prec"):
+    if base in ("senk_ctx", "senk_trsv_plan", "senk_mat", "senk_prec", "senk_gnn"):
         return C.c_void_p
     return _CTYPES[base]
 

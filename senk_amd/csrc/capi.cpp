@@ -34,6 +34,7 @@ struct senk_prec_s {
     senk_ctx ctx = nullptr;
     senk_amd::PrecView view;
     senk_amd::TriLayout lt, ut;
+    senk_gnn gnn = nullptr;  // attached learned correction (used with use_learned)
     int lnnz = 0, unnz = 0;
     std::vector<void *> owned;
     std::vector<int> hlrptr, hurptr;
@@ -1271,9 +1272,16 @@ int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *
     if (!A || !db || !dx || !o || !res) return SENK_ERR_ARG;
     double nrm_b = 0;
     SENK_TRY(senk_nrm2_f64(ctx, db, A->N, &nrm_b));
-    PrecView none;
-    none.N = A->N;
-    const PrecView &pv = M ? M->view : none;
+    PrecView pv;
+    pv.N = A->N;
+    if (M) pv = M->view;
+    LearnedPrec *lp = nullptr;
+    if (o->use_learned) {
+        if (!M || !M->gnn) return SENK_ERR_ARG;
+        lp = gnn_learned(M->gnn);
+        lp->off = false;
+        pv.learned = lp;
+    }
     if (o->mixed) {
         if (!A->v32 || (M && !M->view.l32)) return SENK_ERR_ARG;
         if (o->mixed == 2)
@@ -1296,7 +1304,68 @@ int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *
     double tr = 0;
     SENK_TRY(true_relres(&ctx->c, A->view, db, dx, nrm_b, &tr));
     res->true_relres = tr;
+    if (lp) res->fell_back = lp->off ? 1 : 0;
     return res->status;
+}
+
+// ---------------------------------------------------------------- learned correction
+int senk_gnn_create(senk_ctx ctx, senk_gnn *out) {
+    CTX_CHECK(ctx);
+    if (!out) return SENK_ERR_ARG;
+    return gnn_create(&ctx->c, out);
+}
+int senk_gnn_destroy(senk_gnn g) {
+    if (!g) return SENK_ERR_ARG;
+    gnn_destroy(g);
+    return SENK_OK;
+}
+int senk_gnn_set_params(senk_gnn g, const float *theta) {
+    if (!g || !theta) return SENK_ERR_ARG;
+    return gnn_set_params(g, theta);
+}
+int senk_gnn_get_params(senk_gnn g, float *theta) {
+    if (!g || !theta) return SENK_ERR_ARG;
+    return gnn_get_params(g, theta);
+}
+int senk_gnn_bind(senk_gnn g, senk_mat A) {
+    if (!g || !A || !A->v32) return SENK_ERR_ARG;
+    return gnn_bind(g, A->N, A->rptr, A->cind, A->v32);
+}
+int senk_gnn_apply_f32(senk_gnn g, const float *dr, float *dz) {
+    if (!g || !dr || !dz) return SENK_ERR_ARG;
+    return gnn_apply_f32(g, dr, dz);
+}
+static PrecView plain_view(senk_prec M) {
+    PrecView v = M->view;
+    v.learned = nullptr;
+    return v;
+}
+int senk_gnn_loss_grad(senk_gnn g, senk_mat A, senk_prec M, const float *dR, int nvec, double *loss, double *grad) {
+    if (!g || !A || !M || !dR || nvec <= 0 || !M->view.l32 || !A->v32) return SENK_ERR_ARG;
+    return gnn_loss_grad(g, A->view, plain_view(M), dR, nvec, loss, grad);
+}
+int senk_gnn_train(senk_gnn g, int nmat, const senk_mat *A, const senk_prec *M, const float *const *dR,
+                   const int *nvec, int epochs, int batch, double lr, unsigned seed, double *loss_hist, int verbose) {
+    if (!g || nmat <= 0 || !A || !M || !dR || !nvec) return SENK_ERR_ARG;
+    std::vector<MatView> av((size_t)nmat);
+    std::vector<PrecView> pv((size_t)nmat);
+    for (int k = 0; k < nmat; k++) {
+        if (!A[k] || !M[k] || !dR[k] || nvec[k] <= 0 || A[k]->N != M[k]->view.N) return SENK_ERR_ARG;
+        av[k] = A[k]->view;
+        pv[k] = plain_view(M[k]);
+    }
+    return gnn_train(g, nmat, av.data(), pv.data(), dR, nvec, epochs, batch, lr, seed, loss_hist, verbose);
+}
+int senk_gnn_collect(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, const double *dxstar, int m,
+                     int ncycles, double *dR, double *dE, int *count) {
+    CTX_CHECK(ctx);
+    if (!A || !M || !db || !dR || m <= 0 || ncycles <= 0 || !count) return SENK_ERR_ARG;
+    return collect_pairs(&ctx->c, A->view, plain_view(M), db, dxstar, m, ncycles, dR, dE, count);
+}
+int senk_prec_attach_gnn(senk_prec M, senk_gnn g) {
+    if (!M) return SENK_ERR_ARG;
+    M->gnn = g;
+    return SENK_OK;
 }
 
 }  // extern "C"

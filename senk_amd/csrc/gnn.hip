@@ -1,15 +1,803 @@
-// Learned preconditioner correction (placeholder until the message-passing network
-// lands): no correction is applied.
+// Learned correction of the ILU application: a 3-layer message-passing network over
+// the matrix graph (width 32, BF16 weights on the matrix cores with FP32 accumulation),
+// its backward pass and an Adam training loop, all on the device.
+//
+// Per application to a residual r whose ILU output is z0 (A with diagonal d):
+//   s = |d|^-1/2, Ahat = S A S;  x = [s r, z0 / s] / sigma,  sigma = rms(s r)
+//   h1 = relu(x W1s + (Ahat x) W1n + b1)                    (2 -> 32, FP32 FMAs)
+//   h2 = relu(h1 W2s + (Ahat h1) W2n + b2)                  (64 -> 32, matrix cores)
+//   h3 = relu(h2 W3s + (Ahat h2) W3n + b3)
+//   z  = z0 + sigma * s * (h3 wo + bo)
+// The network is homogeneous (scaling r scales the correction) and with wo = bo = 0 it
+// is plain ILU.  Edge aggregation reads the matrix's own FP32 CSR arrays.  Activations
+// are stored in BF16 (64 bytes per node), the aggregated inputs of the matrix-core
+// layers are rounded to BF16 as their operands, and accumulation is FP32.
+//
+// Parameter vector (FP32, kParams values): W1s[2][32] W1n[2][32] b1[32] W2s[32][32]
+// W2n[32][32] b2[32] W3s[32][32] W3n[32][32] b3[32] wo[32] bo.
 #include "dev_common.hpp"
 #include "solvers.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
 namespace senk_amd {
 
-int learned_apply_f32(Ctx *, const PrecView &, const float *, const double *, float *, const int *) {
-    return SENK_ERR_UNSUPPORTED;
+namespace gnn {
+constexpr int W = 32;  // width
+constexpr int oW1s = 0, oW1n = 64, ob1 = 128, oW2s = 160, oW2n = 1184, ob2 = 2208, oW3s = 2240, oW3n = 3264,
+              ob3 = 4288, owo = 4320, obo = 4352;
+constexpr int kParams = 4353;
+constexpr int kSlot = 7;          // partial-sum slot of the context used by the network
+constexpr int kGradBlocks = 256;  // workgroups of the weight-gradient reductions
+}  // namespace gnn
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// BF16 with round-to-nearest-even (finite inputs).
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
 }
-int learned_apply_f64(Ctx *, const PrecView &, const double *, const double *, double *, const int *) {
-    return SENK_ERR_UNSUPPORTED;
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// ----------------------------------------------------------------- state
+struct GnnGraph {
+    int N = 0;
+    const int *rptr = nullptr, *cind = nullptr;
+    const float *val = nullptr;
+    const int *trptr = nullptr, *tcind = nullptr;  // A^T (== A's arrays when A is symmetric)
+    const float *tval = nullptr;
+    float *s = nullptr;                            // owned
+    int *own_trptr = nullptr, *own_tcind = nullptr;
+    float *own_tval = nullptr;
+};
+struct GnnWork {
+    int N = 0;
+    bool train = false;
+    float2 *xin = nullptr, *xs = nullptr;          // [s r, z0/s] and s*[s r, z0/s]
+    unsigned short *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *m2 = nullptr, *m3 = nullptr;
+    float *chat = nullptr;
+    float2 *x = nullptr, *m1 = nullptr;            // training: normalised inputs and Ahat x
+    float *G = nullptr, *G2 = nullptr, *P = nullptr, *Q = nullptr, *dchat = nullptr;
+    float *z = nullptr, *e = nullptr, *t = nullptr, *tmp = nullptr;
+    double *sc = nullptr;                          // [0] sigma [1] 1/sigma [2] rr [3] ee [4] loss sum
+    float *gpart = nullptr;                        // kGradBlocks x kParams
+};
+
+}  // namespace senk_amd
+
+struct senk_gnn_s {
+    senk_amd::Ctx *c = nullptr;
+    float *theta = nullptr;     // FP32 master parameters
+    float *theta_r = nullptr;   // the same with the weights rounded to BF16 (used by the kernels)
+    senk_amd::bf16x4 *wb = nullptr;  // packed matrix-core operands of layers 2 and 3 (2 x 512)
+    double *grad = nullptr;     // accumulated gradient (kParams)
+    float *adam_m = nullptr, *adam_v = nullptr;
+    long long adam_t = 0;
+    senk_amd::GnnGraph g;       // graph bound for inference
+    senk_amd::GnnWork w;
+    senk_amd::LearnedPrec lp;   // lp.net == this
+};
+
+namespace senk_amd {
+
+namespace {
+using namespace gnn;
+
+// ----------------------------------------------------------------- parameter packing
+// theta_r = theta with the weights rounded to BF16; wb = matrix-core B operands:
+// wb[layer][t*64 + lane] = W[8t + 4*(lane/32) + i][lane%32], i < 4, with W = [Ws; Wn].
+__global__ void k_pack_params(const float *__restrict__ theta, float *theta_r, bf16x4 *wb) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < kParams) {
+        const bool bias = (tid >= ob1 && tid < oW2s) || (tid >= ob2 && tid < oW3s) || (tid >= ob3 && tid < owo) ||
+                          tid == obo;
+        theta_r[tid] = bias ? theta[tid] : bf2f(f2bf(theta[tid]));
+    }
+    if (tid < 2 * 512) {
+        const int layer = tid / 512, q = tid % 512, t = q / 64, lane = q % 64;
+        const int base = layer == 0 ? oW2s : oW3s;  // Ws then Wn follow each other (32x32 each)
+        bf16x4 v;
+        for (int i = 0; i < 4; i++) {
+            const int k = 8 * t + 4 * (lane / 32) + i, o = lane % 32;
+            v[i] = (short)f2bf(theta[base + k * W + o]);
+        }
+        wb[tid] = v;
+    }
+}
+
+// ----------------------------------------------------------------- forward
+// xin = [s r, z0/s], xs = s * xin, partials of (s r)^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gnn_in(int N, const T *__restrict__ r, const double *in_scale,
+                                                   const T *__restrict__ z0, const float *__restrict__ s,
+                                                   float2 *xin, float2 *xs, double *partials) {
+    const double sc = in_scale ? *in_scale : 1.0;
+    double ss = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const float si = s[i];
+        const float rh = si * (float)((double)r[i] * sc), zh = (float)z0[i] / si;
+        xin[i] = make_float2(rh, zh);
+        xs[i] = make_float2(si * rh, si * zh);
+        ss += (double)rh * (double)rh;
+    }
+    const double v[1] = {ss};
+    store_partials<1>(v, partials);
+}
+__global__ __launch_bounds__(kBlock) void k_gnn_sigma(const double *P, int B, int N, double *sc) {
+    const double ss = sum_partials(P, B);
+    if (threadIdx.x == 0) {
+        const double sg = sqrt(ss / N);
+        sc[0] = sg;
+        sc[1] = sg > 0 ? 1.0 / sg : 0.0;
+    }
+}
+// layer 1, one thread per node
+__global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict__ rptr, const int *__restrict__ cind,
+                                                   const float *__restrict__ val, const float *__restrict__ s,
+                                                   const float2 *__restrict__ xin, const float2 *__restrict__ xs,
+                                                   const double *sc, const float *__restrict__ th,
+                                                   unsigned short *h1, float2 *xkeep, float2 *m1keep) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const float inv = (float)sc[1];
+    float a0 = 0, a1 = 0;
+    for (int j = rptr[i]; j < rptr[i + 1]; j++) {
+        const float2 v = xs[cind[j]];
+        a0 += val[j] * v.x;
+        a1 += val[j] * v.y;
+    }
+    const float f = s[i] * inv;
+    const float m0 = a0 * f, m1 = a1 * f;
+    const float2 xv = xin[i];
+    const float x0 = xv.x * inv, x1 = xv.y * inv;
+    if (xkeep) {
+        xkeep[i] = make_float2(x0, x1);
+        m1keep[i] = make_float2(m0, m1);
+    }
+    unsigned short out[W];
+#pragma unroll
+    for (int o = 0; o < W; o++) {
+        float v = th[ob1 + o];
+        v += x0 * th[oW1s + o] + x1 * th[oW1s + W + o];
+        v += m0 * th[oW1n + o] + m1 * th[oW1n + W + o];
+        out[o] = f2bf(fmaxf(v, 0.f));
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(h1 + (size_t)i * W);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint4 u;
+        u.x = out[8 * q + 0] | ((unsigned)out[8 * q + 1] << 16);
+        u.y = out[8 * q + 2] | ((unsigned)out[8 * q + 3] << 16);
+        u.z = out[8 * q + 4] | ((unsigned)out[8 * q + 5] << 16);
+        u.w = out[8 * q + 6] | ((unsigned)out[8 * q + 7] << 16);
+        dst[q] = u;
+    }
+}
+// Layers 2 and 3 on the matrix cores: one wave per tile of 32 nodes.  Lane l feeds node
+// n0 + l%32; its A operand for k-step t (t < 4: own features, t >= 4: aggregated) is
+// features 8t + 4*(l/32) + [0,4) of that node.  D element (row, col) sits in lane
+// col + 32*((row/4)%2), register (row%4) + 4*(row/8).
+template <bool LAST>
+__global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict__ rptr, const int *__restrict__ cind,
+                                                   const float *__restrict__ val, const float *__restrict__ s,
+                                                   const unsigned short *__restrict__ hin, const bf16x4 *__restrict__ wb,
+                                                   const float *__restrict__ bias, unsigned short *hout,
+                                                   unsigned short *mkeep, const float *__restrict__ wo,
+                                                   const float *__restrict__ bo, float *chat) {
+    __shared__ float red[kWavesPerBlock][32][33];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n0 = (blockIdx.x * kWavesPerBlock + wave) * 32;
+    const int half = lane >> 5, col = lane & 31;
+    const int node = n0 + col;
+    const bool act = node < N;
+    bf16x4 a[8];
+    float m[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) m[q] = 0;
+    if (act) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) a[t] = *reinterpret_cast<const bf16x4 *>(hin + (size_t)node * W + 8 * t + 4 * half);
+        for (int j = rptr[node]; j < rptr[node + 1]; j++) {
+            const int cj = cind[j];
+            const float wgt = val[j] * s[cj];
+            const unsigned short *hj = hin + (size_t)cj * W + 4 * half;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const bf16x4 hv = *reinterpret_cast<const bf16x4 *>(hj + 8 * t);
+#pragma unroll
+                for (int i = 0; i < 4; i++) m[4 * t + i] += wgt * bf2f((unsigned short)hv[i]);
+            }
+        }
+        const float si = s[node];
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] *= si;
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; t++) a[t] = bf16x4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        bf16x4 v;
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = (short)f2bf(m[4 * t + i]);
+        a[4 + t] = v;
+        if (mkeep && act) *reinterpret_cast<bf16x4 *>(mkeep + (size_t)node * W + 8 * t + 4 * half) = v;
+    }
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[t], wb[t * 64 + lane], acc, 0, 0, 0);
+    const float b = bias[col];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int row = 8 * (r >> 2) + 4 * half + (r & 3);
+        const int nd = n0 + row;
+        const unsigned short hv = f2bf(fmaxf(acc[r] + b, 0.f));
+        if (hout && nd < N) hout[(size_t)nd * W + col] = hv;
+        if (LAST) red[wave][row][col] = bf2f(hv) * wo[col];
+    }
+    if (LAST) {
+        __syncthreads();
+        if (lane < 32 && n0 + lane < N) {
+            float sum = 0;
+#pragma unroll 8
+            for (int q = 0; q < 32; q++) sum += red[wave][lane][q];
+            chat[n0 + lane] = sum + bo[0];
+        }
+    }
+}
+// z += sigma * s * chat
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gnn_out(int N, const float *__restrict__ s, const float *__restrict__ chat,
+                                                    const double *sc, T *z) {
+    const float sg = (float)sc[0];
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
+        z[i] = (T)((double)z[i] + (double)(sg * s[i] * chat[i]));
+}
+
+// ----------------------------------------------------------------- backward
+// dchat = dz * sigma * s;  G = (dchat wo) * [h3 > 0]
+__global__ __launch_bounds__(kBlock) void k_gb_out(int N, const float *__restrict__ dz, const float *__restrict__ s,
+                                                   const double *sc, const float *__restrict__ th,
+                                                   const unsigned short *__restrict__ h3, float *dchat, float *G) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const float d = dz[i] * (float)sc[0] * s[i];
+    dchat[i] = d;
+#pragma unroll
+    for (int o = 0; o < W; o++) G[(size_t)i * W + o] = bf2f(h3[(size_t)i * W + o]) > 0 ? d * th[owo + o] : 0.f;
+}
+// P = G Ws^T, Q = G Wn^T (per node)
+__global__ __launch_bounds__(kBlock) void k_gb_pq(int N, const float *__restrict__ G, const float *__restrict__ Ws,
+                                                  const float *__restrict__ Wn, float *P, float *Q) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    float g[W];
+#pragma unroll
+    for (int o = 0; o < W; o++) g[o] = G[(size_t)i * W + o];
+    for (int k = 0; k < W; k++) {
+        float p = 0, q = 0;
+#pragma unroll
+        for (int o = 0; o < W; o++) {
+            p += g[o] * Ws[k * W + o];
+            q += g[o] * Wn[k * W + o];
+        }
+        P[(size_t)i * W + k] = p;
+        Q[(size_t)i * W + k] = q;
+    }
+}
+// Gprev = (P + Ahat^T Q) * [hprev > 0]
+__global__ __launch_bounds__(kBlock) void k_gb_agg(int N, const int *__restrict__ trptr, const int *__restrict__ tcind,
+                                                   const float *__restrict__ tval, const float *__restrict__ s,
+                                                   const float *__restrict__ P, const float *__restrict__ Q,
+                                                   const unsigned short *__restrict__ hprev, float *Gprev) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    float acc[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) acc[k] = 0;
+    for (int j = trptr[i]; j < trptr[i + 1]; j++) {
+        const int cj = tcind[j];
+        const float wgt = tval[j] * s[cj];
+#pragma unroll
+        for (int k = 0; k < W; k++) acc[k] += wgt * Q[(size_t)cj * W + k];
+    }
+    const float si = s[i];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const float dh = P[(size_t)i * W + k] + si * acc[k];
+        Gprev[(size_t)i * W + k] = bf2f(hprev[(size_t)i * W + k]) > 0 ? dh : 0.f;
+    }
+}
+// Weight gradients of a matrix-core layer over a contiguous node range per workgroup:
+// dWs[k][o] = sum hin[k] G[o], dWn[k][o] = sum m[k] G[o], db[o] = sum G[o].
+__global__ __launch_bounds__(kBlock) void k_gb_wmc(int N, const unsigned short *__restrict__ hin,
+                                                   const unsigned short *__restrict__ mag, const float *__restrict__ G,
+                                                   int oWs, int oWn, int ob, float *gpart) {
+    __shared__ float th[32][W + 1], tm[32][W + 1], tg[32][W + 1];
+    const int per = (N + gridDim.x - 1) / gridDim.x;
+    const int lo = blockIdx.x * per, hi = min(N, lo + per);
+    const int k = threadIdx.x / 8, o0 = (threadIdx.x % 8) * 4;
+    float as[4] = {0, 0, 0, 0}, an[4] = {0, 0, 0, 0}, ab = 0;
+    for (int n0 = lo; n0 < hi; n0 += 32) {
+        for (int q = threadIdx.x; q < 32 * W; q += kBlock) {
+            const int r = q / W, f = q % W, n = n0 + r;
+            const bool ok = n < hi;
+            th[r][f] = ok ? bf2f(hin[(size_t)n * W + f]) : 0.f;
+            tm[r][f] = ok ? bf2f(mag[(size_t)n * W + f]) : 0.f;
+            tg[r][f] = ok ? G[(size_t)n * W + f] : 0.f;
+        }
+        __syncthreads();
+        for (int r = 0; r < 32; r++) {
+            const float hk = th[r][k], mk = tm[r][k];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                as[q] += hk * tg[r][o0 + q];
+                an[q] += mk * tg[r][o0 + q];
+            }
+            if (threadIdx.x < W) ab += tg[r][threadIdx.x];
+        }
+        __syncthreads();
+    }
+    float *gp = gpart + (size_t)blockIdx.x * kParams;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        gp[oWs + k * W + o0 + q] = as[q];
+        gp[oWn + k * W + o0 + q] = an[q];
+    }
+    if (threadIdx.x < W) gp[ob + threadIdx.x] = ab;
+}
+// Layer-1 and output-layer gradients: dW1s[c][o] = sum x[c] G1[o], dW1n[c][o] = sum m1[c] G1[o],
+// db1 = sum G1, dwo[o] = sum h3[o] dchat, dbo = sum dchat.
+__global__ __launch_bounds__(kBlock) void k_gb_w1o(int N, const float2 *__restrict__ x, const float2 *__restrict__ m1,
+                                                   const float *__restrict__ G1, const unsigned short *__restrict__ h3,
+                                                   const float *__restrict__ dchat, float *gpart) {
+    // thread t < 32: output column t of layer 1; t in [32, 64): wo[t-32]; t == 64: bo
+    const int per = (N + gridDim.x - 1) / gridDim.x;
+    const int lo = blockIdx.x * per, hi = min(N, lo + per);
+    const int t = threadIdx.x;
+    float a[5] = {0, 0, 0, 0, 0};
+    if (t < W) {
+        for (int n = lo; n < hi; n++) {
+            const float g = G1[(size_t)n * W + t];
+            const float2 xv = x[n], mv = m1[n];
+            a[0] += xv.x * g;
+            a[1] += xv.y * g;
+            a[2] += mv.x * g;
+            a[3] += mv.y * g;
+            a[4] += g;
+        }
+    } else if (t < 2 * W) {
+        for (int n = lo; n < hi; n++) a[0] += bf2f(h3[(size_t)n * W + t - W]) * dchat[n];
+    } else if (t == 2 * W) {
+        for (int n = lo; n < hi; n++) a[0] += dchat[n];
+    }
+    float *gp = gpart + (size_t)blockIdx.x * kParams;
+    if (t < W) {
+        gp[oW1s + t] = a[0];
+        gp[oW1s + W + t] = a[1];
+        gp[oW1n + t] = a[2];
+        gp[oW1n + W + t] = a[3];
+        gp[ob1 + t] = a[4];
+    } else if (t < 2 * W) {
+        gp[owo + t - W] = a[0];
+    } else if (t == 2 * W) {
+        gp[obo] = a[0];
+    }
+}
+// grad[p] += sum over workgroups of gpart[wg][p] (fixed order)
+__global__ __launch_bounds__(kBlock) void k_gb_reduce(const float *__restrict__ gpart, int nb, double *grad) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= kParams) return;
+    double s = 0;
+    for (int b = 0; b < nb; b++) s += gpart[(size_t)b * kParams + p];
+    grad[p] += s;
+}
+
+// ----------------------------------------------------------------- loss and optimiser
+// e = Az - r (t holds A z); partials of (r, r) and (e, e)
+__global__ __launch_bounds__(kBlock) void k_gl_err(int N, const float *__restrict__ r, const float *__restrict__ t,
+                                                   float *e, double *partials) {
+    double rr = 0, ee = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const float ev = t[i] - r[i];
+        e[i] = ev;
+        rr += (double)r[i] * (double)r[i];
+        ee += (double)ev * (double)ev;
+    }
+    const double v[2] = {rr, ee};
+    store_partials<2>(v, partials);
+}
+// sc[2] = rr, sc[3] = ee, sc[4] += ee/rr
+__global__ __launch_bounds__(kBlock) void k_gl_fin(const double *P, int B, double *sc) {
+    const double rr = sum_partials(P, B), ee = sum_partials(P + kPartLd, B);
+    if (threadIdx.x == 0) {
+        sc[2] = rr;
+        sc[3] = ee;
+        sc[4] += rr > 0 ? ee / rr : 0.0;
+    }
+}
+// dz = A^T e * (2 / (nb rr))   (t holds A^T e)
+__global__ __launch_bounds__(kBlock) void k_gl_dz(int N, const float *__restrict__ t, const double *sc, int nb,
+                                                  float *dz) {
+    const float f = (float)(2.0 / ((double)nb * sc[2]));
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
+        dz[i] = t[i] * f;
+}
+__global__ void k_adam(int n, float *theta, float *m, float *v, const double *grad, double lr, double b1, double b2,
+                       double eps, double c1, double c2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double g = grad[p];
+    const double mm = b1 * m[p] + (1 - b1) * g, vv = b2 * v[p] + (1 - b2) * g * g;
+    m[p] = (float)mm;
+    v[p] = (float)vv;
+    theta[p] = (float)(theta[p] - lr * (mm / c1) / (sqrt(vv / c2) + eps));
+}
+__global__ void k_zero_d(int n, double *p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+inline int node_grid(int N) { return (N + kBlock - 1) / kBlock; }
+inline int tile_grid(int N) { return (N + 32 * kWavesPerBlock - 1) / (32 * kWavesPerBlock); }
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+    SENK_HIP(hipMalloc((void **)p, sizeof(T) * (n ? n : 1)));
+    return SENK_OK;
+}
+}  // namespace
+
+// ----------------------------------------------------------------- host helpers
+void gnn_graph_free(GnnGraph *g) {
+    for (void *p : {(void *)g->s, (void *)g->own_trptr, (void *)g->own_tcind, (void *)g->own_tval})
+        if (p) (void)hipFree(p);
+    *g = GnnGraph{};
+}
+void gnn_work_free(GnnWork *w) {
+    for (void *p : {(void *)w->xin, (void *)w->xs, (void *)w->h1, (void *)w->h2, (void *)w->h3, (void *)w->m2,
+                    (void *)w->m3, (void *)w->chat, (void *)w->x, (void *)w->m1, (void *)w->G, (void *)w->G2,
+                    (void *)w->P, (void *)w->Q, (void *)w->dchat, (void *)w->z, (void *)w->e, (void *)w->t,
+                    (void *)w->tmp, (void *)w->sc, (void *)w->gpart})
+        if (p) (void)hipFree(p);
+    *w = GnnWork{};
+}
+int gnn_work_ensure(GnnWork *w, int N, bool train) {
+    if (w->N >= N && (w->train || !train)) return SENK_OK;
+    gnn_work_free(w);
+    w->N = N;
+    w->train = train;
+    const size_t nf = (size_t)N * gnn::W;
+    SENK_TRY(dalloc(&w->xin, N));
+    SENK_TRY(dalloc(&w->xs, N));
+    SENK_TRY(dalloc(&w->h1, nf));
+    SENK_TRY(dalloc(&w->h2, nf));
+    SENK_TRY(dalloc(&w->chat, N));
+    SENK_TRY(dalloc(&w->sc, 8));
+    if (train) {
+        SENK_TRY(dalloc(&w->h3, nf));
+        SENK_TRY(dalloc(&w->m2, nf));
+        SENK_TRY(dalloc(&w->m3, nf));
+        SENK_TRY(dalloc(&w->x, N));
+        SENK_TRY(dalloc(&w->m1, N));
+        SENK_TRY(dalloc(&w->G, nf));
+        SENK_TRY(dalloc(&w->G2, nf));
+        SENK_TRY(dalloc(&w->P, nf));
+        SENK_TRY(dalloc(&w->Q, nf));
+        SENK_TRY(dalloc(&w->dchat, N));
+        SENK_TRY(dalloc(&w->z, N));
+        SENK_TRY(dalloc(&w->e, N));
+        SENK_TRY(dalloc(&w->t, N));
+        SENK_TRY(dalloc(&w->tmp, N));
+        SENK_TRY(dalloc(&w->gpart, (size_t)gnn::kGradBlocks * gnn::kParams));
+    }
+    return SENK_OK;
+}
+
+// s = |diag|^-1/2 on the device; the transpose (training) on the host, or A's own arrays
+// when A is symmetric.
+int gnn_graph_build(Ctx *c, int N, const int *drptr, const int *dcind, const float *dval, bool transpose,
+                    GnnGraph *g) {
+    gnn_graph_free(g);
+    g->N = N;
+    g->rptr = drptr;
+    g->cind = dcind;
+    g->val = dval;
+    g->trptr = drptr;
+    g->tcind = dcind;
+    g->tval = dval;
+    std::vector<int> rp((size_t)N + 1);
+    SENK_HIP(hipMemcpyAsync(rp.data(), drptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    const int nnz = rp[N];
+    std::vector<int> ci((size_t)std::max(nnz, 1));
+    std::vector<float> va((size_t)std::max(nnz, 1));
+    SENK_HIP(hipMemcpyAsync(ci.data(), dcind, sizeof(int) * nnz, hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipMemcpyAsync(va.data(), dval, sizeof(float) * nnz, hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> s((size_t)N);
+    for (int i = 0; i < N; i++) {
+        double d = 0;
+        for (int j = rp[i]; j < rp[i + 1]; j++)
+            if (ci[j] == i) d = va[j];
+        if (d == 0) {
+            record_error("learned preconditioner: zero diagonal entry");
+            return SENK_ERR_ARG;
+        }
+        s[i] = (float)(1.0 / std::sqrt(std::fabs(d)));
+    }
+    SENK_TRY(dalloc(&g->s, N));
+    SENK_HIP(hipMemcpy(g->s, s.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    if (!transpose) return SENK_OK;
+    // transpose by counting sort; keep A's arrays when the result is identical
+    std::vector<int> tp((size_t)N + 1, 0), tc((size_t)std::max(nnz, 1));
+    std::vector<float> tv((size_t)std::max(nnz, 1));
+    for (int j = 0; j < nnz; j++) tp[ci[j] + 1]++;
+    for (int i = 0; i < N; i++) tp[i + 1] += tp[i];
+    std::vector<int> fill(tp.begin(), tp.end() - 1);
+    for (int i = 0; i < N; i++)
+        for (int j = rp[i]; j < rp[i + 1]; j++) {
+            const int q = fill[ci[j]]++;
+            tc[q] = i;
+            tv[q] = va[j];
+        }
+    bool same = tp == rp;
+    for (int j = 0; same && j < nnz; j++) same = tc[j] == ci[j] && tv[j] == va[j];
+    if (same) return SENK_OK;
+    SENK_TRY(dalloc(&g->own_trptr, (size_t)N + 1));
+    SENK_TRY(dalloc(&g->own_tcind, (size_t)nnz));
+    SENK_TRY(dalloc(&g->own_tval, (size_t)nnz));
+    SENK_HIP(hipMemcpy(g->own_trptr, tp.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    SENK_HIP(hipMemcpy(g->own_tcind, tc.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+    SENK_HIP(hipMemcpy(g->own_tval, tv.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+    g->trptr = g->own_trptr;
+    g->tcind = g->own_tcind;
+    g->tval = g->own_tval;
+    return SENK_OK;
+}
+
+int gnn_pack(senk_gnn_s *net) {
+    k_pack_params<<<(gnn::kParams + 255) / 256, 256, 0, net->c->stream>>>(net->theta, net->theta_r, net->wb);
+    SENK_LAUNCH_CHECK();
+    return SENK_OK;
+}
+
+// z (holding z0) += correction for residual r; keep = store what the backward pass needs
+template <typename T>
+int gnn_forward(Ctx *c, senk_gnn_s *net, const GnnGraph &g, GnnWork &w, const T *r, const double *in_scale, T *z,
+                bool keep) {
+    using namespace gnn;
+    const int N = g.N;
+    const int B = red_grid(N);
+    double *P = red_slot(c, kSlot);
+    k_gnn_in<T><<<B, kBlock, 0, c->stream>>>(N, r, in_scale, z, g.s, w.xin, w.xs, P);
+    SENK_LAUNCH_CHECK();
+    k_gnn_sigma<<<1, kBlock, 0, c->stream>>>(P, B, N, w.sc);
+    SENK_LAUNCH_CHECK();
+    const float *th = net->theta_r;
+    k_gnn_l1<<<node_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.xin, w.xs, w.sc, th, w.h1,
+                                                       keep ? w.x : nullptr, keep ? w.m1 : nullptr);
+    SENK_LAUNCH_CHECK();
+    k_gnn_mc<false><<<tile_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.h1, net->wb, th + ob2,
+                                                             w.h2, keep ? w.m2 : nullptr, nullptr, nullptr, nullptr);
+    SENK_LAUNCH_CHECK();
+    k_gnn_mc<true><<<tile_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.h2, net->wb + 512,
+                                                            th + ob3, keep ? w.h3 : nullptr, keep ? w.m3 : nullptr,
+                                                            th + owo, th + obo, w.chat);
+    SENK_LAUNCH_CHECK();
+    k_gnn_out<T><<<node_grid(N), kBlock, 0, c->stream>>>(N, g.s, w.chat, w.sc, z);
+    SENK_LAUNCH_CHECK();
+    return SENK_OK;
+}
+
+// Backward pass for one residual: dz (device, N) -> gradient added to net->grad.
+int gnn_backward(Ctx *c, senk_gnn_s *net, const GnnGraph &g, GnnWork &w, const float *dz) {
+    using namespace gnn;
+    const int N = g.N, ng = node_grid(N);
+    const float *th = net->theta_r;
+    k_gb_out<<<ng, kBlock, 0, c->stream>>>(N, dz, g.s, w.sc, th, w.h3, w.dchat, w.G);
+    SENK_LAUNCH_CHECK();
+    // layer 3: grads, then G2 = dh2 * [h2 > 0]
+    k_gb_wmc<<<kGradBlocks, kBlock, 0, c->stream>>>(N, w.h2, w.m3, w.G, oW3s, oW3n, ob3, w.gpart);
+    SENK_LAUNCH_CHECK();
+    k_gb_pq<<<ng, kBlock, 0, c->stream>>>(N, w.G, th + oW3s, th + oW3n, w.P, w.Q);
+    SENK_LAUNCH_CHECK();
+    k_gb_agg<<<ng, kBlock, 0, c->stream>>>(N, g.trptr, g.tcind, g.tval, g.s, w.P, w.Q, w.h2, w.G2);
+    SENK_LAUNCH_CHECK();
+    // layer 2: grads, then G = dh1 * [h1 > 0]
+    k_gb_wmc<<<kGradBlocks, kBlock, 0, c->stream>>>(N, w.h1, w.m2, w.G2, oW2s, oW2n, ob2, w.gpart);
+    SENK_LAUNCH_CHECK();
+    k_gb_pq<<<ng, kBlock, 0, c->stream>>>(N, w.G2, th + oW2s, th + oW2n, w.P, w.Q);
+    SENK_LAUNCH_CHECK();
+    k_gb_agg<<<ng, kBlock, 0, c->stream>>>(N, g.trptr, g.tcind, g.tval, g.s, w.P, w.Q, w.h1, w.G);
+    SENK_LAUNCH_CHECK();
+    // layer 1 and output layer
+    k_gb_w1o<<<kGradBlocks, kBlock, 0, c->stream>>>(N, w.x, w.m1, w.G, w.h3, w.dchat, w.gpart);
+    SENK_LAUNCH_CHECK();
+    k_gb_reduce<<<(kParams + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(w.gpart, kGradBlocks, net->grad);
+    SENK_LAUNCH_CHECK();
+    return SENK_OK;
+}
+
+// One residual of a batch of nb: z0 = M^-1 r (FP32 factors), z = z0 + correction,
+// loss term ||A z - r||^2/||r||^2 added to sc[4], gradient of loss/nb added to grad.
+int gnn_step(Ctx *c, senk_gnn_s *net, const GnnGraph &g, GnnWork &w, const MatView &A, const PrecView &M,
+             const float *r, int nb) {
+    const int N = g.N;
+    SENK_TRY(prec_apply<float>(c, M, r, nullptr, w.z, w.tmp, nullptr));
+    SENK_TRY(gnn_forward<float>(c, net, g, w, r, nullptr, w.z, true));
+    SENK_TRY(launch_spmv_csr<float>(c, N, g.rptr, g.cind, g.val, w.z, w.t, A.avg_row));
+    const int B = red_grid(N);
+    double *P = red_slot(c, gnn::kSlot);
+    k_gl_err<<<B, kBlock, 0, c->stream>>>(N, r, w.t, w.e, P);
+    SENK_LAUNCH_CHECK();
+    k_gl_fin<<<1, kBlock, 0, c->stream>>>(P, B, w.sc);
+    SENK_LAUNCH_CHECK();
+    SENK_TRY(launch_spmv_csr<float>(c, N, g.trptr, g.tcind, g.tval, w.e, w.t, A.avg_row));
+    k_gl_dz<<<node_grid(N), kBlock, 0, c->stream>>>(N, w.t, w.sc, nb, w.e);  // dz into e
+    SENK_LAUNCH_CHECK();
+    return gnn_backward(c, net, g, w, w.e);
+}
+
+int learned_apply_f32(Ctx *c, const PrecView &M, const float *r, const double *in_scale, float *out,
+                      const int *) {
+    if (M.learned->off) return SENK_OK;
+    senk_gnn_s *net = M.learned->net;
+    if (!net || net->g.N != M.N) return SENK_ERR_ARG;
+    SENK_TRY(gnn_work_ensure(&net->w, M.N, net->w.train));
+    return gnn_forward<float>(c, net, net->g, net->w, r, in_scale, out, false);
+}
+int learned_apply_f64(Ctx *c, const PrecView &M, const double *r, const double *in_scale, double *out,
+                      const int *) {
+    if (M.learned->off) return SENK_OK;
+    senk_gnn_s *net = M.learned->net;
+    if (!net || net->g.N != M.N) return SENK_ERR_ARG;
+    SENK_TRY(gnn_work_ensure(&net->w, M.N, net->w.train));
+    return gnn_forward<double>(c, net, net->g, net->w, r, in_scale, out, false);
+}
+
+// ----------------------------------------------------------------- handle operations
+int gnn_create(Ctx *c, senk_gnn_s **out) {
+    auto *net = new senk_gnn_s();
+    net->c = c;
+    net->lp.net = net;
+    int st = SENK_OK;
+    if ((st = dalloc(&net->theta, gnn::kParams)) || (st = dalloc(&net->theta_r, gnn::kParams)) ||
+        (st = dalloc(&net->wb, 1024)) || (st = dalloc(&net->grad, gnn::kParams)) ||
+        (st = dalloc(&net->adam_m, gnn::kParams)) || (st = dalloc(&net->adam_v, gnn::kParams))) {
+        gnn_destroy(net);
+        return st;
+    }
+    (void)hipMemsetAsync(net->theta, 0, sizeof(float) * gnn::kParams, c->stream);
+    (void)hipMemsetAsync(net->adam_m, 0, sizeof(float) * gnn::kParams, c->stream);
+    (void)hipMemsetAsync(net->adam_v, 0, sizeof(float) * gnn::kParams, c->stream);
+    if ((st = gnn_pack(net))) {
+        gnn_destroy(net);
+        return st;
+    }
+    *out = net;
+    return SENK_OK;
+}
+void gnn_destroy(senk_gnn_s *net) {
+    if (!net) return;
+    if (net->c) (void)hipStreamSynchronize(net->c->stream);
+    for (void *p : {(void *)net->theta, (void *)net->theta_r, (void *)net->wb, (void *)net->grad,
+                    (void *)net->adam_m, (void *)net->adam_v})
+        if (p) (void)hipFree(p);
+    gnn_graph_free(&net->g);
+    gnn_work_free(&net->w);
+    delete net;
+}
+int gnn_set_params(senk_gnn_s *net, const float *h) {
+    SENK_HIP(hipMemcpyAsync(net->theta, h, sizeof(float) * gnn::kParams, hipMemcpyHostToDevice, net->c->stream));
+    SENK_TRY(gnn_pack(net));
+    SENK_HIP(hipStreamSynchronize(net->c->stream));
+    return SENK_OK;
+}
+int gnn_get_params(senk_gnn_s *net, float *h) {
+    SENK_HIP(hipMemcpyAsync(h, net->theta, sizeof(float) * gnn::kParams, hipMemcpyDeviceToHost, net->c->stream));
+    SENK_HIP(hipStreamSynchronize(net->c->stream));
+    return SENK_OK;
+}
+int gnn_bind(senk_gnn_s *net, int N, const int *drptr, const int *dcind, const float *dval) {
+    if (!dval) return SENK_ERR_ARG;  // the network aggregates over the FP32 values
+    SENK_TRY(gnn_graph_build(net->c, N, drptr, dcind, dval, false, &net->g));
+    return gnn_work_ensure(&net->w, N, net->w.train);
+}
+LearnedPrec *gnn_learned(senk_gnn_s *net) { return &net->lp; }
+int gnn_apply_f32(senk_gnn_s *net, const float *r, float *z) {
+    if (net->g.N <= 0) return SENK_ERR_ARG;
+    return gnn_forward<float>(net->c, net, net->g, net->w, r, nullptr, z, false);
+}
+
+// Mean loss over nvec residuals (device, FP32, rows of N) and its gradient (host, FP64).
+int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const float *R, int nvec, double *loss,
+                  double *grad_host) {
+    Ctx *c = net->c;
+    if (!A.v32 || nvec <= 0) return SENK_ERR_ARG;
+    GnnGraph g;
+    int st = gnn_graph_build(c, A.N, A.rptr, A.cind, A.v32, true, &g);
+    if (st == SENK_OK) st = gnn_work_ensure(&net->w, A.N, true);
+    if (st == SENK_OK) {
+        k_zero_d<<<(gnn::kParams + 255) / 256, 256, 0, c->stream>>>(gnn::kParams, net->grad);
+        k_zero_d<<<1, 64, 0, c->stream>>>(8, net->w.sc);
+        for (int b = 0; b < nvec && st == SENK_OK; b++)
+            st = gnn_step(c, net, g, net->w, A, M, R + (size_t)b * A.N, nvec);
+    }
+    double sc[8] = {0};
+    if (st == SENK_OK && (hipMemcpyAsync(sc, net->w.sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                          (grad_host && hipMemcpyAsync(grad_host, net->grad, sizeof(double) * gnn::kParams,
+                                                       hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+                          hipStreamSynchronize(c->stream) != hipSuccess))
+        st = SENK_ERR_HIP;
+    if (loss) *loss = sc[4] / nvec;
+    gnn_graph_free(&g);
+    return st;
+}
+
+// Adam over the residual sets of several matrices: every step takes `batch` residuals of
+// one matrix; matrices and residual order are shuffled per epoch from `seed`.
+int gnn_train(senk_gnn_s *net, int nmat, const MatView *As, const PrecView *Ms, const float *const *Rs,
+              const int *nvecs, int epochs, int batch, double lr, unsigned seed, double *loss_hist, int verbose) {
+    Ctx *c = net->c;
+    if (nmat <= 0 || epochs <= 0 || batch <= 0) return SENK_ERR_ARG;
+    std::vector<GnnGraph> gs((size_t)nmat);
+    int st = SENK_OK, maxN = 0;
+    for (int k = 0; k < nmat && st == SENK_OK; k++) {
+        if (!As[k].v32 || !Ms[k].l32) st = SENK_ERR_ARG;
+        else st = gnn_graph_build(c, As[k].N, As[k].rptr, As[k].cind, As[k].v32, true, &gs[k]);
+        maxN = std::max(maxN, As[k].N);
+    }
+    if (st == SENK_OK) st = gnn_work_ensure(&net->w, maxN, true);
+    // (matrix, first residual) pairs of every step of an epoch
+    std::vector<std::pair<int, int>> steps;
+    for (int k = 0; k < nmat; k++)
+        for (int b = 0; b < nvecs[k]; b += batch) steps.emplace_back(k, b);
+    std::mt19937 rng(seed);
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+    for (int ep = 0; ep < epochs && st == SENK_OK; ep++) {
+        std::shuffle(steps.begin(), steps.end(), rng);
+        k_zero_d<<<1, 64, 0, c->stream>>>(8, net->w.sc);
+        long long nterms = 0;
+        for (const auto &sp : steps) {
+            const int k = sp.first, nb = std::min(batch, nvecs[k] - sp.second);
+            k_zero_d<<<(gnn::kParams + 255) / 256, 256, 0, c->stream>>>(gnn::kParams, net->grad);
+            for (int b = 0; b < nb && st == SENK_OK; b++)
+                st = gnn_step(c, net, gs[k], net->w, As[k], Ms[k], Rs[k] + (size_t)(sp.second + b) * As[k].N, nb);
+            nterms += nb;
+            if (st) break;
+            net->adam_t++;
+            const double c1 = 1 - std::pow(b1, (double)net->adam_t), c2 = 1 - std::pow(b2, (double)net->adam_t);
+            k_adam<<<(gnn::kParams + 255) / 256, 256, 0, c->stream>>>(gnn::kParams, net->theta, net->adam_m,
+                                                                      net->adam_v, net->grad, lr, b1, b2, eps, c1, c2);
+            if (hipGetLastError() != hipSuccess || (st = gnn_pack(net))) {
+                if (!st) st = SENK_ERR_HIP;
+                break;
+            }
+        }
+        double sc[8] = {0};
+        if (st == SENK_OK && (hipMemcpyAsync(sc, net->w.sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream) !=
+                                  hipSuccess ||
+                              hipStreamSynchronize(c->stream) != hipSuccess))
+            st = SENK_ERR_HIP;
+        if (st == SENK_OK && loss_hist) loss_hist[ep] = sc[4] / std::max(1LL, nterms);
+        if (st == SENK_OK && verbose) {
+            std::printf("epoch %d loss %.6e\n", ep, sc[4] / std::max(1LL, nterms));
+            std::fflush(stdout);
+        }
+    }
+    for (auto &g : gs) gnn_graph_free(&g);
+    return st;
 }
 
 }  // namespace senk_amd

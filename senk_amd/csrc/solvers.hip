@@ -222,6 +222,8 @@ struct SolveTimer {
     }
 };
 
+void monitor_learned(const PrecView &M, double rho, double rho_prev);
+
 // ================================================================= GMRES kernels
 // e[0] = ||r0||, sig[0] = 1/||r0||, state reset (cycle start).
 __global__ __launch_bounds__(kBlock) void k_gmres_start(const double *P, int B, double *e, double *sig, int *state) {
@@ -836,6 +838,7 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
     SENK_TRY(tm.start(c));
     const double tol = p.nrm_b * p.eps;
     int converged = 0, breakdown = 0, last = 0, cycles = 0, nh = 0, hstate[4] = {0, 0, 0, 0};
+    double restart_prev = 0;
     const int B = red_grid(N);
     for (int i = 0; i < p.outer; i++) {
         cycles++;
@@ -845,6 +848,14 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
         k_gcr_restart<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, tol, p.nrm_b, state, dhist, hidx, hcap);
         SENK_LAUNCH_CHECK();
         SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+        if (M.learned && !M.learned->off) {
+            // the restart residual is the true one: watch it for stagnation
+            double rn = 0;
+            SENK_HIP(hipMemcpyAsync(&rn, dhist + hidx, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            SENK_HIP(hipStreamSynchronize(c->stream));
+            if (i > 0) monitor_learned(M, rn, restart_prev);
+            restart_prev = rn;
+        }
         SENK_HIP(hipStreamSynchronize(c->stream));
         nh = hidx + 1;
         last = i * m;
@@ -1286,6 +1297,13 @@ __global__ void k_inv_sqrt(const double *s, double *inv) {
     if (threadIdx.x == 0) *inv = 1.0 / sqrt(*s);
 }
 
+// The learned correction is switched off for the rest of a solve when an outer step
+// (or a restart cycle) fails to reduce the FP64 residual by this factor.
+constexpr double kStagnation = 0.5;
+void monitor_learned(const PrecView &M, double rho, double rho_prev) {
+    if (M.learned && !M.learned->off && rho_prev > 0 && !(rho <= kStagnation * rho_prev)) M.learned->off = true;
+}
+
 SolveParams inner_params(const senk_solve_opts &o) {
     SolveParams ip;
     ip.solver = o.solver;
@@ -1335,7 +1353,9 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
         SENK_LAUNCH_CHECK();
         double h2 = 0;
         SENK_TRY(read_scalar(c, dsc, &h2));
+        const double prev = relres;
         relres = std::sqrt(h2) / nrm_b;
+        if (steps > 0) monitor_learned(M, relres, prev);
         if (hist && steps < hist_cap) hist[steps] = relres;
         if (o.verbose) {
             std::printf("IR %d %e\n", steps, relres);
@@ -1444,7 +1464,9 @@ int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, d
             k_inv_sqrt<<<1, 64, 0, c->stream>>>(sc, sc + 1);
             SENK_LAUNCH_CHECK();
             SENK_TRY(read_scalar(c, sc, &h2));
+            const double rho_prev = rho;
             rho = std::sqrt(h2);
+            monitor_learned(M, rho, rho_prev);
             outer_its++;
             if (hist && hn < hist_cap) hist[hn++] = rho / nrm_b;
             if (o.verbose) {
@@ -1469,6 +1491,46 @@ int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, d
     res->t_solve_ms = ms;
     res->status = status;
     (void)cycles;
+    return SENK_OK;
+}
+
+// Residual/error pairs of the plain FP64 GMRES(m): after each restart cycle from
+// x0 = 0, r = b - A x (and x* - x) become row k of R (E).
+int collect_pairs(Ctx *c, const MatView &A, const PrecView &M, const double *b, const double *xstar, int m,
+                  int ncycles, double *R, double *E, int *count) {
+    const int N = A.N;
+    void *wsp;
+    SENK_TRY(ctx_outer(c, bytes_of<double>(N) + bytes_of<double>(4), &wsp));
+    Carver cv{(char *)wsp, 0, 0};
+    double *x = cv.take<double>(N), *d = cv.take<double>(4);
+    SENK_HIP(hipMemsetAsync(x, 0, sizeof(double) * N, c->stream));
+    int rg = 0;
+    SENK_TRY(residual<double>(c, A, x, b, R, &rg));
+    k_norm_fin<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, d, nullptr);
+    SENK_LAUNCH_CHECK();
+    double h = 0;
+    SENK_TRY(read_scalar(c, d, &h));
+    const double nrm_b = std::sqrt(h);
+    SolveParams p;
+    p.solver = SENK_GMRES;
+    p.m = m;
+    p.outer = 1;
+    p.eps = 0;  // run whole cycles
+    p.nrm_b = nrm_b;
+    int k = 0;
+    while (k < ncycles) {
+        senk_result res;
+        SENK_TRY(krylov_solve<double>(c, A, M, b, x, p, &res, nullptr, 0));
+        double *rk = R + (size_t)k * N;
+        SENK_TRY(residual<double>(c, A, x, b, rk, &rg));
+        k_norm_fin<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotRes), rg, d, nullptr);
+        SENK_LAUNCH_CHECK();
+        if (E && xstar) SENK_TRY(launch_axpyz<double>(c, N, -1.0, x, xstar, E + (size_t)k * N));
+        SENK_TRY(read_scalar(c, d, &h));
+        k++;
+        if (!(std::sqrt(h) > 1e-13 * nrm_b)) break;
+    }
+    *count = k;
     return SENK_OK;
 }
 

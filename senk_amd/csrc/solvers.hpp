@@ -3,6 +3,8 @@
 
 #include "internal.hpp"
 
+struct senk_gnn_s;
+
 namespace senk_amd {
 
 struct MatView {
@@ -20,7 +22,12 @@ struct MatView {
     const float *bv32 = nullptr;
 };
 
-struct LearnedPrec;  // gnn.hip
+// The learned correction as a preconditioner component (gnn.hip); off = switched off
+// by a solver's stagnation monitor, leaving plain ILU.
+struct LearnedPrec {
+    senk_gnn_s *net = nullptr;
+    bool off = false;
+};
 
 struct PrecView {
     int N = 0;
@@ -71,6 +78,23 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
 // FP64 flexible GCR(o.outer_m) whose preconditioner is an FP32 inner Krylov solve.
 int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x,
                  const senk_solve_opts &o, double nrm_b, senk_result *res, double *hist, int hist_cap);
+
+// Learned correction (gnn.hip).
+int gnn_create(Ctx *c, senk_gnn_s **out);
+void gnn_destroy(senk_gnn_s *net);
+int gnn_set_params(senk_gnn_s *net, const float *host_theta);
+int gnn_get_params(senk_gnn_s *net, float *host_theta);
+int gnn_bind(senk_gnn_s *net, int N, const int *drptr, const int *dcind, const float *dval);
+LearnedPrec *gnn_learned(senk_gnn_s *net);
+int gnn_apply_f32(senk_gnn_s *net, const float *r, float *z);
+int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const float *R, int nvec, double *loss,
+                  double *grad_host);
+int gnn_train(senk_gnn_s *net, int nmat, const MatView *As, const PrecView *Ms, const float *const *Rs,
+              const int *nvecs, int epochs, int batch, double lr, unsigned seed, double *loss_hist, int verbose);
+
+// Restart residuals (and errors against xstar) of the plain FP64 GMRES(m) with M.
+int collect_pairs(Ctx *c, const MatView &A, const PrecView &M, const double *b, const double *xstar, int m,
+                  int ncycles, double *R, double *E, int *count);
 
 // ||b - A x|| / nrm_b in FP64 (synchronises).
 int true_relres(Ctx *c, const MatView &A, const double *b, const double *x, double nrm_b, double *out);

@@ -1,0 +1,175 @@
+"""The learned ILU correction on the GPU against its CPU oracle (oracle/gnn_oracle.py).
+
+Tolerances: the oracle emulates the device's BF16 roundings (weights, stored
+activations, matrix-core operands), so what remains is FP32 accumulation order and
+the odd BF16 tie that rounds the other way: 1e-2 relative on the correction and 3e-2
+relative on the gradient.  A wrong matrix-core lane layout or a missing term gives
+errors of order one.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import senk_amd as S
+from senk_amd import gnn as N
+from oracle import gnn_oracle as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_params(seed=0, wo_scale=0.3):
+    rng = np.random.default_rng(seed)
+    th = N.init_params(seed).astype(np.float64)
+    o = 0
+    for name, shp in N.SHAPES:
+        n = int(np.prod(shp))
+        if name == "wo":
+            th[o:o + n] = rng.normal(0, wo_scale, n)
+        elif name.startswith("b"):
+            th[o:o + n] = rng.normal(0, 0.1, n)
+        o += n
+    return th.astype(np.float32)
+
+
+def _oracle_graph(sysm):
+    return G.Graph(sysm.rptr, sysm.cind, sysm.val.astype(np.float32).astype(np.float64))
+
+
+def _ilu32(ctx, sysm, r32):
+    dr = ctx.array(r32)
+    dz = ctx.zeros(sysm.N, np.float32)
+    S.call("senk_prec_apply_f32", sysm.M, dr.p, dz.p)
+    return dr, dz
+
+
+@pytest.fixture(scope="module")
+def small(ctx):
+    s = N.System(ctx, 10, 7, bsize=8)  # N = 1000: the last 32-node tile is partial
+    yield s
+    s.close()
+
+
+def test_forward_matches_oracle(ctx, small):
+    th = _random_params(1)
+    net = N.Gnn(ctx, th)
+    net.bind(small.A)
+    rng = np.random.default_rng(2)
+    r = rng.normal(size=small.N).astype(np.float32)
+    dr, dz = _ilu32(ctx, small, r)
+    z0 = dz.download().astype(np.float64)
+    S.call("senk_gnn_apply_f32", net.h, dr.p, dz.p)
+    z = dz.download().astype(np.float64)
+    zo = G.forward(th.astype(np.float64), _oracle_graph(small), r.astype(np.float64), z0, emulate_bf16=True)
+    c, co = z - z0, zo - z0
+    assert np.linalg.norm(co) > 1e-3 * np.linalg.norm(z0)  # the correction is not trivial
+    assert np.linalg.norm(c - co) <= 1e-2 * np.linalg.norm(co), np.linalg.norm(c - co) / np.linalg.norm(co)
+    # zero parameters: exactly the ILU output
+    net.set_params(np.zeros(N.N_PARAMS, np.float32))
+    dr, dz = _ilu32(ctx, small, r)
+    S.call("senk_gnn_apply_f32", net.h, dr.p, dz.p)
+    assert np.array_equal(dz.download(), z0.astype(np.float32))
+    net.close()
+
+
+def test_loss_and_gradient_match_oracle(ctx, small):
+    th = _random_params(3)
+    net = N.Gnn(ctx, th)
+    rng = np.random.default_rng(4)
+    R = [rng.normal(size=small.N).astype(np.float32) for _ in range(3)]
+    Z0 = [_ilu32(ctx, small, r)[1].download().astype(np.float64) for r in R]
+    dR = ctx.array(np.concatenate(R))
+    loss, grad = net.loss_grad(small.A, small.M, dR, len(R))
+    g = _oracle_graph(small)
+    lo, go = G.loss_grad(th.astype(np.float64), g, [r.astype(np.float64) for r in R], Z0, emulate_bf16=True)
+    assert abs(loss - lo) <= 1e-3 * lo, (loss, lo)
+    assert np.linalg.norm(grad - go) <= 3e-2 * np.linalg.norm(go), np.linalg.norm(grad - go) / np.linalg.norm(go)
+    # every parameter block points the same way
+    o = 0
+    for name, shp in N.SHAPES:
+        n = int(np.prod(shp))
+        a, b = grad[o:o + n], go[o:o + n]
+        if np.linalg.norm(b) > 1e-8 * np.linalg.norm(go):
+            assert a @ b >= 0.98 * np.linalg.norm(a) * np.linalg.norm(b), name
+        o += n
+    net.close()
+
+
+def test_collected_pairs_are_consistent(ctx, small):
+    db, dxs = ctx.array(small.b), ctx.array(small.xs)
+    k = 6
+    dR, dE = ctx.zeros(k * small.N), ctx.zeros(k * small.N)
+    cnt = C.c_int()
+    S.call("senk_gnn_collect", ctx.h, small.A, small.M, db.p, dxs.p, 5, k, dR.p, dE.p, C.byref(cnt))
+    assert 1 <= cnt.value <= k
+    R = dR.download().reshape(k, small.N)[: cnt.value]
+    E = dE.download().reshape(k, small.N)[: cnt.value]
+    rows = np.repeat(np.arange(small.N), np.diff(small.rptr))
+    for r, e in zip(R, E):
+        Ae = np.bincount(rows, weights=small.val * e[small.cind], minlength=small.N)
+        assert np.linalg.norm(Ae - r) <= 1e-10 * np.linalg.norm(small.b)
+    norms = np.linalg.norm(R, axis=1)
+    assert np.all(norms[1:] < norms[:-1])  # restart residuals of GMRES decrease
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    sets = [N.TrainingSet(ctx, n, 100 + k, bsize=8, m=10, ncycles=8) for k, n in enumerate([12, 16, 12, 16, 12, 16])]
+    net = N.Gnn(ctx, N.init_params(0))
+    hist = net.train(sets, epochs=25, batch=4, lr=3e-3, seed=0)
+    for s in sets:
+        s.close()
+    yield net, hist
+    net.close()
+
+
+def test_training_lowers_the_loss(trained):
+    _, hist = trained
+    assert np.all(np.isfinite(hist))
+    assert hist[-1] < 0.9 * hist[0], hist
+
+
+def _solve(ctx, sysm, **kw):
+    db = ctx.array(sysm.b)
+    dx = ctx.zeros(sysm.N)
+    o = S.solve_opts(**kw)
+    res = S.senk_result()
+    S.call("senk_solve", ctx.h, sysm.A, sysm.M, db.p, dx.p, C.byref(o), C.byref(res), None, 0)
+    return res
+
+
+def test_learned_fgcr_against_plain(ctx, trained):
+    # A/B on a held-out matrix of the family: FGCR(30) with the ILU application alone and
+    # with the trained correction added (FP64 outer; and in the nested FP32 inner solve)
+    net, _ = trained
+    held = N.System(ctx, 16, 777, bsize=8)
+    net.bind(held.A)
+    net.attach(held.M)
+    plain = _solve(ctx, held, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=3000, check_every=1)
+    learned = _solve(ctx, held, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=3000, check_every=1, use_learned=1)
+    print(f"\n[learned] FGCR(30) iterations: plain {plain.iters}, learned {learned.iters} "
+          f"(fell back: {learned.fell_back})")
+    assert plain.converged and learned.converged
+    assert learned.true_relres <= 1e-9
+    assert learned.fell_back == 0
+    assert learned.iters < plain.iters
+    nest_p = _solve(ctx, held, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=3000, mixed=2, inner_eps=1e-3,
+                    inner_max=30, outer_m=20)
+    nest_l = _solve(ctx, held, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=3000, mixed=2, inner_eps=1e-3,
+                    inner_max=30, outer_m=20, use_learned=1)
+    print(f"[learned] nested FP32 FGCR inner iterations: plain {nest_p.iters}, learned {nest_l.iters}")
+    assert nest_l.converged and nest_l.true_relres <= 1e-10
+    held.close()
+
+
+def test_bad_network_falls_back(ctx):
+    held = N.System(ctx, 12, 55, bsize=8)
+    net = N.Gnn(ctx, _random_params(9, wo_scale=200.0))
+    net.bind(held.A)
+    net.attach(held.M)
+    res = _solve(ctx, held, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=3000, mixed=2, inner_eps=1e-3,
+                 inner_max=30, outer_m=20, use_learned=1)
+    assert res.converged == 1 and res.true_relres <= 1e-10
+    assert res.fell_back == 1
+    net.close()
+    held.close()
