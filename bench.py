@@ -1,22 +1,28 @@
 """Benchmark of SenK's preconditioned Krylov hot path on one MI355X.
 
-Workload (one GPU): the 3D 27-point convection-diffusion system on a 128^3 grid
-(n = 2,097,152, nnz = 55,742,968; unit diffusion, central convection with velocity
-Pe*(1, 0.5, 0.25), Pe = 10), ABMC ordering with blocks of 16 ("connect"), ILU(0),
-right-hand side b = A x* with x* ~ U[-1,1] from seed 1, x0 = 0, target 1e-10 relative
-residual.  One "step" is one full solve with all inputs resident in HBM (setup is timed
-separately, as the reference's driver times only the solve).
+Headline workload (one GPU): the 3D 7-point variable-coefficient diffusion system
+-div(k grad u) = f on a 128^3 grid (n = 2,097,152, nnz = 14,581,760; k = exp(Gaussian
+random field, correlation length 8 cells, sigma 1.5), seed 1; harmonic-mean face
+coefficients), ABMC ordering ("connect"), ILU(0), right-hand side b = A x* with
+x* ~ U[-1,1] from seed 1, x0 = 0, target 1e-10 relative residual.  One step is one full
+solve with every input resident in HBM (setup timed separately, as the reference's
+driver times only the solve).
 
-Reported on the one JSON line:
-  value        time-to-1e-10 of the mixed-precision solve (FP32 matrix/ILU(0) factors and
-               inner GMRES(30), FP64 iterative refinement), ms per solve
-  fp64_gmres   the plain FP64 ILU(0)-GMRES(30) solve (time, iterations, true residual)
-  fp64_bicgstab / mixed details / SpMV GB/s (CSR and SELL-C-sigma, FP64 and FP32)
-  roofline     FP64 CSR SpMV: algorithmic bytes / average launch time (HIP events on the
-               library's stream) against 8 TB/s
-  cpu_baseline the reference's own OpenMP AbmcIluGmresm (oracle/_ref) on the host cores
+  value          time-to-1e-10 of the fastest mixed-precision solve: FP64 flexible GCR
+                 outer iteration around FP32 inner Krylov solves with FP32 SpMV and FP32
+                 ILU(0) factors (ms per solve, median of --steps)
+  fp64_gmres30   the plain FP64 ILU(0)-GMRES(30) solve it is compared with (the
+                 reference's AbmcIluGmresm algorithm), at its fastest ABMC block size
+  learned_*      the same with the trained message-passing correction of the ILU
+                 application (weights from senk_amd/data/gnn_vardiff.npz)
+  plain_fp64_c2  the 27-point convection-diffusion 128^3 system (Pe 10, ABMC blocks of
+                 16): FP64 ILU(0)-GMRES(30) and ILU(0)-BiCGStab, and SpMV rates
+  roofline       FP32 SELL-C-sigma SpMV of the headline solve: algorithmic bytes / average
+                 launch time (HIP events on the library stream) against 8 TB/s
+  cpu_baseline   the reference's own OpenMP AbmcIluGmresm (oracle/_ref) on the host cores,
+                 the full FP64 solve of the same system and factors
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--kernels-only]
 """
 from __future__ import annotations
 
@@ -37,6 +43,7 @@ sys.path.insert(0, str(ROOT))
 import senk_amd as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "time-to-1e-10 residual & iters: mixed/learned vs FP64 ILU(0)-GMRES; SpMV GB/s"
 
 
 def log(*a):
@@ -47,8 +54,8 @@ def spmv_bytes(n, nnz, sv, si=4):
     return nnz * (sv + si) + (n + 1) * si + 2 * n * sv
 
 
-def build_system(n, bsize):
-    return build_system_kind("convdiff", n, bsize)
+def ilu_bytes(n, lnnz, unnz, sv, si=4):
+    return (lnnz + unnz) * (sv + si) + 2 * (n + 1) * si + 4 * n * sv
 
 
 def build_system_kind(kind, n, bsize, seed=1):
@@ -68,19 +75,49 @@ def build_system_kind(kind, n, bsize, seed=1):
     t2 = time.perf_counter()
     val2, cind2, rptr2 = S.reordering(val, cind, rptr, LP, RP, N)
     t3 = time.perf_counter()
-    # b = A x* in the original ordering, then permuted with the rows
-    b = _host_spmv(rptr, cind, val, xs)
-    bp = b[LP].copy()
-    return dict(N=N, nnz=int(rptr[-1]), rptr=rptr2, cind=cind2, val=val2, b=bp, xs=xs[LP].copy(), nc=nc,
-                cptr=cptr, LP=LP, t_gen=t1 - t0, t_abmc=t2 - t1, t_reorder=t3 - t2)
+    rows = np.repeat(np.arange(N), np.diff(rptr))
+    b = np.bincount(rows, weights=val * xs[cind], minlength=N)
+    return dict(N=N, nnz=int(rptr[-1]), rptr=rptr2, cind=cind2, val=val2, b=b[LP].copy(), xs=xs[LP].copy(), nc=nc,
+                cptr=cptr, bsize=bsize, t_gen=t1 - t0, t_abmc=t2 - t1, t_reorder=t3 - t2)
 
 
-def _host_spmv(rptr, cind, val, x):
-    rows = np.repeat(np.arange(len(rptr) - 1), np.diff(rptr))
-    return np.bincount(rows, weights=val * x[cind], minlength=len(rptr) - 1)
+class Device:
+    """A system on the device: matrix (FP64 + FP32 values, SELL-C-sigma) and ILU(0) (FP64 + FP32)."""
+
+    def __init__(self, ctx, sy):
+        self.ctx, self.sy, self.N = ctx, sy, sy["N"]
+        self.A = C.c_void_p()
+        t0 = time.perf_counter()
+        S.call("senk_mat_create", ctx.h, self.N, S._ptr(sy["rptr"]), S._ptr(sy["cind"]), S._ptr(sy["val"]), 1 | 2,
+               C.byref(self.A))
+        self.t_upload = time.perf_counter() - t0
+        self.M = C.c_void_p()
+        opts = S.senk_ilu_opts(S.SENK_ILU0, 0, 0, 0, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32), 0)
+        t0 = time.perf_counter()
+        S.call("senk_prec_create_ilu", ctx.h, self.A, S._ptr(sy["rptr"]), S._ptr(sy["cind"]), S._ptr(sy["val"]),
+               S._ptr(sy["cptr"]), sy["nc"], sy["bsize"], C.byref(opts), C.byref(self.M))
+        self.t_ilu = time.perf_counter() - t0
+        self.db = ctx.array(sy["b"])
+        ln, un = C.c_int(), C.c_int()
+        S.call("senk_prec_factor_nnz", self.M, C.byref(ln), C.byref(un), None, None)
+        self.lnnz, self.unnz = ln.value, un.value
+
+    def solve(self, **kw):
+        dx = self.ctx.zeros(self.N)
+        o = S.solve_opts(**kw)
+        res = S.senk_result()
+        S.call("senk_solve", self.ctx.h, self.A, self.M, self.db.p, dx.p, C.byref(o), C.byref(res), None, 0)
+        d = res.as_dict()
+        d["err_vs_xstar"] = float(np.linalg.norm(dx.download() - self.sy["xs"]) / np.linalg.norm(self.sy["xs"]))
+        return d
+
+    def close(self):
+        S.lib().senk_prec_destroy(self.M)
+        S.lib().senk_mat_destroy(self.A)
 
 
 def time_kernel(ctx, fn, reps):
+    """Average time of fn() in ms, HIP events on the library's stream."""
     e0, e1 = C.c_void_p(), C.c_void_p()
     S.call("senk_event_create", ctx.h, C.byref(e0))
     S.call("senk_event_create", ctx.h, C.byref(e1))
@@ -96,73 +133,117 @@ def time_kernel(ctx, fn, reps):
     return ms.value / reps
 
 
-def solve(ctx, A, M, db, N, **kw):
-    dx = ctx.zeros(N)
-    o = S.solve_opts(**kw)
-    res = S.senk_result()
-    t0 = time.perf_counter()
-    S.call("senk_solve", ctx.h, A, M, db.p, dx.p, C.byref(o), C.byref(res), None, 0)
-    wall = (time.perf_counter() - t0) * 1e3
-    d = res.as_dict()
-    d["wall_ms"] = wall
-    return d, dx
+def kernel_rates(ctx, dev, reps=50):
+    N, nnz = dev.N, dev.sy["nnz"]
+    dx = ctx.array(np.random.default_rng(0).uniform(-1, 1, N))
+    dy = ctx.zeros(N)
+    dx32, dy32 = ctx.zeros(N, np.float32), ctx.zeros(N, np.float32)
+    S.call("senk_convert_f64_f32", ctx.h, dx.p, dx32.p, N)
+    t64 = time_kernel(ctx, lambda: S.call("senk_mat_spmv_f64", dev.A, dx.p, dy.p), reps)
+    t32 = time_kernel(ctx, lambda: S.call("senk_mat_spmv_f32", dev.A, dx32.p, dy32.p), reps)
+    p64 = time_kernel(ctx, lambda: S.call("senk_prec_apply_f64", dev.M, dx.p, dy.p), 20)
+    p32 = time_kernel(ctx, lambda: S.call("senk_prec_apply_f32", dev.M, dx32.p, dy32.p), 20)
+    out = {"spmv_sell_f64_us": t64 * 1e3, "spmv_sell_f64_GBps": spmv_bytes(N, nnz, 8) / t64 / 1e6,
+           "spmv_sell_f32_us": t32 * 1e3, "spmv_sell_f32_GBps": spmv_bytes(N, nnz, 4) / t32 / 1e6,
+           "ilu_apply_f64_us": p64 * 1e3, "ilu_apply_f64_GBps": ilu_bytes(N, dev.lnnz, dev.unnz, 8) / p64 / 1e6,
+           "ilu_apply_f32_us": p32 * 1e3, "ilu_apply_f32_GBps": ilu_bytes(N, dev.lnnz, dev.unnz, 4) / p32 / 1e6,
+           "colors": dev.sy["nc"], "bsize": dev.sy["bsize"]}
+    return out
 
 
-def cpu_baseline(sy, ctx, M, gpu_iters):
-    """The reference's AbmcIluGmresm (oracle/_ref) on the host: one restart cycle of 30
-    iterations on the same system and factors (the device factors equal the reference's
-    Ilu0 bit for bit), scaled to the GPU run's iteration count; plus 20 reference SpMVs."""
+def copy_rate(ctx, nbytes=1 << 30, reps=10):
+    """A plain device copy of known size (the counter calibration of the roofline traffic)."""
+    n = nbytes // 8
+    a, b = ctx.zeros(n), ctx.zeros(n)
+    t = time_kernel(ctx, lambda: S.call("senk_copy_f64", ctx.h, a.p, b.p, n), reps)
+    return {"copy_bytes": 2 * n * 8, "copy_us": t * 1e3, "copy_GBps": 2 * n * 8 / t / 1e6}
+
+
+def timed(dev, steps, warmup, **kw):
+    for _ in range(warmup):
+        dev.solve(**kw)
+    runs = [dev.solve(**kw) for _ in range(steps)]
+    d = runs[-1]
+    d["t_solve_ms_median"] = statistics.median(r["t_solve_ms"] for r in runs)
+    return d
+
+
+def cpu_baseline(dev, gpu_iters):
+    """The reference's AbmcIluGmresm (oracle/_ref, OpenMP) on the host: the full FP64
+    ILU(0)-GMRES(30) solve of the same reordered system with the same factors (the device
+    factors equal the reference's Ilu0 bit for bit), plus 20 reference SpMVs."""
     try:
         from oracle import oracle as O
     except Exception as e:  # pragma: no cover
         return {"error": f"oracle unavailable: {e}"}
     ref = O.ref()
-    kind = "reference"
     if ref is None:
         return {"error": "oracle/_ref not built"}
-    N = sy["N"]
-    ln, un = C.c_int(), C.c_int()
-    S.call("senk_prec_factor_nnz", M, C.byref(ln), C.byref(un), None, None)
-    lr, lc, lv = np.zeros(N + 1, np.int32), np.zeros(ln.value, np.int32), np.zeros(ln.value)
-    ur, uc, uv = np.zeros(N + 1, np.int32), np.zeros(un.value, np.int32), np.zeros(un.value)
-    S.call("senk_prec_download", M, O.P(lr), O.P(lc), O.P(lv), O.P(ur), O.P(uc), O.P(uv))
-    x = np.zeros(N)
+    sy, N = dev.sy, dev.N
+    lr, lc, lv = np.zeros(N + 1, np.int32), np.zeros(dev.lnnz, np.int32), np.zeros(dev.lnnz)
+    ur, uc, uv = np.zeros(N + 1, np.int32), np.zeros(dev.unnz, np.int32), np.zeros(dev.unnz)
+    S.call("senk_prec_download", dev.M, O.P(lr), O.P(lc), O.P(lv), O.P(ur), O.P(uc), O.P(uv))
     b = sy["b"]
     nrm_b = float(np.linalg.norm(b))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    # silence the reference's stdout lines while it runs
     sys.stdout.flush()
     saved = os.dup(1)
     devnull = os.open(os.devnull, os.O_WRONLY)
     os.dup2(devnull, 1)
     try:
+        # one cycle first: the full solve runs only if it fits the time budget
+        x = np.zeros(N)
         t0 = time.perf_counter()
         ref.ref_abmc_ilu_gmresm_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(lv), O.P(lc), O.P(lr),
-                                  O.P(uv), O.P(uc), O.P(ur), O.P(sy["cptr"]), sy["nc"], 16, O.P(b), O.P(x), nrm_b,
-                                  1, 30, N, 1e-10)
+                                  O.P(uv), O.P(uc), O.P(ur), O.P(sy["cptr"]), sy["nc"], sy["bsize"], O.P(b), O.P(x),
+                                  nrm_b, 1, 30, N, 1e-10)
         t_cycle = time.perf_counter() - t0
+        full = t_cycle / 30 * gpu_iters <= 40.0
+        if full:
+            x = np.zeros(N)
+            t0 = time.perf_counter()
+            ref.ref_abmc_ilu_gmresm_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(lv), O.P(lc), O.P(lr),
+                                      O.P(uv), O.P(uc), O.P(ur), O.P(sy["cptr"]), sy["nc"], sy["bsize"], O.P(b),
+                                      O.P(x), nrm_b, 200, 30, N, 1e-10)
+            t_full = time.perf_counter() - t0
         y = np.zeros(N)
         xx = np.random.default_rng(0).uniform(-1, 1, N)
         ref.ref_spmv_csr_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(xx), O.P(y), N)
         t0 = time.perf_counter()
-        reps = 20
-        for _ in range(reps):
+        for _ in range(20):
             ref.ref_spmv_csr_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(xx), O.P(y), N)
-        t_spmv = (time.perf_counter() - t0) / reps
+        t_spmv = (time.perf_counter() - t0) / 20
         ref.ref_flush()
     finally:
         os.dup2(saved, 1)
         os.close(saved)
         os.close(devnull)
-    per_iter_ms = t_cycle * 1e3 / 30
-    return {
-        "value": per_iter_ms * gpu_iters, "unit": "ms", "cores": cores, "kind": kind,
-        "sample": f"one restart cycle (30 iterations) of the reference AbmcIluGmresm<double>(m=30) on the same "
-                  f"128^3 system and ILU(0) factors took {t_cycle:.2f} s; value = per-iteration time x the "
-                  f"{gpu_iters} iterations the FP64 GPU solve needed",
-        "ms_per_iteration": per_iter_ms,
-        "spmv_GBps": spmv_bytes(N, sy["nnz"], 8) / t_spmv / 1e9,
-    }
+    if full:
+        rows = np.repeat(np.arange(N), np.diff(sy["rptr"]))
+        relres = float(np.linalg.norm(b - np.bincount(rows, weights=sy["val"] * x[sy["cind"]], minlength=N)) / nrm_b)
+        value, sample = t_full * 1e3, (f"the full reference AbmcIluGmresm<double>(m=30) solve to 1e-10 of the same "
+                                       f"128^3 system and ILU(0) factors (true relres {relres:.2e})")
+    else:
+        value, sample = t_cycle * 1e3 / 30 * gpu_iters, (f"one restart cycle (30 iterations) of the reference "
+                                                         f"AbmcIluGmresm<double>(m=30) took {t_cycle:.2f} s; value = "
+                                                         f"per-iteration time x the {gpu_iters} GPU iterations")
+    return {"value": value, "unit": "ms", "cores": cores, "kind": "reference", "sample": sample,
+            "ms_per_iteration": t_cycle * 1e3 / 30, "spmv_GBps": spmv_bytes(N, sy["nnz"], 8) / t_spmv / 1e9}
+
+
+def load_learned(ctx):
+    try:
+        from senk_amd import gnn as G
+        if not G.DEFAULT_WEIGHTS.exists():
+            return None, None
+        theta, meta = G.load()
+        return G.Gnn(ctx, theta), meta
+    except Exception as e:  # pragma: no cover
+        log(f"[bench] learned correction unavailable: {e}")
+        return None, None
+
+
+MIXED = dict(m=30, eps=1e-10, max_iter=6000, mixed=2, inner_eps=1e-3, outer_m=20)
 
 
 def main():
@@ -171,112 +252,96 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--bsize", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true", help="kernel rates and the copy calibration only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not supported (use --gpus 1)"}))
         sys.exit(2)
-    n, bsize = args.n, args.bsize
-    log(f"[bench] building {n}^3 27-point convection-diffusion system")
-    sy = build_system(n, bsize)
-    N, nnz = sy["N"], sy["nnz"]
-    log(f"[bench] N={N} nnz={nnz} colors={sy['nc']} (gen {sy['t_gen']:.1f}s abmc {sy['t_abmc']:.1f}s "
-        f"reorder {sy['t_reorder']:.1f}s)")
+    n = args.n
     ctx = S.Context(0)
-    A = C.c_void_p()
-    t0 = time.perf_counter()
-    S.call("senk_mat_create", ctx.h, N, S._ptr(sy["rptr"]), S._ptr(sy["cind"]), S._ptr(sy["val"]), 1 | 2, C.byref(A))
-    t_upload = time.perf_counter() - t0
-    M = C.c_void_p()
-    opts = S.senk_ilu_opts(S.SENK_ILU0, 0, 0, 0, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32))
-    t0 = time.perf_counter()
-    S.call("senk_prec_create_ilu", ctx.h, A, S._ptr(sy["rptr"]), S._ptr(sy["cind"]), S._ptr(sy["val"]),
-           S._ptr(sy["cptr"]), sy["nc"], bsize, C.byref(opts), C.byref(M))
-    t_ilu = time.perf_counter() - t0
-    log(f"[bench] upload {t_upload:.2f}s, device ILU(0) setup {t_ilu:.2f}s")
-    db = ctx.array(sy["b"])
-
-    # ---- kernel rates (HIP events on the library stream)
-    dx = ctx.array(np.random.default_rng(0).uniform(-1, 1, N))
-    dy = ctx.zeros(N)
-    dr, dc, dv = ctx.array(sy["rptr"]), ctx.array(sy["cind"]), ctx.array(sy["val"])
-    reps = 50
-    t_csr = time_kernel(ctx, lambda: S.call("senk_spmv_csr_f64", ctx.h, dv.p, dc.p, dr.p, dx.p, dy.p, N), reps)
-    t_mat = time_kernel(ctx, lambda: S.call("senk_mat_spmv_f64", A, dx.p, dy.p), reps)
-    dx32, dy32 = ctx.zeros(N, np.float32), ctx.zeros(N, np.float32)
-    t_mat32 = time_kernel(ctx, lambda: S.call("senk_mat_spmv_f32", A, dx32.p, dy32.p), reps)
-    t_prec = time_kernel(ctx, lambda: S.call("senk_prec_apply_f64", M, dx.p, dy.p), 10)
-    t_prec32 = time_kernel(ctx, lambda: S.call("senk_prec_apply_f32", M, dx32.p, dy32.p), 10)
-    ln, un, nl, nu = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-    S.call("senk_prec_factor_nnz", M, C.byref(ln), C.byref(un), C.byref(nl), C.byref(nu))
-    b64 = spmv_bytes(N, nnz, 8)
-    b32 = spmv_bytes(N, nnz, 4)
-    ilu_bytes64 = (ln.value + un.value) * 12 + 2 * (N + 1) * 4 + 4 * N * 8
-    ilu_bytes32 = (ln.value + un.value) * 8 + 2 * (N + 1) * 4 + 4 * N * 4
-    del dv, dc, dr
-    kern = {
-        "spmv_csr_f64_us": t_csr * 1e3, "spmv_csr_f64_GBps": b64 / (t_csr * 1e-3) / 1e9,
-        "spmv_sellcs_f64_us": t_mat * 1e3, "spmv_sellcs_f64_GBps": b64 / (t_mat * 1e-3) / 1e9,
-        "spmv_sellcs_f32_us": t_mat32 * 1e3, "spmv_sellcs_f32_GBps": b32 / (t_mat32 * 1e-3) / 1e9,
-        "ilu_apply_f64_us": t_prec * 1e3, "ilu_apply_f64_GBps": ilu_bytes64 / (t_prec * 1e-3) / 1e9,
-        "ilu_apply_f32_us": t_prec32 * 1e3, "ilu_apply_f32_GBps": ilu_bytes32 / (t_prec32 * 1e-3) / 1e9,
-        "colors": sy["nc"],
-    }
-    log(f"[bench] kernels: {json.dumps(kern)}")
-
-    # ---- solves
-    runs = {"fp64_gmres": dict(solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000),
-            "fp64_gmres_cgs2": dict(solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000, ortho=S.SENK_CGS2),
-            "fp64_bicgstab": dict(solver=S.SENK_BICGSTAB, m=30, eps=1e-10, max_iter=6000),
-            "mixed_gmres": dict(solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000, mixed=1, inner_eps=1e-5,
-                                inner_max=600, ortho=S.SENK_CGS2)}
-    results = {}
-    for name, kw in runs.items():
-        for _ in range(args.warmup):
-            solve(ctx, A, M, db, N, **kw)
-        times, last = [], None
-        for _ in range(args.steps if name == "mixed_gmres" else max(1, min(args.steps, 2))):
-            d, dxs = solve(ctx, A, M, db, N, **kw)
-            times.append(d["t_solve_ms"])
-            last = d
-        x = dxs.download()
-        last["err_vs_xstar"] = float(np.linalg.norm(x - sy["xs"]) / np.linalg.norm(sy["xs"]))
-        last["t_solve_ms_median"] = statistics.median(times)
-        results[name] = last
-        log(f"[bench] {name}: {json.dumps(last)}")
-    # error of the mixed solution against the FP64 solve
-    ref64 = results["fp64_gmres"]
-    mixed = results["mixed_gmres"]
+    log(f"[bench] building the {n}^3 variable-coefficient diffusion system (ABMC blocks of 8 and 4)")
+    sy8 = build_system_kind("vardiff", n, 8)
+    d8 = Device(ctx, sy8)
+    kern = kernel_rates(ctx, d8)
+    cal = copy_rate(ctx)
+    log(f"[bench] kernels: {json.dumps(kern)} copy: {json.dumps(cal)}")
+    if args.kernels_only:
+        print(json.dumps({"kernels": kern, "copy": cal}), flush=True)
+        return
+    sy4 = build_system_kind("vardiff", n, 4)
+    d4 = Device(ctx, sy4)
+    K, W = args.steps, args.warmup
+    runs = {}
+    # plain FP64 (the reference's method) at both block sizes; the baseline is the faster
+    for tag, dev in (("b8", d8), ("b4", d4)):
+        runs[f"fp64_gmres30_{tag}"] = timed(dev, K, W, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000)
+        log(f"[bench] fp64_gmres30_{tag}: {json.dumps(runs[f'fp64_gmres30_{tag}'])}")
+    runs["fp64_bicgstab_b8"] = timed(d8, K, W, solver=S.SENK_BICGSTAB, m=30, eps=1e-10, max_iter=6000)
+    for tag, dev in (("b8", d8), ("b4", d4)):
+        runs[f"mixed_bicgstab_{tag}"] = timed(dev, K, W, solver=S.SENK_BICGSTAB, inner_max=40, **MIXED)
+        runs[f"mixed_fgcr_{tag}"] = timed(dev, K, W, solver=S.SENK_FGCR, inner_max=30, **MIXED)
+        log(f"[bench] mixed_{tag}: {json.dumps(runs[f'mixed_bicgstab_{tag}'])}")
+    net, meta = load_learned(ctx)
+    if net is not None:
+        net.bind(d8.A)
+        net.attach(d8.M)
+        runs["learned_fgcr30_fp64_b8"] = timed(d8, 1, 1, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
+                                               check_every=1, use_learned=1)
+        runs["plain_fgcr30_fp64_b8"] = timed(d8, 1, 1, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
+                                             check_every=1)
+        runs["learned_mixed_fgcr_b8"] = timed(d8, 1, 1, solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED)
+        dx32, dz32 = ctx.zeros(d8.N, np.float32), ctx.zeros(d8.N, np.float32)
+        kern["learned_apply_f32_us"] = 1e3 * time_kernel(
+            ctx, lambda: S.call("senk_gnn_apply_f32", net.h, dx32.p, dz32.p), 10)
+        S.call("senk_prec_attach_gnn", d8.M, None)
+        net.close()
+    base_key = min(("fp64_gmres30_b8", "fp64_gmres30_b4"), key=lambda k: runs[k]["t_solve_ms_median"])
+    mixed_key = min((k for k in runs if k.startswith("mixed_")), key=lambda k: runs[k]["t_solve_ms_median"])
+    base, best = runs[base_key], runs[mixed_key]
     cpu = None
     if not args.no_cpu:
-        log("[bench] CPU baseline (reference OpenMP path)")
-        cpu = cpu_baseline(sy, ctx, M, ref64["iters"])
+        log("[bench] CPU baseline (reference OpenMP AbmcIluGmresm)")
+        cpu = cpu_baseline(d8 if base_key.endswith("b8") else d4, base["iters"])
         log(f"[bench] cpu: {json.dumps(cpu)}")
-
-    value = mixed["t_solve_ms_median"]
+    c2 = None
+    if not args.no_c2:
+        log(f"[bench] building the {n}^3 27-point convection-diffusion system (ABMC blocks of 16)")
+        syc = build_system_kind("convdiff", n, 16)
+        dc = Device(ctx, syc)
+        kc = kernel_rates(ctx, dc)
+        c2 = {"kernels": kc,
+              "fp64_gmres30": timed(dc, 2, 1, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000),
+              "fp64_bicgstab": timed(dc, 2, 1, solver=S.SENK_BICGSTAB, eps=1e-10, max_iter=6000),
+              "mixed_bicgstab": timed(dc, 2, 1, solver=S.SENK_BICGSTAB, inner_max=40, **MIXED)}
+        log(f"[bench] c2: {json.dumps(c2)}")
+        dc.close()
+    keep = ("iters", "restarts", "converged", "fell_back", "t_solve_ms_median", "true_relres", "err_vs_xstar")
+    value = best["t_solve_ms_median"]
     line = {
-        "metric": "time-to-1e-10 residual & iters: mixed/learned vs FP64 ILU(0)-GMRES; SpMV GB/s",
-        "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "metric": METRIC, "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 inner + f64 refinement (fp64 path reported beside it)", "data": "synthetic",
-        "config": {"workload": f"convdiff27 {n}^3 Pe=10, ABMC bsize {bsize} connect, ILU(0), tol 1e-10",
-                   "n": N, "nnz": nnz},
-        "fp64_gmres30": {k: ref64[k] for k in ("iters", "t_solve_ms_median", "true_relres", "err_vs_xstar")},
-        "fp64_gmres30_cgs2": {k: results["fp64_gmres_cgs2"][k] for k in ("iters", "t_solve_ms_median", "true_relres")},
-        "fp64_bicgstab": {k: results["fp64_bicgstab"][k] for k in ("iters", "t_solve_ms_median", "true_relres")},
-        "mixed_ir_gmres30": {k: mixed[k] for k in ("iters", "restarts", "t_solve_ms_median", "true_relres",
-                                                     "err_vs_xstar")},
-        "speedup_mixed_vs_fp64_gmres": round(ref64["t_solve_ms_median"] / value, 3),
-        "kernels": kern,
-        "setup_s": {"abmc": sy["t_abmc"], "reorder": sy["t_reorder"], "device_ilu0_and_split": t_ilu},
-        "roofline": {"kernel": "spmv_csr_f64", "bound": "hbm", "achieved": round(kern["spmv_csr_f64_GBps"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(kern["spmv_csr_f64_GBps"] / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+        "dtype": "f32 inner solve + f64 outer iteration", "data": "synthetic",
+        "config": {"workload": f"vardiff7 {n}^3 (k = exp(GRF, corr 8, sigma 1.5), seed 1), ABMC connect, ILU(0), "
+                               f"tol 1e-10", "n": d8.N, "nnz": sy8["nnz"], "mixed": mixed_key,
+                   "fp64_baseline": base_key},
+        "speedup_vs_fp64_gmres30": round(base["t_solve_ms_median"] / value, 3),
+        "runs": {k: {q: v[q] for q in keep if q in v} for k, v in runs.items()},
+        "kernels": kern, "copy_calibration": cal,
+        "learned_training": None if meta is None else {k: meta[k] for k in ("count", "sizes", "epochs", "loss_hist",
+                                                                              "t_train_s", "t_data_s") if k in meta},
+        "plain_fp64_c2": None if c2 is None else {
+            "kernels": c2["kernels"], **{k: {q: v[q] for q in keep if q in v} for k, v in c2.items() if k != "kernels"}},
+        "setup_s": {"abmc": sy8["t_abmc"], "reorder": sy8["t_reorder"], "upload": d8.t_upload,
+                    "device_ilu0_split_layout": d8.t_ilu},
+        "roofline": {"kernel": "spmv_sellcs_f32 (vardiff7 128^3)", "bound": "hbm",
+                     "achieved": round(kern["spmv_sell_f32_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
     }
-    S.lib().senk_prec_destroy(M)
-    S.lib().senk_mat_destroy(A)
+    d8.close()
+    d4.close()
     print(json.dumps(line), flush=True)
 
 

@@ -46,6 +46,10 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
     return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+// Stored position of feature f (= 8t + 4h + i) inside a node's 32 BF16 features:
+// 16h + 4t + i, so that the 16 features one lane feeds to the matrix core (half h,
+// t = 0..3) are 32 contiguous bytes.
+__device__ __forceinline__ int fpos(int f) { return ((f >> 2) & 1) * 16 + (f >> 3) * 4 + (f & 3); }
 
 // ----------------------------------------------------------------- state
 struct GnnGraph {
@@ -162,13 +166,13 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
         xkeep[i] = make_float2(x0, x1);
         m1keep[i] = make_float2(m0, m1);
     }
-    unsigned short out[W];
+    unsigned short out[W];  // in stored order
 #pragma unroll
     for (int o = 0; o < W; o++) {
         float v = th[ob1 + o];
         v += x0 * th[oW1s + o] + x1 * th[oW1s + W + o];
         v += m0 * th[oW1n + o] + m1 * th[oW1n + W + o];
-        out[o] = f2bf(fmaxf(v, 0.f));
+        out[((o >> 2) & 1) * 16 + (o >> 3) * 4 + (o & 3)] = f2bf(fmaxf(v, 0.f));
     }
     uint4 *dst = reinterpret_cast<uint4 *>(h1 + (size_t)i * W);
 #pragma unroll
@@ -183,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
 }
 // Layers 2 and 3 on the matrix cores: one wave per tile of 32 nodes.  Lane l feeds node
 // n0 + l%32; its A operand for k-step t (t < 4: own features, t >= 4: aggregated) is
-// features 8t + 4*(l/32) + [0,4) of that node.  D element (row, col) sits in lane
+// features 8t + 4*(l/32) + [0,4) of that node (stored contiguously, see fpos).  D element (row, col) sits in lane
 // col + 32*((row/4)%2), register (row%4) + 4*(row/8).
 template <bool LAST>
 __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict__ rptr, const int *__restrict__ cind,
@@ -203,17 +207,39 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
 #pragma unroll
     for (int q = 0; q < 16; q++) m[q] = 0;
     if (act) {
+        // own features: 32 contiguous bytes (stored order), k-steps 0..3
+        const uint4 *hs = reinterpret_cast<const uint4 *>(hin + (size_t)node * W + 16 * half);
+        const uint4 u0 = hs[0], u1 = hs[1];
+        a[0] = bf16x4{(short)(u0.x & 0xFFFF), (short)(u0.x >> 16), (short)(u0.y & 0xFFFF), (short)(u0.y >> 16)};
+        a[1] = bf16x4{(short)(u0.z & 0xFFFF), (short)(u0.z >> 16), (short)(u0.w & 0xFFFF), (short)(u0.w >> 16)};
+        a[2] = bf16x4{(short)(u1.x & 0xFFFF), (short)(u1.x >> 16), (short)(u1.y & 0xFFFF), (short)(u1.y >> 16)};
+        a[3] = bf16x4{(short)(u1.z & 0xFFFF), (short)(u1.z >> 16), (short)(u1.w & 0xFFFF), (short)(u1.w >> 16)};
+        // aggregation: four neighbours' loads in flight at a time
+        const int e = rptr[node + 1];
+        for (int j0 = rptr[node]; j0 < e; j0 += 4) {
+            float wgt[4];
+            uint4 hv[4][2];
 #pragma unroll
-        for (int t = 0; t < 4; t++) a[t] = *reinterpret_cast<const bf16x4 *>(hin + (size_t)node * W + 8 * t + 4 * half);
-        for (int j = rptr[node]; j < rptr[node + 1]; j++) {
-            const int cj = cind[j];
-            const float wgt = val[j] * s[cj];
-            const unsigned short *hj = hin + (size_t)cj * W + 4 * half;
+            for (int q = 0; q < 4; q++) {
+                if (j0 + q < e) {
+                    const int cj = cind[j0 + q];
+                    wgt[q] = val[j0 + q] * s[cj];
+                    const uint4 *hj = reinterpret_cast<const uint4 *>(hin + (size_t)cj * W + 16 * half);
+                    hv[q][0] = hj[0];
+                    hv[q][1] = hj[1];
+                }
+            }
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const bf16x4 hv = *reinterpret_cast<const bf16x4 *>(hj + 8 * t);
+            for (int q = 0; q < 4; q++) {
+                if (j0 + q < e) {
+                    const unsigned wd[8] = {hv[q][0].x, hv[q][0].y, hv[q][0].z, hv[q][0].w,
+                                            hv[q][1].x, hv[q][1].y, hv[q][1].z, hv[q][1].w};
 #pragma unroll
-                for (int i = 0; i < 4; i++) m[4 * t + i] += wgt * bf2f((unsigned short)hv[i]);
+                    for (int u = 0; u < 8; u++) {
+                        m[2 * u] += wgt[q] * bf2f((unsigned short)(wd[u] & 0xFFFF));
+                        m[2 * u + 1] += wgt[q] * bf2f((unsigned short)(wd[u] >> 16));
+                    }
+                }
             }
         }
         const float si = s[node];
@@ -229,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
 #pragma unroll
         for (int i = 0; i < 4; i++) v[i] = (short)f2bf(m[4 * t + i]);
         a[4 + t] = v;
-        if (mkeep && act) *reinterpret_cast<bf16x4 *>(mkeep + (size_t)node * W + 8 * t + 4 * half) = v;
+        if (mkeep && act) *reinterpret_cast<bf16x4 *>(mkeep + (size_t)node * W + 16 * half + 4 * t) = v;
     }
     f32x16 acc;
 #pragma unroll
@@ -242,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
         const int row = 8 * (r >> 2) + 4 * half + (r & 3);
         const int nd = n0 + row;
         const unsigned short hv = f2bf(fmaxf(acc[r] + b, 0.f));
-        if (hout && nd < N) hout[(size_t)nd * W + col] = hv;
+        if (hout && nd < N) hout[(size_t)nd * W + fpos(col)] = hv;
         if (LAST) red[wave][row][col] = bf2f(hv) * wo[col];
     }
     if (LAST) {
@@ -274,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_out(int N, const float *__restric
     const float d = dz[i] * (float)sc[0] * s[i];
     dchat[i] = d;
 #pragma unroll
-    for (int o = 0; o < W; o++) G[(size_t)i * W + o] = bf2f(h3[(size_t)i * W + o]) > 0 ? d * th[owo + o] : 0.f;
+    for (int o = 0; o < W; o++) G[(size_t)i * W + o] = bf2f(h3[(size_t)i * W + fpos(o)]) > 0 ? d * th[owo + o] : 0.f;
 }
 // P = G Ws^T, Q = G Wn^T (per node)
 __global__ __launch_bounds__(kBlock) void k_gb_pq(int N, const float *__restrict__ G, const float *__restrict__ Ws,
@@ -315,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_agg(int N, const int *__restrict_
 #pragma unroll
     for (int k = 0; k < W; k++) {
         const float dh = P[(size_t)i * W + k] + si * acc[k];
-        Gprev[(size_t)i * W + k] = bf2f(hprev[(size_t)i * W + k]) > 0 ? dh : 0.f;
+        Gprev[(size_t)i * W + k] = bf2f(hprev[(size_t)i * W + fpos(k)]) > 0 ? dh : 0.f;
     }
 }
 // Weight gradients of a matrix-core layer over a contiguous node range per workgroup:
@@ -332,8 +358,8 @@ __global__ __launch_bounds__(kBlock) void k_gb_wmc(int N, const unsigned short *
         for (int q = threadIdx.x; q < 32 * W; q += kBlock) {
             const int r = q / W, f = q % W, n = n0 + r;
             const bool ok = n < hi;
-            th[r][f] = ok ? bf2f(hin[(size_t)n * W + f]) : 0.f;
-            tm[r][f] = ok ? bf2f(mag[(size_t)n * W + f]) : 0.f;
+            th[r][f] = ok ? bf2f(hin[(size_t)n * W + fpos(f)]) : 0.f;
+            tm[r][f] = ok ? bf2f(mag[(size_t)n * W + fpos(f)]) : 0.f;
             tg[r][f] = ok ? G[(size_t)n * W + f] : 0.f;
         }
         __syncthreads();
@@ -377,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_w1o(int N, const float2 *__restri
             a[4] += g;
         }
     } else if (t < 2 * W) {
-        for (int n = lo; n < hi; n++) a[0] += bf2f(h3[(size_t)n * W + t - W]) * dchat[n];
+        for (int n = lo; n < hi; n++) a[0] += bf2f(h3[(size_t)n * W + fpos(t - W)]) * dchat[n];
     } else if (t == 2 * W) {
         for (int n = lo; n < hi; n++) a[0] += dchat[n];
     }
