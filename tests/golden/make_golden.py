@@ -97,8 +97,8 @@ def small_case(name, rptr, cind, val, bsizes, ps):
     print(f"wrote {name}.npz")
 
 
-def c1_case():
-    """c1: 7-point Poisson 32^3, ABMC bsize 4 "connect", ILU(0), AbmcIluGmresm m=30 to
+def poisson32_case():
+    """7-point Poisson 32^3, ABMC bsize 4 "connect", ILU(0), AbmcIluGmresm m=30 to
     1e-8 with b = A x*, x* ~ U[-1,1] (seed 1)."""
     ref = O.ref()
     rptr, cind, val = S.gen_poisson7(32)
@@ -125,12 +125,12 @@ def c1_case():
     t = np.zeros(N)
     ref.ref_spmv_csr_d(P(v), P(c), P(r), P(x), P(t), N)
     true_rel = float(np.linalg.norm(bp - t) / np.linalg.norm(bp))
-    np.savez_compressed(OUT / "c1_poisson32_abmc4.npz", LP=lp, size_color=sc[: nc + 1], num_color=np.array(nc),
+    np.savez_compressed(OUT / "poisson7_32_abmc4.npz", LP=lp, size_color=sc[: nc + 1], num_color=np.array(nc),
                         gmres_iters=np.array(it))
-    meta = {"config": "c1: poisson7 32^3, ABMC bsize 4 connect, ILU(0), AbmcIluGmresm m=30 eps=1e-8, b=A x*",
+    meta = {"config": "poisson7 32^3, ABMC bsize 4 connect, ILU(0), AbmcIluGmresm m=30 eps=1e-8, b=A x*",
             "num_color": int(nc), "gmres_iters": it, "res_line": res_line, "true_relres": true_rel,
             "sha256_LP": sha(lp), "sha256_ilu0": sha(f), "sha256_size_color": sha(sc[: nc + 1])}
-    (OUT / "c1_poisson32_abmc4.json").write_text(json.dumps(meta, indent=1) + "\n")
+    (OUT / "poisson7_32_abmc4.json").write_text(json.dumps(meta, indent=1) + "\n")
     print("wrote c1", meta)
 
 
@@ -164,12 +164,12 @@ def main():
     small_case("unstructured_300", *S.gen_unstructured(300, 4), bsizes=(4, 10), ps=(1, 2))
     small_case("elasticity_3", *S.gen_elasticity3(3, 4), bsizes=(3, 9), ps=(1,))
     small_case("vardiff7_6", *S.gen_vardiff7(6, 3, 4.0, 1.5), bsizes=(4, 12), ps=(1,))
-    c1_case()
+    poisson32_case()
     if "--full" in sys.argv:
-        digest_case("c2_convdiff128_abmc16", lambda: S.gen_convdiff27(128, 10.0), 16)
-        digest_case("c4_vardiff128_abmc16", lambda: S.gen_vardiff7(128, 1, 8.0, 1.5), 16)
-        digest_case("c5_elasticity96_abmc48", lambda: S.gen_elasticity3(96, 1), 48, ilup=1)
-        digest_case("c3_unstructured4M_abmc16", lambda: S.gen_unstructured(4194304, 1), 16)
+        digest_case("convdiff27_128_abmc16", lambda: S.gen_convdiff27(128, 10.0), 16)
+        digest_case("vardiff7_128_abmc16", lambda: S.gen_vardiff7(128, 1, 8.0, 1.5), 16)
+        digest_case("elasticity3_96_abmc48", lambda: S.gen_elasticity3(96, 1), 48, ilup=1)
+        digest_case("unstructured_4194304_abmc16", lambda: S.gen_unstructured(4194304, 1), 16)
 
 
 if __name__ == "__main__":

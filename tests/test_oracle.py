@@ -406,7 +406,7 @@ def test_krylov_drivers_match_reference(capfd, solver):
 
 # ------------------------------------------------------------------ golden fixtures
 def _golden():
-    return sorted(p for p in GOLDEN.glob("*.npz") if not p.name.startswith("c1_"))
+    return sorted(p for p in GOLDEN.glob("*.npz") if not p.name.startswith("poisson7_32_"))
 
 
 @pytest.mark.skipif(not _golden(), reason="no golden fixtures")
@@ -446,10 +446,10 @@ def test_oracle_against_golden(path):
         assert abs(it - int(g["gmres_iters"])) <= 1
 
 
-def test_oracle_c1_against_reference_fixture():
-    """c1 (7-point Poisson 32^3, ABMC bsize 4 connect, ILU(0)-GMRES(30) to 1e-8): the
+def test_oracle_poisson32_against_reference_fixture():
+    """7-point Poisson 32^3, ABMC bsize 4 connect, ILU(0)-GMRES(30) to 1e-8: the
     reference's ordering arrays and iteration count, as committed fixtures."""
-    g = np.load(GOLDEN / "c1_poisson32_abmc4.npz", allow_pickle=False)
+    g = np.load(GOLDEN / "poisson7_32_abmc4.npz", allow_pickle=False)
     rptr, cind, val = S.gen_poisson7(32)
     N = len(rptr) - 1
     sc, lp, rp = np.zeros(N + 1, np.int32), np.zeros(N, np.int32), np.zeros(N, np.int32)

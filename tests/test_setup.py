@@ -192,8 +192,8 @@ def test_csr2csc():
     assert np.array_equal(At, A.T)
 
 
-def test_c1_ordering_matches_reference_fixture():
-    g = np.load(GOLDEN / "c1_poisson32_abmc4.npz", allow_pickle=False)
+def test_poisson32_ordering_matches_reference_fixture():
+    g = np.load(GOLDEN / "poisson7_32_abmc4.npz", allow_pickle=False)
     rptr, cind, val = S.gen_poisson7(32)
     N = len(rptr) - 1
     nc, sc, LP, RP = S.abmc_permutation(cind, rptr, N, 4, False, "connect")
@@ -210,13 +210,13 @@ def _sha(*arrays):
 
 
 @pytest.mark.parametrize("name,gen,bsize", [
-    ("c2_convdiff128_abmc16", lambda: S.gen_convdiff27(128, 10.0), 16),
-    ("c4_vardiff128_abmc16", lambda: S.gen_vardiff7(128, 1, 8.0, 1.5), 16),
-    ("c5_elasticity96_abmc48", lambda: S.gen_elasticity3(96, 1), 48),
-    ("c3_unstructured4M_abmc16", lambda: S.gen_unstructured(4194304, 1), 16),
+    ("convdiff27_128_abmc16", lambda: S.gen_convdiff27(128, 10.0), 16),
+    ("vardiff7_128_abmc16", lambda: S.gen_vardiff7(128, 1, 8.0, 1.5), 16),
+    ("elasticity3_96_abmc48", lambda: S.gen_elasticity3(96, 1), 48),
+    ("unstructured_4194304_abmc16", lambda: S.gen_unstructured(4194304, 1), 16),
 ])
 def test_full_size_ordering_digest(name, gen, bsize):
-    """The benchmark systems' orderings (and c5's ILU(1) pattern) against the
+    """The benchmark systems' orderings (and the elasticity system's ILU(1) pattern) against the
     reference's digests at full size."""
     path = GOLDEN / f"{name}.json"
     if not path.exists():
