@@ -159,6 +159,24 @@ def copy_rate(ctx, nbytes=1 << 30, reps=10):
     return {"copy_bytes": 2 * n * 8, "copy_us": t * 1e3, "copy_GBps": 2 * n * 8 / t / 1e6}
 
 
+def traffic_from_profile(path=ROOT / "profiles" / "spmv_sell_f32_vardiff128.md"):
+    """HBM bytes per SpMV launch from the committed counter summary: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for a device copy
+    of known size."""
+    try:
+        vals = {}
+        for line in path.read_text().splitlines():
+            if line.startswith("traffic:"):  # "traffic: key=value key=value ..."
+                for item in line.split(":", 1)[1].split():
+                    k, v = item.split("=")
+                    vals[k] = float(v)
+        fr = vals["copy_read_bytes"] / (vals["copy_fetch_kib"] * 1024)
+        fw = vals["copy_write_bytes"] / (vals["copy_write_kib"] * 1024)
+        return int(vals["fetch_kib"] * 1024 * fr + vals["write_kib"] * 1024 * fw)
+    except Exception:
+        return None
+
+
 def timed(dev, steps, warmup, **kw):
     for _ in range(warmup):
         dev.solve(**kw)
@@ -255,6 +273,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--kernels-only", action="store_true", help="kernel rates and the copy calibration only")
+    ap.add_argument("--run", default=None, help="time only the solve configuration of this name (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not supported (use --gpus 1)"}))
@@ -273,6 +292,17 @@ def main():
     sy4 = build_system_kind("vardiff", n, 4)
     d4 = Device(ctx, sy4)
     K, W = args.steps, args.warmup
+    if args.run:
+        # one configuration alone (for a kernel trace of exactly that solve)
+        cfgs = {"fp64_gmres30_b8": (d8, dict(solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000)),
+                "mixed_bicgstab_b4": (d4, dict(solver=S.SENK_BICGSTAB, inner_max=40, **MIXED)),
+                "mixed_bicgstab_b8": (d8, dict(solver=S.SENK_BICGSTAB, inner_max=40, **MIXED))}
+        dev, kw = cfgs[args.run]
+        r = timed(dev, K, W, **kw)
+        print(json.dumps({"run": args.run, **r}), flush=True)
+        d8.close()
+        d4.close()
+        return
     runs = {}
     # plain FP64 (the reference's method) at both block sizes; the baseline is the faster
     for tag, dev in (("b8", d8), ("b4", d4)):
@@ -337,7 +367,8 @@ def main():
                     "device_ilu0_split_layout": d8.t_ilu},
         "roofline": {"kernel": "spmv_sellcs_f32 (vardiff7 128^3)", "bound": "hbm",
                      "achieved": round(kern["spmv_sell_f32_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(kern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(kern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile()},
         "cpu_baseline": cpu,
     }
     d8.close()
