@@ -148,6 +148,14 @@ def kernel_rates(ctx, dev, reps=50):
            "ilu_apply_f64_us": p64 * 1e3, "ilu_apply_f64_GBps": ilu_bytes(N, dev.lnnz, dev.unnz, 8) / p64 / 1e6,
            "ilu_apply_f32_us": p32 * 1e3, "ilu_apply_f32_GBps": ilu_bytes(N, dev.lnnz, dev.unnz, 4) / p32 / 1e6,
            "colors": dev.sy["nc"], "bsize": dev.sy["bsize"]}
+    # triangular-solve launch modes: one launch per level, then the default single sweep
+    for mode in (0, 1):
+        S.call("senk_prec_set_trsv_mode", dev.M, mode)
+        out[f"ilu_apply_f64_us_mode{mode}"] = time_kernel(
+            ctx, lambda: S.call("senk_prec_apply_f64", dev.M, dx.p, dy.p), 20) * 1e3
+        out[f"ilu_apply_f32_us_mode{mode}"] = time_kernel(
+            ctx, lambda: S.call("senk_prec_apply_f32", dev.M, dx32.p, dy32.p), 20) * 1e3
+    S.call("senk_prec_check", dev.M)
     return out
 
 

@@ -39,7 +39,9 @@ enum senk_status {
     SENK_ERR_NO_DEVICE = 7,      /* no HIP device                                              */
     SENK_ERR_ALLOC = 8,          /* allocation failure (senk_utils.hpp:26,38,51)               */
     SENK_ERR_UNSUPPORTED = 9,    /* shape outside what the kernels support                     */
-    SENK_ERR_SCHEDULE = 10       /* a factor entry couples two blocks scheduled together       */
+    SENK_ERR_SCHEDULE = 10,      /* a factor entry couples two blocks scheduled together       */
+    SENK_ERR_TIMEOUT = 11        /* a single-launch triangular solve gave up waiting on an
+                                    earlier level (the result of that solve is invalid)      */
 };
 const char *senk_status_string(int status);
 /* Last HIP error text recorded by the library (thread-local), for diagnostics. */
@@ -387,6 +389,15 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
 int senk_prec_destroy(senk_prec M);
 int senk_prec_apply_f64(senk_prec M, const double *dx, double *dy);
 int senk_prec_apply_f32(senk_prec M, const float *dx, float *dy);
+/* Triangular-solve launch mode of the sliced layout (no reference counterpart; the
+ * reference runs one OpenMP parallel loop per color, senk_sparse.hpp:207-246):
+ * 0 one launch per level; 1 (default) one launch per triangle, each gather waiting
+ * until the row it reads is final.  Results are bit-identical in both modes.
+ * Applications of one preconditioner must not overlap (one stream). */
+int senk_prec_set_trsv_mode(senk_prec M, int mode);
+/* Synchronise and report SENK_ERR_TIMEOUT if a single-launch solve since the last check
+ * gave up a wait (never expected; senk_solve checks by itself). */
+int senk_prec_check(senk_prec M);
 int senk_prec_factor_nnz(senk_prec M, int *lnnz, int *unnz, int *nlevels_l, int *nlevels_u);
 /* Copy the device factors back (host arrays sized by senk_prec_factor_nnz). */
 int senk_prec_download(senk_prec M, int *lrptr, int *lcind, double *lval, int *urptr, int *ucind, double *uval);

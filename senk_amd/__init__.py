@@ -26,6 +26,7 @@ STATUS = {
     0: "SENK_OK", 1: "SENK_ERR_ARG", 2: "SENK_ERR_HIP", 3: "SENK_ERR_ZERO_PIVOT",
     4: "SENK_ERR_DIVISIBILITY", 5: "SENK_ERR_BREAKDOWN", 6: "SENK_ERR_NOT_CONVERGED",
     7: "SENK_ERR_NO_DEVICE", 8: "SENK_ERR_ALLOC", 9: "SENK_ERR_UNSUPPORTED", 10: "SENK_ERR_SCHEDULE",
+    11: "SENK_ERR_TIMEOUT",
 }
 SENK_GMRES, SENK_FGMRES, SENK_GCR, SENK_FGCR, SENK_BICGSTAB = range(5)
 SENK_MGS, SENK_CGS2 = 0, 1

@@ -163,6 +163,7 @@ const char *senk_status_string(int s) {
         case SENK_ERR_ALLOC: return "allocation failure";
         case SENK_ERR_UNSUPPORTED: return "unsupported shape";
         case SENK_ERR_SCHEDULE: return "factor entry couples two blocks of one launch";
+        case SENK_ERR_TIMEOUT: return "triangular solve timed out waiting on an earlier level";
         default: return "unknown status";
     }
 }
@@ -1229,6 +1230,19 @@ int senk_prec_apply_f32(senk_prec M, const float *dx, float *dy) {
     if (!M || !M->view.l32) return SENK_ERR_ARG;
     return prec_apply<float>(&M->ctx->c, M->view, dx, nullptr, dy, nullptr, nullptr);
 }
+int senk_prec_set_trsv_mode(senk_prec M, int mode) {
+    if (!M || mode < senk_amd::kTrsvLevels || mode > senk_amd::kTrsvSweep) return SENK_ERR_ARG;
+    SENK_HIP(hipStreamSynchronize(M->ctx->c.stream));
+    M->lt.mode = mode;
+    M->ut.mode = mode;
+    return SENK_OK;
+}
+int senk_prec_check(senk_prec M) {
+    if (!M) return SENK_ERR_ARG;
+    const int a = senk_amd::tri_layout_check(&M->ctx->c, &M->lt);
+    const int b = senk_amd::tri_layout_check(&M->ctx->c, &M->ut);
+    return a ? a : b;
+}
 int senk_prec_factor_nnz(senk_prec M, int *lnnz, int *unnz, int *nlev_l, int *nlev_u) {
     if (!M) return SENK_ERR_ARG;
     if (lnnz) *lnnz = M->lnnz;
@@ -1305,6 +1319,7 @@ int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *
     SENK_TRY(true_relres(&ctx->c, A->view, db, dx, nrm_b, &tr));
     res->true_relres = tr;
     if (lp) res->fell_back = lp->off ? 1 : 0;
+    if (M) SENK_TRY(senk_prec_check(M));
     return res->status;
 }
 

@@ -10,6 +10,17 @@ namespace senk_amd {
 
 constexpr int kBlock = 256;           // threads per workgroup (4 wave64)
 constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kXcds = 8;               // MI355X: 8 accelerator dies, one L2 each
+
+// The dispatcher hands workgroup b to XCD b % 8.  Renumbered, XCD j works through the
+// j-th contiguous eighth of the grid, so that neighbouring rows -- and the vector
+// entries they gather -- share one L2 instead of being fetched by every XCD.  A
+// bijection on [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block() {
+    const int n = (int)gridDim.x, b = (int)blockIdx.x;
+    const int x = b % kXcds, per = n / kXcds, rem = n % kXcds;
+    return x * per + (x < rem ? x : rem) + b / kXcds;
+}
 
 // One DPP move of a double (two 32-bit halves).  Lanes whose source is outside the
 // row or whose row is masked off receive 0.

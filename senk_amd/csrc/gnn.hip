@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
                                                    const float2 *__restrict__ xin, const float2 *__restrict__ xs,
                                                    const double *sc, const float *__restrict__ th,
                                                    unsigned short *h1, float2 *xkeep, float2 *m1keep) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= N) return;
     const float inv = (float)sc[1];
     float a0 = 0, a1 = 0;
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
                                                    const float *__restrict__ bo, float *chat) {
     __shared__ float red[kWavesPerBlock][32][33];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n0 = (blockIdx.x * kWavesPerBlock + wave) * 32;
+    const int n0 = (xcd_block() * kWavesPerBlock + wave) * 32;
     const int half = lane >> 5, col = lane & 31;
     const int node = n0 + col;
     const bool act = node < N;

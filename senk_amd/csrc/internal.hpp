@@ -82,28 +82,38 @@ struct Schedule {
 // slices of 64 rows; per row the external entries (value + column) and the in-block
 // entries (value; column = bit of `mask`), each group stored packed and in the order
 // the solve consumes it.  Built once per pattern, filled per precision.
+// Launch modes of the layout solve: one launch per level, or one launch per triangle
+// whose gathers wait for the rows they read (sentinel-initialised output).
+constexpr int kTrsvLevels = 0, kTrsvSweep = 1;
 struct TriLayout {
     int bsize = 0;
     bool upper = false;
-    std::vector<int> lev_slice;  // host, nlev + 1: first slice of each level
+    int n = 0;                   // rows
     int nslices = 0;
+    int nsteps = 0;              // non-empty levels, in solve order
+    std::vector<int> step_slice; // host, nsteps + 1: first slice of each step
     long long n_ext = 0, n_int = 0;
-    int2 *soff = nullptr;        // per slice: offset of its external / in-block entries
-    int *next = nullptr;         // per slot (slice*64 + lane): external entries of the row
-    void *mask = nullptr;        // per slot: in-block columns (u32, or u64 when bsize == 64)
+    int4 *sinfo = nullptr;       // per slice: external offset, in-block offset, first block entry, blocks
+    void *rw = nullptr;          // per slot (slice*64 + lane): external count and in-block mask
+    int *next = nullptr;         // bsize 64 only: per slot external count
     int *scol = nullptr;         // n_ext external columns
     bool cols_filled = false;
     double *ve64 = nullptr, *vi64 = nullptr, *d64 = nullptr;  // values; d = inverted diagonal (U)
     float *ve32 = nullptr, *vi32 = nullptr, *d32 = nullptr;
+    unsigned *sync = nullptr;    // sweep time-out flag, spares
+    void *mid64 = nullptr, *mid32 = nullptr;  // lower layout: sweep intermediate vector
+    int mode = kTrsvSweep;
 };
+int tri_layout_check(Ctx *c, TriLayout *L);
 int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rptr, const int *cind,
                      TriLayout *L);
 template <typename T>
 int tri_layout_fill(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, const int *cind, const T *val);
 void tri_layout_free(TriLayout *L);
+// ILU application out = U^-1 L^-1 (in_scale * in) with the two layouts; in may equal out.
 template <typename T>
-int launch_sptrsv_layout(Ctx *c, const Schedule &s, const TriLayout &L, const T *x, T *y, const double *in_scale,
-                         const int *stop);
+int launch_ilu_layout(Ctx *c, const Schedule &ls, const TriLayout &lt, const Schedule &us, const TriLayout &ut,
+                      const T *in, T *out, const double *in_scale, const int *stop);
 
 // ----------------------------------------------------------------- launchers (kernels.hip)
 template <typename T>

@@ -26,7 +26,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_csr(int N, const int *__restric
                                                      const T *__restrict__ x, T *__restrict__ y,
                                                      const int *__restrict__ stop) {
     if (stop && *stop) return;
-    const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const long long gid = (long long)xcd_block() * kBlock + threadIdx.x;
     const int row = (int)(gid / G), q = (int)(gid % G);
     T acc = 0;
     if (row < N) {
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
                                                       T *__restrict__ y, int last_short,
                                                       const int *__restrict__ stop) {
     if (stop && *stop) return;
-    const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const long long gid = (long long)xcd_block() * kBlock + threadIdx.x;
     const int s = (int)(gid / C), k = (int)(gid % C);
     if (s >= nslices) return;
     const int h = (last_short && s == nslices - 1) ? N - s * C : C;

@@ -1,11 +1,12 @@
 // Triangular factors in a solve layout for the colored / level-scheduled SpTRSV.
 //
-// The rows one launch solves (the blocks of one schedule level, in schedule order) are
-// cut into slices of 64 rows, one wave per slice, one lane per row; a slice therefore
-// holds 64/BS whole blocks.  Every row's entries are stored in the order the
-// sequential solve consumes them (senk_sptrsv.hpp: L ascending, U descending after the
-// inverted diagonal), split in two groups:
-//   external entries  column outside the row's block, solved by an earlier launch;
+// The rows of one schedule level (its blocks, in schedule order) are cut into slices
+// of 64 rows, one wave per slice, one lane per row; a slice therefore holds 64/BS whole
+// blocks.  Slices are numbered in solve order (L: levels ascending, U: descending).
+// Every row's entries are stored in the order the sequential solve consumes them
+// (senk_sptrsv.hpp: L ascending, U descending after the inverted diagonal), split in
+// two groups:
+//   external entries  column outside the row's block, solved in an earlier level;
 //                     value + column, k-th entry of every lane stored together;
 //   in-block entries  column inside the block; value only, the column is the bit of a
 //                     per-row mask, entries of step c of every lane stored together.
@@ -13,10 +14,13 @@
 // gaps: a lane's position is the slice offset plus the entries of earlier k, plus the
 // number of lower lanes that also have a k-th entry (wave ballot + mbcnt).  Loads are
 // therefore contiguous runs, the stored volume is exactly nnz, and no padding exists.
-// The arithmetic per row -- the operations and their order -- is that of the CSR
-// kernels, so results are bit-identical to the sequential solve.
+// Per slice one 16-byte descriptor (entry offsets, first block, block count) and per
+// row one word (external count and in-block mask) are all a wave reads before it can
+// address its entries.  The arithmetic per row -- the operations and their order -- is
+// that of the CSR kernels, so results are bit-identical to the sequential solve.
 #include "dev_common.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace senk_amd {
@@ -24,39 +28,64 @@ namespace senk_amd {
 template <int BS>
 using MaskT = typename std::conditional<(BS > 32), unsigned long long, unsigned>::type;
 
+// Per-row word: BS <= 16 external count << 16 | mask; BS == 32 count << 32 | mask;
+// BS == 64 the mask alone (the count in a separate array).
+template <int BS>
+struct RowWord {
+    static constexpr bool kPacked = BS <= 32;
+    using W = typename std::conditional<(BS <= 16), unsigned, unsigned long long>::type;
+    static constexpr int kShift = BS <= 16 ? 16 : 32;
+    static constexpr long long kMaxExt = BS <= 16 ? 0xffff : 0x7fffffff;
+    __device__ __forceinline__ static W pack(int ne, MaskT<BS> mk) {
+        if constexpr (kPacked) return ((W)ne << kShift) | (W)mk;
+        else return (W)mk;
+    }
+    __device__ __forceinline__ static void unpack(W w, const int *next, size_t slot, int &ne, MaskT<BS> &mk) {
+        if constexpr (kPacked) {
+            ne = (int)(w >> kShift);
+            mk = (MaskT<BS>)(w & (((W)1 << kShift) - 1));
+        } else {
+            ne = next[slot];
+            mk = (MaskT<BS>)w;
+        }
+    }
+};
+
 // number of set bits of b in the lanes below this one
 __device__ __forceinline__ int lanes_below(unsigned long long b) {
     return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
 }
 
-// Row of this lane: slice w of a level whose blocks are blk_list[blk0 .. blk0+nblk)
-// (blk_list null: consecutive block numbers from blk0).
+// Row of this lane in a slice whose descriptor is si = {external offset, in-block
+// offset, first block entry, blocks}: blocks blk_list[si.z ..] (blk_list null:
+// consecutive block numbers from si.z).
 template <int BS>
 struct LaneRow {
     int r, base, row;
     bool active;
-    __device__ __forceinline__ LaneRow(int w, int blk0, int nblk, const int *blk_list) {
+    __device__ __forceinline__ LaneRow(int4 si, const int *blk_list) {
         const int lane = threadIdx.x & 63;
-        const int lb = w * (64 / BS) + lane / BS;
+        const int lb = lane / BS;
         r = lane % BS;
-        active = lb < nblk;
-        base = active ? (blk_list ? blk_list[blk0 + lb] : blk0 + lb) * BS : 0;
+        active = lb < si.w;
+        base = active ? (blk_list ? blk_list[si.z + lb] : si.z + lb) * BS : 0;
         row = base + r;
     }
 };
 
-// Per row: number of external entries and in-block mask; per slice: entry counts.
-// err is set when a row breaks the assumptions of the layout (strictly increasing
-// columns; L strictly lower; U starting with its diagonal slot, then columns > row).
+// Per row: the row word; per slice: entry counts.  err = 1 when a row breaks the
+// assumptions of the layout (strictly increasing columns; L strictly lower; U starting
+// with its diagonal slot, then columns > row), 2 when a count does not fit the word.
 template <int BS, bool UPPER>
-__global__ __launch_bounds__(kBlock) void k_tri_count(int slice0, int nsl, int blk0, int nblk,
+__global__ __launch_bounds__(kBlock) void k_tri_count(int nslices, const int4 *__restrict__ sinfo,
                                                       const int *__restrict__ blk_list, const int *__restrict__ rptr,
-                                                      const int *__restrict__ cind, int *__restrict__ next,
-                                                      MaskT<BS> *__restrict__ mask, int *__restrict__ ecnt,
-                                                      int *__restrict__ icnt, int *__restrict__ err) {
+                                                      const int *__restrict__ cind,
+                                                      typename RowWord<BS>::W *__restrict__ rw, int *__restrict__ next,
+                                                      int *__restrict__ ecnt, int *__restrict__ icnt,
+                                                      int *__restrict__ err) {
     const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (w >= nsl) return;
-    const LaneRow<BS> lr(w, blk0, nblk, blk_list);
+    if (w >= nslices) return;
+    const LaneRow<BS> lr(sinfo[w], blk_list);
     int ne = 0;
     MaskT<BS> mk = 0;
     bool bad = false;
@@ -84,38 +113,41 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(int slice0, int nsl, int b
             }
         }
     }
-    const size_t slot = (size_t)(slice0 + w) * 64 + (threadIdx.x & 63);
-    next[slot] = ne;
-    mask[slot] = mk;
+    const bool wide = ne > RowWord<BS>::kMaxExt;
+    const size_t slot = (size_t)w * 64 + (threadIdx.x & 63);
+    rw[slot] = RowWord<BS>::pack(wide ? 0 : ne, mk);
+    if (!RowWord<BS>::kPacked) next[slot] = ne;
     const double se = wave_sum((double)ne), si = wave_sum((double)__popcll((unsigned long long)mk));
     if ((threadIdx.x & 63) == 0) {
-        ecnt[slice0 + w] = (int)se;
-        icnt[slice0 + w] = (int)si;
+        ecnt[w] = (int)se;
+        icnt[w] = (int)si;
     }
     if (bad) *err = 1;
+    else if (wide) *err = 2;
 }
 
 // Scatter one factor's values (and, once, the external columns) into the layout.
 template <typename T, int BS, bool UPPER>
-__global__ __launch_bounds__(kBlock) void k_tri_fill(int slice0, int nsl, int blk0, int nblk,
+__global__ __launch_bounds__(kBlock) void k_tri_fill(int nslices, const int4 *__restrict__ sinfo,
                                                      const int *__restrict__ blk_list, const int *__restrict__ rptr,
                                                      const int *__restrict__ cind, const T *__restrict__ val,
-                                                     const int2 *__restrict__ soff, const int *__restrict__ next,
-                                                     const MaskT<BS> *__restrict__ mask, int *__restrict__ scol,
+                                                     const typename RowWord<BS>::W *__restrict__ rw,
+                                                     const int *__restrict__ next, int *__restrict__ scol,
                                                      T *__restrict__ ve, T *__restrict__ vi, T *__restrict__ dinv) {
     const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (w >= nsl) return;
-    const LaneRow<BS> lr(w, blk0, nblk, blk_list);
-    const size_t slot = (size_t)(slice0 + w) * 64 + (threadIdx.x & 63);
-    const int ne = next[slot];
-    const MaskT<BS> mk = mask[slot];
+    if (w >= nslices) return;
+    const int4 si = sinfo[w];
+    const LaneRow<BS> lr(si, blk_list);
+    const size_t slot = (size_t)w * 64 + (threadIdx.x & 63);
+    int ne;
+    MaskT<BS> mk;
+    RowWord<BS>::unpack(rw[slot], next, slot, ne, mk);
     int s = 0, e = 0;
     if (lr.active) {
         s = rptr[lr.row];
         e = rptr[lr.row + 1];
     }
-    const int2 so = soff[slice0 + w];
-    int pos = so.x;
+    int pos = si.x;
     for (int k = 0;; k++) {
         const unsigned long long b = __ballot(k < ne);
         if (!b) break;
@@ -128,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_fill(int slice0, int nsl, int bl
         pos += __popcll(b);
     }
     int q = UPPER ? e - 1 - ne : s + ne;
-    int ipos = so.y;
+    int ipos = si.y;
     for (int i = 0; i < BS; i++) {
         const int cc = UPPER ? BS - 1 - i : i;
         const bool on = (mk >> cc) & 1;
@@ -142,38 +174,80 @@ __global__ __launch_bounds__(kBlock) void k_tri_fill(int slice0, int nsl, int bl
     if (UPPER) dinv[slot] = lr.active ? val[s] : T(0);
 }
 
-// One launch = one schedule level.  Phase 1 subtracts the external entries, CH gathers
-// in flight per lane; phase 2 runs the BS in-block steps: at step c row c of every
-// block is final and is broadcast inside the wave (U rows first scale by their
-// inverted diagonal), and the rows holding column base+c subtract it.
-template <typename T, int BS, bool UPPER>
-__global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, int blk0, int nblk,
-                                                        const int *__restrict__ blk_list,
-                                                        const int2 *__restrict__ soff, const int *__restrict__ next,
-                                                        const MaskT<BS> *__restrict__ mask,
-                                                        const int *__restrict__ scol, const T *__restrict__ ve,
-                                                        const T *__restrict__ vi, const T *__restrict__ dinv,
-                                                        const T *x, T *y, const double *__restrict__ in_scale,
-                                                        const int *__restrict__ stop_flag) {
-    if (stop_flag && *stop_flag) return;
-    const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (w >= nsl) return;
-    const LaneRow<BS> lr(w, blk0, nblk, blk_list);
-    const size_t slot = (size_t)(slice0 + w) * 64 + (threadIdx.x & 63);
-    const int ne = next[slot];
-    const MaskT<BS> mk = mask[slot];
-    const int2 so = soff[slice0 + w];
+// ------------------------------------------------------------ single-launch sweep
+// In the sweep every row of the output starts as a sentinel: a signalling NaN, which
+// no arithmetic produces (results are canonicalised before they are stored, so an
+// input holding the pattern cannot leak through either).  A gather of an earlier row
+// that still reads the sentinel polls again; a row's value is stored once, so the
+// first non-sentinel value read is the final one.  Stores of the output and the polls
+// go to the device-coherent level (every XCD has its own L2).
+template <typename T>
+struct Bits {
+    using U = unsigned;
+    static constexpr U kSentinel = 0x7fa5a5a5u;
+};
+template <>
+struct Bits<double> {
+    using U = unsigned long long;
+    static constexpr U kSentinel = 0x7ff4a5a5a5a5a5a5ull;
+};
+template <typename T>
+__device__ __forceinline__ typename Bits<T>::U load_agent(const T *p) {
+    using U = typename Bits<T>::U;
+    return __hip_atomic_load((const U *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void store_agent(T *p, T v) {
+    using U = typename Bits<T>::U;
+    __hip_atomic_store((U *)p, __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float canonical(float v) { return __builtin_canonicalizef(v); }
+__device__ __forceinline__ double canonical(double v) { return __builtin_canonicalize(v); }
+
+// Sweep words (unsigned, device): time-out flag, spares.
+constexpr int kSweepErr = 0, kSweepWords = 4;
+// polls of one pending gather before the wave gives up and raises the time-out flag;
+// a poll takes about a microsecond, so this bounds a wait at a few seconds
+constexpr int kSweepSpinLimit = 1 << 22;
+
+// The rows of one slice.  The wave's own data is requested first (descriptor, row
+// words, right-hand side, first chunk of external entries, first in-block values);
+// phase 1 then subtracts the external entries, CH gathers in flight per lane; phase 2
+// runs the BS in-block steps: at step c row c of every block is final and is broadcast
+// inside the wave (U rows first scale by their inverted diagonal), and the rows holding
+// column base+c subtract it.  SWEEP: gathers wait on the sentinel, the row is stored
+// device-coherent, and reset[row] (when given) receives the sentinel once x[row] has
+// been read.
+template <typename T, int BS, bool UPPER, bool SWEEP>
+__device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
+                                            const typename RowWord<BS>::W *__restrict__ rw,
+                                            const int *__restrict__ next, const int *__restrict__ scol,
+                                            const T *__restrict__ ve, const T *__restrict__ vi,
+                                            const T *__restrict__ dinv, const T *x, T *y,
+                                            const double *__restrict__ in_scale, T *reset, unsigned *err) {
+    using U = typename Bits<T>::U;
+    const int4 si = sinfo[slice];
+    const LaneRow<BS> lr(si, blk_list);
+    const size_t slot = (size_t)slice * 64 + (threadIdx.x & 63);
+    int ne;
+    MaskT<BS> mk;
+    RowWord<BS>::unpack(rw[slot], next, slot, ne, mk);
     const T dv = UPPER ? dinv[slot] : T(0);
     T t = 0;
     if (lr.active) {
         t = x[lr.row];
         if (!UPPER && in_scale) t = (T)((double)t * *in_scale);
+        if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
     }
     constexpr int CH = 8;
-    int pos = so.x;
-    for (int k0 = 0; __ballot(k0 < ne); k0 += CH) {
+    constexpr int IC = BS < 8 ? BS : 8;
+    int pos = si.x, k0 = 0;
+    int cc[CH];
+    bool on[CH];
+    T vv[CH];
+    bool more = __ballot(0 < ne) != 0;
+    auto load_chunk = [&]() {
         int idx[CH];
-        bool on[CH];
 #pragma unroll
         for (int q = 0; q < CH; q++) {
             on[q] = k0 + q < ne;
@@ -181,35 +255,80 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, int
             idx[q] = pos + lanes_below(b);
             pos += __popcll(b);
         }
-        int cc[CH];
-        T vv[CH], yv[CH];
 #pragma unroll
         for (int q = 0; q < CH; q++)
             if (on[q]) {
                 cc[q] = scol[idx[q]];
                 vv[q] = ve[idx[q]];
             }
+    };
+    if (more) load_chunk();
+    T iv0[IC];
+    bool on0[IC];
+    int ipos = si.y;
 #pragma unroll
-        for (int q = 0; q < CH; q++)
-            if (on[q]) yv[q] = y[cc[q]];
+    for (int q = 0; q < IC; q++) {
+        const int c = UPPER ? BS - 1 - q : q;
+        on0[q] = (mk >> c) & 1;
+        const unsigned long long b = __ballot(on0[q]);
+        iv0[q] = on0[q] ? vi[ipos + lanes_below(b)] : T(0);
+        ipos += __popcll(b);
+    }
+    while (more) {
+        T yv[CH];
+        if (SWEEP) {
+            U yb[CH];
+            bool pend = false;
+#pragma unroll
+            for (int q = 0; q < CH; q++)
+                if (on[q]) yb[q] = load_agent(y + cc[q]);
+#pragma unroll
+            for (int q = 0; q < CH; q++) pend |= on[q] && yb[q] == Bits<T>::kSentinel;
+            for (int spin = 0; __ballot(pend); spin++) {
+                if (spin >= kSweepSpinLimit) {
+                    if (pend) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                pend = false;
+#pragma unroll
+                for (int q = 0; q < CH; q++)
+                    if (on[q] && yb[q] == Bits<T>::kSentinel) {
+                        yb[q] = load_agent(y + cc[q]);
+                        pend |= yb[q] == Bits<T>::kSentinel;
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < CH; q++) yv[q] = __builtin_bit_cast(T, yb[q]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < CH; q++)
+                if (on[q]) yv[q] = y[cc[q]];
+        }
 #pragma unroll
         for (int q = 0; q < CH; q++)
             if (on[q]) t -= vv[q] * yv[q];
+        k0 += CH;
+        more = __ballot(k0 < ne) != 0;
+        if (more) load_chunk();
     }
     const int lbase = (threadIdx.x & 63) - lr.r;
-    constexpr int IC = BS < 8 ? BS : 8;
-    int ipos = so.y;
 #pragma unroll
     for (int i0 = 0; i0 < BS; i0 += IC) {
         T iv[IC];
-        bool on[IC];
+        bool onq[IC];
 #pragma unroll
         for (int q = 0; q < IC; q++) {
-            const int c = UPPER ? BS - 1 - (i0 + q) : i0 + q;
-            on[q] = (mk >> c) & 1;
-            const unsigned long long b = __ballot(on[q]);
-            iv[q] = on[q] ? vi[ipos + lanes_below(b)] : T(0);
-            ipos += __popcll(b);
+            if (i0 == 0) {
+                onq[q] = on0[q];
+                iv[q] = iv0[q];
+            } else {
+                const int c = UPPER ? BS - 1 - (i0 + q) : i0 + q;
+                onq[q] = (mk >> c) & 1;
+                const unsigned long long b = __ballot(onq[q]);
+                iv[q] = onq[q] ? vi[ipos + lanes_below(b)] : T(0);
+                ipos += __popcll(b);
+            }
         }
 #pragma unroll
         for (int q = 0; q < IC; q++) {
@@ -217,11 +336,58 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, int
             if (UPPER && lr.r == c) t = t * dv;
             if (BS > 1) {
                 const T yc = __shfl(t, lbase + c, 64);
-                if (on[q]) t -= iv[q] * yc;
+                if (onq[q]) t -= iv[q] * yc;
             }
         }
     }
-    if (lr.active) y[lr.row] = t;
+    if (lr.active) {
+        if (SWEEP) store_agent(y + lr.row, canonical(t));
+        else y[lr.row] = t;
+    }
+}
+
+// One launch = one schedule level (slices slice0 .. slice0 + nsl).
+template <typename T, int BS, bool UPPER>
+__global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, const int4 *__restrict__ sinfo,
+                                                        const int *__restrict__ blk_list,
+                                                        const typename RowWord<BS>::W *__restrict__ rw,
+                                                        const int *__restrict__ next, const int *__restrict__ scol,
+                                                        const T *__restrict__ ve, const T *__restrict__ vi,
+                                                        const T *__restrict__ dinv, const T *x, T *y,
+                                                        const double *__restrict__ in_scale,
+                                                        const int *__restrict__ stop_flag) {
+    if (stop_flag && *stop_flag) return;
+    const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (w >= nsl) return;
+    solve_slice<T, BS, UPPER, false>(slice0 + w, sinfo, blk_list, rw, next, scol, ve, vi, dinv, x, y, in_scale,
+                                     nullptr, nullptr);
+}
+
+// The whole triangle in one cooperative launch: every wave of the grid is resident at
+// once, and wave g solves slices g, g + P, g + 2P, ... (P waves in the grid).  A gather
+// only waits on rows of earlier levels, i.e. of smaller slice numbers; the smallest
+// unfinished slice therefore waits on nothing unfinished and its wave, being resident
+// and done with its own smaller slices, works on it: the sweep always drains.  y must
+// hold the sentinel in every row on entry.
+template <typename T, int BS, bool UPPER>
+__global__ __launch_bounds__(kBlock) void k_sptrsv_sweep(int nslices, const int4 *__restrict__ sinfo,
+                                                         const int *__restrict__ blk_list,
+                                                         const typename RowWord<BS>::W *__restrict__ rw,
+                                                         const int *__restrict__ next, const int *__restrict__ scol,
+                                                         const T *__restrict__ ve, const T *__restrict__ vi,
+                                                         const T *__restrict__ dinv, const T *x, T *y,
+                                                         const double *__restrict__ in_scale, T *reset,
+                                                         unsigned *err, const int *__restrict__ stop_flag) {
+    if (stop_flag && *stop_flag) return;
+    const int nw = (int)gridDim.x * kWavesPerBlock;
+    for (int t = (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6); t < nslices; t += nw)
+        solve_slice<T, BS, UPPER, true>(t, sinfo, blk_list, rw, next, scol, ve, vi, dinv, x, y, in_scale, reset, err);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_sentinel(int n, T *p) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = __builtin_bit_cast(T, Bits<T>::kSentinel);
 }
 
 // ----------------------------------------------------------------- host side
@@ -231,61 +397,98 @@ inline int waves_grid(int nsl) { return (nsl + kWavesPerBlock - 1) / kWavesPerBl
 bool layout_bsize_ok(int bs) { return bs >= 1 && bs <= 64 && (64 % bs) == 0; }
 
 template <int BS>
-int count_levels(Ctx *c, const Schedule &s, const TriLayout &L, const int *rptr, const int *cind, int *ecnt,
-                 int *icnt, int *err) {
-    for (int l = 0; l < s.nlev; l++) {
-        const int b0 = s.lev_ptr[l], nb = s.lev_ptr[l + 1] - b0;
-        const int sl0 = L.lev_slice[l], nsl = L.lev_slice[l + 1] - sl0;
-        if (nsl <= 0) continue;
-        if (L.upper)
-            k_tri_count<BS, true><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, b0, nb, s.dblk, rptr, cind, L.next, (MaskT<BS> *)L.mask, ecnt, icnt, err);
-        else
-            k_tri_count<BS, false><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, b0, nb, s.dblk, rptr, cind, L.next, (MaskT<BS> *)L.mask, ecnt, icnt, err);
-        SENK_LAUNCH_CHECK();
-    }
+int count_rows(Ctx *c, const Schedule &s, const TriLayout &L, const int *rptr, const int *cind, int *ecnt, int *icnt,
+               int *err) {
+    using W = typename RowWord<BS>::W;
+    if (L.upper)
+        k_tri_count<BS, true><<<waves_grid(L.nslices), kBlock, 0, c->stream>>>(
+            L.nslices, L.sinfo, s.dblk, rptr, cind, (W *)L.rw, L.next, ecnt, icnt, err);
+    else
+        k_tri_count<BS, false><<<waves_grid(L.nslices), kBlock, 0, c->stream>>>(
+            L.nslices, L.sinfo, s.dblk, rptr, cind, (W *)L.rw, L.next, ecnt, icnt, err);
+    SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
 
 template <typename T, int BS>
-int fill_levels(Ctx *c, const Schedule &s, const TriLayout &L, const int *rptr, const int *cind, const T *val,
-                int *scol, T *ve, T *vi, T *dinv) {
-    for (int l = 0; l < s.nlev; l++) {
-        const int b0 = s.lev_ptr[l], nb = s.lev_ptr[l + 1] - b0;
-        const int sl0 = L.lev_slice[l], nsl = L.lev_slice[l + 1] - sl0;
-        if (nsl <= 0) continue;
-        if (L.upper)
-            k_tri_fill<T, BS, true><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, b0, nb, s.dblk, rptr, cind, val, L.soff, L.next, (const MaskT<BS> *)L.mask, scol, ve, vi,
-                dinv);
-        else
-            k_tri_fill<T, BS, false><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, b0, nb, s.dblk, rptr, cind, val, L.soff, L.next, (const MaskT<BS> *)L.mask, scol, ve, vi,
-                dinv);
-        SENK_LAUNCH_CHECK();
-    }
+int fill_rows(Ctx *c, const Schedule &s, const TriLayout &L, const int *rptr, const int *cind, const T *val, int *scol,
+              T *ve, T *vi, T *dinv) {
+    using W = typename RowWord<BS>::W;
+    if (L.upper)
+        k_tri_fill<T, BS, true><<<waves_grid(L.nslices), kBlock, 0, c->stream>>>(
+            L.nslices, L.sinfo, s.dblk, rptr, cind, val, (const W *)L.rw, L.next, scol, ve, vi, dinv);
+    else
+        k_tri_fill<T, BS, false><<<waves_grid(L.nslices), kBlock, 0, c->stream>>>(
+            L.nslices, L.sinfo, s.dblk, rptr, cind, val, (const W *)L.rw, L.next, scol, ve, vi, dinv);
+    SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
 
 template <typename T, int BS>
 int solve_levels(Ctx *c, const Schedule &s, const TriLayout &L, const T *ve, const T *vi, const T *dinv, const T *x,
                  T *y, const double *in_scale, const int *stop) {
-    for (int step = 0; step < s.nlev; step++) {
-        const int l = L.upper ? s.nlev - 1 - step : step;
-        const int b0 = s.lev_ptr[l], nb = s.lev_ptr[l + 1] - b0;
-        const int sl0 = L.lev_slice[l], nsl = L.lev_slice[l + 1] - sl0;
-        if (nsl <= 0) continue;
+    using W = typename RowWord<BS>::W;
+    for (int q = 0; q < L.nsteps; q++) {
+        const int sl0 = L.step_slice[q], nsl = L.step_slice[q + 1] - sl0;
         if (L.upper)
             k_sptrsv_sell<T, BS, true><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, b0, nb, s.dblk, L.soff, L.next, (const MaskT<BS> *)L.mask, L.scol, ve, vi, dinv, x, y,
-                nullptr, stop);
+                sl0, nsl, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, x, y, nullptr, stop);
         else
             k_sptrsv_sell<T, BS, false><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, b0, nb, s.dblk, L.soff, L.next, (const MaskT<BS> *)L.mask, L.scol, ve, vi, dinv, x, y,
-                in_scale, stop);
+                sl0, nsl, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, x, y, in_scale, stop);
         SENK_LAUNCH_CHECK();
     }
+    return SENK_OK;
+}
+
+// Resident workgroups of a sweep kernel on the current device (the cooperative grid).
+template <typename T, int BS, bool UPPER>
+int sweep_grid() {
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    if (!cached[dev]) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sptrsv_sweep<T, BS, UPPER>, kBlock, 0) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        cached[dev] = per_cu * cus;
+    }
+    return cached[dev];
+}
+
+template <typename T, int BS, bool UPPER>
+int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const T *ve, const T *vi, const T *dinv, const T *x,
+                 T *y, const double *in_scale, T *reset, const int *stop) {
+    using W = typename RowWord<BS>::W;
+    int grid = sweep_grid<T, BS, UPPER>();
+    if (grid <= 0) return SENK_ERR_HIP;
+    grid = std::min(grid, waves_grid(L.nslices));
+    int nslices = L.nslices;
+    const int4 *sinfo = L.sinfo;
+    const int *blk = s.dblk, *next = L.next, *scol = L.scol;
+    const W *rw = (const W *)L.rw;
+    unsigned *err = L.sync + kSweepErr;
+    const double *sc = UPPER ? nullptr : in_scale;
+    void *args[] = {&nslices, &sinfo, &blk, &rw, &next, &scol, &ve, &vi, &dinv, &x, &y, &sc, &reset, &err, &stop};
+    SENK_HIP(hipLaunchCooperativeKernel((const void *)k_sptrsv_sweep<T, BS, UPPER>, dim3(grid), dim3(kBlock), args, 0,
+                                        c->stream));
+    return SENK_OK;
+}
+
+template <typename T, int BS>
+int solve_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const T *ve, const T *vi, const T *dinv, const T *x,
+                T *y, const double *in_scale, T *reset, const int *stop) {
+    if (L.upper) return launch_sweep<T, BS, true>(c, s, L, ve, vi, dinv, x, y, in_scale, reset, stop);
+    return launch_sweep<T, BS, false>(c, s, L, ve, vi, dinv, x, y, in_scale, reset, stop);
+}
+
+template <typename T>
+int fill_sentinel(Ctx *c, int n, T *p) {
+    if (n <= 0) return SENK_OK;
+    k_fill_sentinel<T><<<(n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(n, p);
+    SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
 
@@ -300,11 +503,13 @@ int solve_levels(Ctx *c, const Schedule &s, const TriLayout &L, const T *ve, con
         case 64: { constexpr int B = 64; return CALL; } \
         default: return SENK_ERR_UNSUPPORTED;           \
     }
+size_t row_word_bytes(int bs) { return bs <= 16 ? sizeof(unsigned) : sizeof(unsigned long long); }
 }  // namespace
 
 void tri_layout_free(TriLayout *L) {
-    for (void *p : {(void *)L->soff, (void *)L->next, L->mask, (void *)L->scol, (void *)L->ve64, (void *)L->vi64,
-                    (void *)L->d64, (void *)L->ve32, (void *)L->vi32, (void *)L->d32})
+    for (void *p : {(void *)L->sinfo, (void *)L->next, L->rw, (void *)L->scol, (void *)L->ve64, (void *)L->vi64,
+                    (void *)L->d64, (void *)L->ve32, (void *)L->vi32, (void *)L->d32, (void *)L->sync, L->mid64,
+                    L->mid32})
         if (p) (void)hipFree(p);
     *L = TriLayout{};
 }
@@ -313,16 +518,27 @@ int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rp
                      TriLayout *L) {
     const int BS = s.bsize;
     if (!layout_bsize_ok(BS) || (long long)s.lev_ptr.size() < (long long)s.nlev + 1) return SENK_ERR_UNSUPPORTED;
+    if ((long long)s.lev_ptr[s.nlev] * BS != N) return SENK_ERR_SCHEDULE;
     tri_layout_free(L);
     L->bsize = BS;
     L->upper = upper;
-    L->lev_slice.assign((size_t)s.nlev + 1, 0);
-    for (int l = 0; l < s.nlev; l++)
-        L->lev_slice[l + 1] = L->lev_slice[l] + slices_of(s.lev_ptr[l + 1] - s.lev_ptr[l], BS);
-    L->nslices = L->lev_slice[s.nlev];
-    if ((long long)s.lev_ptr[s.nlev] * BS != N) return SENK_ERR_SCHEDULE;
+    L->n = N;
+    // steps = non-empty levels in solve order; slices numbered in that order
+    std::vector<int4> si;
+    L->step_slice.assign(1, 0);
+    const int per = 64 / BS;
+    for (int q = 0; q < s.nlev; q++) {
+        const int l = upper ? s.nlev - 1 - q : q;
+        const int b0 = s.lev_ptr[l], nb = s.lev_ptr[l + 1] - b0;
+        const int nsl = slices_of(nb, BS);
+        if (nsl <= 0) continue;
+        for (int w = 0; w < nsl; w++) si.push_back(make_int4(0, 0, b0 + w * per, std::min(per, nb - w * per)));
+        L->step_slice.push_back((int)si.size());
+    }
+    L->nsteps = (int)L->step_slice.size() - 1;
+    L->nslices = (int)si.size();
+    if (L->nslices == 0) return SENK_OK;
     const size_t nslots = (size_t)L->nslices * 64;
-    const size_t mbytes = BS > 32 ? sizeof(unsigned long long) : sizeof(unsigned);
     int *ecnt = nullptr, *icnt = nullptr, *err = nullptr;
     int st = SENK_OK;
     std::vector<int> he(L->nslices), hi(L->nslices);
@@ -334,13 +550,18 @@ int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rp
         if (code) tri_layout_free(L);
         return code;
     };
-    if (L->nslices == 0) return SENK_OK;
-    if (hipMalloc(&L->next, nslots * sizeof(int)) != hipSuccess || hipMalloc(&L->mask, nslots * mbytes) != hipSuccess ||
+    if (hipMalloc(&L->sinfo, si.size() * sizeof(int4)) != hipSuccess ||
+        hipMalloc(&L->rw, nslots * row_word_bytes(BS)) != hipSuccess ||
+        (BS > 32 && hipMalloc(&L->next, nslots * sizeof(int)) != hipSuccess) ||
         hipMalloc(&ecnt, L->nslices * sizeof(int)) != hipSuccess ||
-        hipMalloc(&icnt, L->nslices * sizeof(int)) != hipSuccess || hipMalloc(&err, sizeof(int)) != hipSuccess)
+        hipMalloc(&icnt, L->nslices * sizeof(int)) != hipSuccess || hipMalloc(&err, sizeof(int)) != hipSuccess ||
+        hipMalloc(&L->sync, kSweepWords * sizeof(unsigned)) != hipSuccess)
         return fail(SENK_ERR_ALLOC);
-    if (hipMemsetAsync(err, 0, sizeof(int), c->stream) != hipSuccess) return fail(SENK_ERR_HIP);
-    auto count = [&]() -> int { SENK_BS_DISPATCH(BS, count_levels<B>(c, s, *L, rptr, cind, ecnt, icnt, err)) };
+    if (hipMemcpyAsync(L->sinfo, si.data(), si.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemsetAsync(err, 0, sizeof(int), c->stream) != hipSuccess ||
+        hipMemsetAsync(L->sync, 0, kSweepWords * sizeof(unsigned), c->stream) != hipSuccess)
+        return fail(SENK_ERR_HIP);
+    auto count = [&]() -> int { SENK_BS_DISPATCH(BS, count_rows<B>(c, s, *L, rptr, cind, ecnt, icnt, err)) };
     if ((st = count())) return fail(st);
     if (hipMemcpyAsync(he.data(), ecnt, he.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipMemcpyAsync(hi.data(), icnt, hi.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
@@ -348,20 +569,18 @@ int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rp
         hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(SENK_ERR_HIP);
     if (herr) return fail(SENK_ERR_UNSUPPORTED);
-    std::vector<int2> so(L->nslices);
     long long ne = 0, ni = 0;
     for (int i = 0; i < L->nslices; i++) {
-        so[i] = make_int2((int)ne, (int)ni);
+        si[i].x = (int)ne;
+        si[i].y = (int)ni;
         ne += he[i];
         ni += hi[i];
+        if (ne > INT32_MAX || ni > INT32_MAX) return fail(SENK_ERR_UNSUPPORTED);
     }
-    if (ne > INT32_MAX || ni > INT32_MAX) return fail(SENK_ERR_UNSUPPORTED);
     L->n_ext = ne;
     L->n_int = ni;
-    if (hipMalloc(&L->soff, so.size() * sizeof(int2)) != hipSuccess ||
-        hipMalloc(&L->scol, (size_t)(ne ? ne : 1) * sizeof(int)) != hipSuccess)
-        return fail(SENK_ERR_ALLOC);
-    if (hipMemcpyAsync(L->soff, so.data(), so.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+    if (hipMalloc(&L->scol, (size_t)(ne ? ne : 1) * sizeof(int)) != hipSuccess) return fail(SENK_ERR_ALLOC);
+    if (hipMemcpyAsync(L->sinfo, si.data(), si.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(SENK_ERR_HIP);
     L->cols_filled = false;
@@ -377,8 +596,15 @@ int tri_layout_fill(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, co
     if (!*ve) SENK_HIP(hipMalloc(ve, (size_t)(L->n_ext ? L->n_ext : 1) * sizeof(T)));
     if (!*vi) SENK_HIP(hipMalloc(vi, (size_t)(L->n_int ? L->n_int : 1) * sizeof(T)));
     if (L->upper && !*dv) SENK_HIP(hipMalloc(dv, (size_t)L->nslices * 64 * sizeof(T)));
+    // the lower layout owns the intermediate vector of the sweep (L output, U input),
+    // kept all-sentinel between applications
+    void **mid = std::is_same<T, double>::value ? &L->mid64 : &L->mid32;
+    if (!L->upper && !*mid) {
+        SENK_HIP(hipMalloc(mid, (size_t)L->n * sizeof(T)));
+        SENK_TRY(fill_sentinel<T>(c, L->n, (T *)*mid));
+    }
     int *scol = L->cols_filled ? nullptr : L->scol;
-    auto run = [&]() -> int { SENK_BS_DISPATCH(L->bsize, (fill_levels<T, B>(c, s, *L, rptr, cind, val, scol, *ve, *vi, *dv))) };
+    auto run = [&]() -> int { SENK_BS_DISPATCH(L->bsize, (fill_rows<T, B>(c, s, *L, rptr, cind, val, scol, *ve, *vi, *dv))) };
     SENK_TRY(run());
     L->cols_filled = true;
     return SENK_OK;
@@ -387,18 +613,58 @@ template int tri_layout_fill<double>(Ctx *, const Schedule &, TriLayout *, const
 template int tri_layout_fill<float>(Ctx *, const Schedule &, TriLayout *, const int *, const int *, const float *);
 
 template <typename T>
-int launch_sptrsv_layout(Ctx *c, const Schedule &s, const TriLayout &L, const T *x, T *y, const double *in_scale,
-                         const int *stop) {
-    const T *ve = std::is_same<T, double>::value ? (const T *)L.ve64 : (const T *)L.ve32;
-    const T *vi = std::is_same<T, double>::value ? (const T *)L.vi64 : (const T *)L.vi32;
-    const T *dv = std::is_same<T, double>::value ? (const T *)L.d64 : (const T *)L.d32;
-    if (L.nslices == 0) return SENK_OK;
-    if (!ve || !vi || (L.upper && !dv) || L.bsize != s.bsize) return SENK_ERR_ARG;
-    SENK_BS_DISPATCH(L.bsize, (solve_levels<T, B>(c, s, L, ve, vi, dv, x, y, in_scale, stop)))
+int launch_ilu_layout(Ctx *c, const Schedule &ls, const TriLayout &lt, const Schedule &us, const TriLayout &ut,
+                      const T *in, T *out, const double *in_scale, const int *stop) {
+    const bool f64 = std::is_same<T, double>::value;
+    const T *lve = (const T *)(f64 ? (void *)lt.ve64 : (void *)lt.ve32);
+    const T *lvi = (const T *)(f64 ? (void *)lt.vi64 : (void *)lt.vi32);
+    const T *uve = (const T *)(f64 ? (void *)ut.ve64 : (void *)ut.ve32);
+    const T *uvi = (const T *)(f64 ? (void *)ut.vi64 : (void *)ut.vi32);
+    const T *udv = (const T *)(f64 ? (void *)ut.d64 : (void *)ut.d32);
+    T *mid = (T *)(f64 ? lt.mid64 : lt.mid32);
+    if (lt.nslices == 0 || ut.nslices == 0) return lt.nslices == ut.nslices ? SENK_OK : SENK_ERR_ARG;
+    if (!lve || !lvi || !uve || !uvi || !udv || lt.bsize != ls.bsize || ut.bsize != us.bsize || lt.upper ||
+        !ut.upper || lt.n != ut.n)
+        return SENK_ERR_ARG;
+    if (lt.mode == kTrsvSweep && mid && lt.sync && ut.sync) {
+        // L: in -> mid (every row of out becomes the sentinel for the U sweep);
+        // U: mid -> out (mid returns to the sentinel as its rows are read)
+        auto l = [&]() -> int {
+            SENK_BS_DISPATCH(lt.bsize, (solve_sweep<T, B>(c, ls, lt, lve, lvi, nullptr, in, mid, in_scale, out, stop)))
+        };
+        auto u = [&]() -> int {
+            SENK_BS_DISPATCH(ut.bsize, (solve_sweep<T, B>(c, us, ut, uve, uvi, udv, mid, out, nullptr, mid, stop)))
+        };
+        SENK_TRY(l());
+        return u();
+    }
+    auto l = [&]() -> int {
+        SENK_BS_DISPATCH(lt.bsize, (solve_levels<T, B>(c, ls, lt, lve, lvi, nullptr, in, out, in_scale, stop)))
+    };
+    auto u = [&]() -> int {
+        SENK_BS_DISPATCH(ut.bsize, (solve_levels<T, B>(c, us, ut, uve, uvi, udv, out, out, nullptr, stop)))
+    };
+    SENK_TRY(l());
+    return u();
 }
-template int launch_sptrsv_layout<double>(Ctx *, const Schedule &, const TriLayout &, const double *, double *,
-                                          const double *, const int *);
-template int launch_sptrsv_layout<float>(Ctx *, const Schedule &, const TriLayout &, const float *, float *,
-                                         const double *, const int *);
+template int launch_ilu_layout<double>(Ctx *, const Schedule &, const TriLayout &, const Schedule &,
+                                       const TriLayout &, const double *, double *, const double *, const int *);
+template int launch_ilu_layout<float>(Ctx *, const Schedule &, const TriLayout &, const Schedule &, const TriLayout &,
+                                      const float *, float *, const double *, const int *);
+
+int tri_layout_check(Ctx *c, TriLayout *L) {
+    if (!L->sync) return SENK_OK;
+    unsigned err = 0;
+    SENK_HIP(hipMemcpyAsync(&err, L->sync + kSweepErr, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    if (!err) return SENK_OK;
+    // a wait that gave up leaves sentinels or stale rows in the intermediate vector:
+    // start the next application from a clean state
+    SENK_HIP(hipMemsetAsync(L->sync, 0, kSweepWords * sizeof(unsigned), c->stream));
+    if (L->mid64) SENK_TRY(fill_sentinel<double>(c, L->n, (double *)L->mid64));
+    if (L->mid32) SENK_TRY(fill_sentinel<float>(c, L->n, (float *)L->mid32));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    return SENK_ERR_TIMEOUT;
+}
 
 }  // namespace senk_amd

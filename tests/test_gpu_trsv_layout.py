@@ -5,6 +5,10 @@ only when asked (csr_only = 1).  Both paths must give the oracle's sequential
 L-then-U solve bit for bit in FP64 (same operations in the same order per row), and
 the same bits as each other in FP32, over every block size the layout takes, for
 color schedules and for dependency-level schedules (ILU(p) fill, natural ordering).
+The layout solve runs in each launch mode (one launch per level; one launch per
+triangle whose gathers wait for the rows they read): both give the same bits, also
+when applied many times in a row (the sweep's intermediate vector and ticket counters
+must return to their initial state after every application) and in place.
 """
 import ctypes as C
 
@@ -16,6 +20,7 @@ from oracle import oracle as O
 from tests.helpers import abmc_system, oracle_ilu0
 
 pytestmark = pytest.mark.gpu
+MODES = (0, 1)  # per-level launches, single-launch sweep
 
 
 def _expected(sy, kind, p):
@@ -59,26 +64,26 @@ def _check(ctx, sy, kind, p, bsize, colors, seed=3):
         x = np.random.default_rng(seed).uniform(-1, 1, N)
         want = _oracle_apply(sp, x)
         dx = ctx.array(x)
-        got = []
-        for M in (Ml, Mc):
-            dy = ctx.zeros(N)
-            S.call("senk_prec_apply_f64", M, dx.p, dy.p)
-            got.append(dy.download())
-        assert np.array_equal(got[0], want), np.max(np.abs(got[0] - want))
-        assert np.array_equal(got[1], want)
+        dy = ctx.zeros(N)
+        S.call("senk_prec_apply_f64", Mc, dx.p, dy.p)
+        assert np.array_equal(dy.download(), want)
         x32 = x.astype(np.float32)
         dx32 = ctx.array(x32)
-        got32 = []
-        for M in (Ml, Mc):
-            dy = ctx.array(np.zeros(N, np.float32))
-            S.call("senk_prec_apply_f32", M, dx32.p, dy.p)
-            got32.append(dy.download())
-        assert np.array_equal(got32[0], got32[1])
-        assert np.max(np.abs(got32[0] - want)) <= 1e-4 * max(1.0, np.max(np.abs(want)))
-        # applying twice gives the same bits (no state carried between applications)
-        dy = ctx.zeros(N)
-        S.call("senk_prec_apply_f64", Ml, dx.p, dy.p)
-        assert np.array_equal(dy.download(), want)
+        dy32 = ctx.array(np.zeros(N, np.float32))
+        S.call("senk_prec_apply_f32", Mc, dx32.p, dy32.p)
+        want32 = dy32.download()
+        assert np.max(np.abs(want32 - want)) <= 1e-4 * max(1.0, np.max(np.abs(want)))
+        for mode in MODES:
+            S.call("senk_prec_set_trsv_mode", Ml, mode)
+            for rep in range(2):  # a second application starts from the counters the first left
+                dy = ctx.zeros(N)
+                S.call("senk_prec_apply_f64", Ml, dx.p, dy.p)
+                got = dy.download()
+                assert np.array_equal(got, want), (mode, rep, np.max(np.abs(got - want)))
+                dy32 = ctx.array(np.zeros(N, np.float32))
+                S.call("senk_prec_apply_f32", Ml, dx32.p, dy32.p)
+                assert np.array_equal(dy32.download(), want32), (mode, rep)
+            S.call("senk_prec_check", Ml)
     finally:
         for M in (Ml, Mc):
             S.lib().senk_prec_destroy(M)
@@ -122,3 +127,63 @@ def test_layout_partial_last_slice(ctx):
     sy = abmc_system(rptr, cind, val, 2)
     assert any((np.diff(sy["cptr"]) * 2) % 64)
     _check(ctx, sy, S.SENK_ILU0, 0, 2, colors=True)
+
+
+@pytest.mark.parametrize("bsize", [4, 8])
+def test_sweep_modes_vardiff64(ctx, bsize):
+    # 262,144 rows, thousands of slices per color: many waves wait at once.  Every mode
+    # gives the per-level result's bits over 12 applications in a row, and the
+    # preconditioned solve takes the same iterations in every mode.
+    rptr, cind, val = S.gen_vardiff7(64, 7, 8.0, 1.5)
+    sy = abmc_system(rptr, cind, val, bsize)
+    N = sy["N"]
+    A, (Ml, Mc) = _handles(ctx, sy, S.SENK_ILU0, 0, bsize, True)
+    try:
+        rng = np.random.default_rng(11)
+        xs = [rng.uniform(-1, 1, N) for _ in range(3)]
+        ref = {}
+        for mode in MODES:
+            S.call("senk_prec_set_trsv_mode", Ml, mode)
+            for rep in range(4):
+                for i, x in enumerate(xs):
+                    dx, dy = ctx.array(x), ctx.zeros(N)
+                    S.call("senk_prec_apply_f64", Ml, dx.p, dy.p)
+                    dx32, dy32 = ctx.array(x.astype(np.float32)), ctx.array(np.zeros(N, np.float32))
+                    S.call("senk_prec_apply_f32", Ml, dx32.p, dy32.p)
+                    got = (dy.download(), dy32.download())
+                    if (i, 0) not in ref:
+                        ref[(i, 0)], ref[(i, 1)] = got
+                    assert np.array_equal(got[0], ref[(i, 0)]), (mode, rep, i)
+                    assert np.array_equal(got[1], ref[(i, 1)]), (mode, rep, i)
+            S.call("senk_prec_check", Ml)
+        # the CSR kernels agree with the first mode's bits
+        dx, dy = ctx.array(xs[0]), ctx.zeros(N)
+        S.call("senk_prec_apply_f64", Mc, dx.p, dy.p)
+        assert np.array_equal(dy.download(), ref[(0, 0)])
+        # in place (input and output one vector), in every mode
+        for mode in MODES:
+            S.call("senk_prec_set_trsv_mode", Ml, mode)
+            dxy = ctx.array(xs[1])
+            S.call("senk_prec_apply_f64", Ml, dxy.p, dxy.p)
+            assert np.array_equal(dxy.download(), ref[(1, 0)]), mode
+            dxy32 = ctx.array(xs[1].astype(np.float32))
+            S.call("senk_prec_apply_f32", Ml, dxy32.p, dxy32.p)
+            assert np.array_equal(dxy32.download(), ref[(1, 1)]), mode
+        S.call("senk_prec_check", Ml)
+        b = np.random.default_rng(5).uniform(-1, 1, N)
+        iters = []
+        for mode in MODES:
+            S.call("senk_prec_set_trsv_mode", Ml, mode)
+            db, dxx = ctx.array(b), ctx.zeros(N)
+            o = S.solve_opts(solver=S.SENK_BICGSTAB, eps=1e-10, max_iter=2000)
+            res = S.senk_result()
+            S.call("senk_solve", ctx.h, A, Ml, db.p, dxx.p, C.byref(o), C.byref(res), None, 0)
+            assert res.converged
+            iters.append((res.iters, dxx.download()))
+        for it, x in iters[1:]:
+            assert it == iters[0][0]
+            assert np.array_equal(x, iters[0][1])
+    finally:
+        for M in (Ml, Mc):
+            S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
