@@ -133,7 +133,7 @@ def time_kernel(ctx, fn, reps):
     return ms.value / reps
 
 
-def kernel_rates(ctx, dev, reps=50):
+def kernel_rates(ctx, dev, reps=50, sweep_modes=False):
     N, nnz = dev.N, dev.sy["nnz"]
     dx = ctx.array(np.random.default_rng(0).uniform(-1, 1, N))
     dy = ctx.zeros(N)
@@ -148,8 +148,11 @@ def kernel_rates(ctx, dev, reps=50):
            "ilu_apply_f64_us": p64 * 1e3, "ilu_apply_f64_GBps": ilu_bytes(N, dev.lnnz, dev.unnz, 8) / p64 / 1e6,
            "ilu_apply_f32_us": p32 * 1e3, "ilu_apply_f32_GBps": ilu_bytes(N, dev.lnnz, dev.unnz, 4) / p32 / 1e6,
            "colors": dev.sy["nc"], "bsize": dev.sy["bsize"]}
-    # triangular-solve launch modes: one launch per level, then the default single sweep
-    for mode in (0, 1):
+    if not sweep_modes:
+        return out
+    # triangular-solve launch modes: one cooperative sweep per triangle, then the default
+    # one launch per level
+    for mode in (1, 0):
         S.call("senk_prec_set_trsv_mode", dev.M, mode)
         out[f"ilu_apply_f64_us_mode{mode}"] = time_kernel(
             ctx, lambda: S.call("senk_prec_apply_f64", dev.M, dx.p, dy.p), 20) * 1e3
@@ -282,6 +285,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--kernels-only", action="store_true", help="kernel rates and the copy calibration only")
     ap.add_argument("--run", default=None, help="time only the solve configuration of this name (profiling)")
+    ap.add_argument("--sweep-modes", action="store_true",
+                    help="also time the ILU application with one cooperative launch per triangle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not supported (use --gpus 1)"}))
@@ -291,7 +296,7 @@ def main():
     log(f"[bench] building the {n}^3 variable-coefficient diffusion system (ABMC blocks of 8 and 4)")
     sy8 = build_system_kind("vardiff", n, 8)
     d8 = Device(ctx, sy8)
-    kern = kernel_rates(ctx, d8)
+    kern = kernel_rates(ctx, d8, sweep_modes=args.sweep_modes)
     cal = copy_rate(ctx)
     log(f"[bench] kernels: {json.dumps(kern)} copy: {json.dumps(cal)}")
     if args.kernels_only:

@@ -389,11 +389,16 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
 int senk_prec_destroy(senk_prec M);
 int senk_prec_apply_f64(senk_prec M, const double *dx, double *dy);
 int senk_prec_apply_f32(senk_prec M, const float *dx, float *dy);
+/* FP32 vectors, FP16 factor values (precision bit SENK_FACTOR_F16; no reference
+ * counterpart, the reference keeps its factors in the solver's type T): each row
+ * stores value * 2^-e in FP16 with one exponent e per row (the binary exponent of the
+ * row's largest magnitude), the U inverted diagonal stays FP32, arithmetic is FP32. */
+int senk_prec_apply_f16(senk_prec M, const float *dx, float *dy);
 /* Triangular-solve launch mode of the sliced layout (no reference counterpart; the
  * reference runs one OpenMP parallel loop per color, senk_sparse.hpp:207-246):
- * 0 one launch per level; 1 (default) one launch per triangle, each gather waiting
- * until the row it reads is final.  Results are bit-identical in both modes.
- * Applications of one preconditioner must not overlap (one stream). */
+ * 0 (default) one launch per level; 1 one cooperative launch per triangle, each
+ * gather waiting until the row it reads is final.  Results are bit-identical in both
+ * modes.  Applications of one preconditioner must not overlap (one stream). */
 int senk_prec_set_trsv_mode(senk_prec M, int mode);
 /* Synchronise and report SENK_ERR_TIMEOUT if a single-launch solve since the last check
  * gave up a wait (never expected; senk_solve checks by itself). */
@@ -413,7 +418,8 @@ typedef struct {
     int mixed;           /* 1: FP32 matrix/factors/inner Krylov + FP64 iterative refinement;
                             2: FP32 inner Krylov as the preconditioner of an FP64 flexible
                                GCR(outer_m) outer iteration (nested)                      */
-    int factor_prec;     /* senk_factor_precision used by the inner solve                */
+    int factor_prec;     /* senk_factor_precision of the FP32 applications of the mixed
+                            forms: F32 (default when the FP32 copy exists) or F16        */
     double inner_eps;    /* mixed: inner relative tolerance per refinement step          */
     int inner_max;       /* mixed: inner iteration budget per refinement step            */
     int check_every;     /* GCR: 1 = test ||r|| every step (reference: restarts only)    */

@@ -1192,11 +1192,20 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
                 if ((st = tri_layout_fill<float>(c, v.ls, &M->lt, lr, lc, v.l32))) break;
                 if ((st = tri_layout_fill<float>(c, v.us, &M->ut, ur, uc, v.u32))) break;
             }
+            if (opts->precision & (1 << SENK_FACTOR_F16)) {
+                if ((st = tri_layout_fill_f16(c, v.ls, &M->lt, lr, lc, l64))) break;
+                if ((st = tri_layout_fill_f16(c, v.us, &M->ut, ur, uc, u64))) break;
+            }
             v.lt = &M->lt;
             v.ut = &M->ut;
         } else {
             tri_layout_free(&M->lt);
             tri_layout_free(&M->ut);
+            // FP16 factors live in the solve layout only
+            if (opts->precision & (1 << SENK_FACTOR_F16)) {
+                st = SENK_ERR_UNSUPPORTED;
+                break;
+            }
         }
         if (hipStreamSynchronize(c->stream) != hipSuccess) st = SENK_ERR_HIP;
     } while (0);
@@ -1229,6 +1238,12 @@ int senk_prec_apply_f64(senk_prec M, const double *dx, double *dy) {
 int senk_prec_apply_f32(senk_prec M, const float *dx, float *dy) {
     if (!M || !M->view.l32) return SENK_ERR_ARG;
     return prec_apply<float>(&M->ctx->c, M->view, dx, nullptr, dy, nullptr, nullptr);
+}
+int senk_prec_apply_f16(senk_prec M, const float *dx, float *dy) {
+    if (!M || !M->lt.ve16 || !M->ut.ve16) return SENK_ERR_ARG;
+    PrecView v = M->view;
+    v.f16 = true;
+    return prec_apply<float>(&M->ctx->c, v, dx, nullptr, dy, nullptr, nullptr);
 }
 int senk_prec_set_trsv_mode(senk_prec M, int mode) {
     if (!M || mode < senk_amd::kTrsvLevels || mode > senk_amd::kTrsvSweep) return SENK_ERR_ARG;
@@ -1296,8 +1311,13 @@ int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *
         lp->off = false;
         pv.learned = lp;
     }
+    if (o->factor_prec == SENK_FACTOR_F16) {
+        // FP16 factors serve the FP32 applications of the mixed forms
+        if (!o->mixed || !M || !M->lt.ve16 || !M->ut.ve16) return SENK_ERR_ARG;
+        pv.f16 = true;
+    }
     if (o->mixed) {
-        if (!A->v32 || (M && !M->view.l32)) return SENK_ERR_ARG;
+        if (!A->v32 || (M && !M->view.l32 && !pv.f16)) return SENK_ERR_ARG;
         if (o->mixed == 2)
             SENK_TRY(nested_solve(&ctx->c, A->view, pv, db, dx, *o, nrm_b, res, hist, hist_cap));
         else

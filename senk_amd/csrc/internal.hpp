@@ -102,7 +102,11 @@ struct TriLayout {
     float *ve32 = nullptr, *vi32 = nullptr, *d32 = nullptr;
     unsigned *sync = nullptr;    // sweep time-out flag, spares
     void *mid64 = nullptr, *mid32 = nullptr;  // lower layout: sweep intermediate vector
-    int mode = kTrsvSweep;
+    // FP16 copy: values scaled by 2^-e per row (e in sx16), FP32 inverted diagonal (U)
+    void *ve16 = nullptr, *vi16 = nullptr;
+    float *d16 = nullptr;
+    signed char *sx16 = nullptr;
+    int mode = kTrsvLevels;
 };
 int tri_layout_check(Ctx *c, TriLayout *L);
 int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rptr, const int *cind,
@@ -111,9 +115,11 @@ template <typename T>
 int tri_layout_fill(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, const int *cind, const T *val);
 void tri_layout_free(TriLayout *L);
 // ILU application out = U^-1 L^-1 (in_scale * in) with the two layouts; in may equal out.
+// f16 (T = float only): the FP16 copies of the factor values.
 template <typename T>
 int launch_ilu_layout(Ctx *c, const Schedule &ls, const TriLayout &lt, const Schedule &us, const TriLayout &ut,
-                      const T *in, T *out, const double *in_scale, const int *stop);
+                      const T *in, T *out, const double *in_scale, const int *stop, bool f16 = false);
+int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, const int *cind, const double *val);
 
 // ----------------------------------------------------------------- launchers (kernels.hip)
 template <typename T>

@@ -497,7 +497,10 @@ int prec_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T
         const T *lv = std::is_same<T, double>::value ? (const T *)M.l64 : (const T *)M.l32;
         const T *uv = std::is_same<T, double>::value ? (const T *)M.u64 : (const T *)M.u32;
         if (!lv || !uv) return SENK_ERR_ARG;
-        if (layout_has<T>(M.lt) && layout_has<T>(M.ut)) {
+        if (std::is_same<T, float>::value && M.f16) {
+            if (!M.lt || !M.ut) return SENK_ERR_ARG;
+            SENK_TRY(launch_ilu_layout<T>(c, M.ls, *M.lt, M.us, *M.ut, in, out, in_scale, stop, true));
+        } else if (layout_has<T>(M.lt) && layout_has<T>(M.ut)) {
             SENK_TRY(launch_ilu_layout<T>(c, M.ls, *M.lt, M.us, *M.ut, in, out, in_scale, stop));
         } else {
             SENK_TRY(launch_sptrsv<T>(c, M.ls, false, M.lrptr, M.lcind, lv, in, out, in_scale, stop));

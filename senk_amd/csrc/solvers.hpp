@@ -39,6 +39,8 @@ struct PrecView {
     const float *l32 = nullptr, *u32 = nullptr;
     // optional solve layouts of the CSR factors (used when they hold the precision)
     const TriLayout *lt = nullptr, *ut = nullptr;
+    // FP32 applications read the layouts' FP16 factor copies (per-row scaled)
+    bool f16 = false;
     // BCSR factors
     int bnl = 0, bnw = 0;
     const int *blrptr = nullptr, *blcind = nullptr, *burptr = nullptr, *bucind = nullptr;

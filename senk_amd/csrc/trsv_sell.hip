@@ -20,6 +20,8 @@
 // that of the CSR kernels, so results are bit-identical to the sequential solve.
 #include "dev_common.hpp"
 
+#include <hip/hip_fp16.h>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -50,6 +52,19 @@ struct RowWord {
         }
     }
 };
+
+// Stored factor values: FP64 / FP32 as computed with, or FP16 holding value * 2^-e with
+// one exponent e per row (|stored| < 1; FP32 arithmetic on 2^e * stored, which is exact).
+template <typename V>
+constexpr bool kHalfV = std::is_same<V, __half>::value;
+template <typename T, typename V>
+__device__ __forceinline__ T decode(V v, int e) {
+    if constexpr (kHalfV<V>) return ldexpf(__half2float(v), e);
+    else return (T)v;
+}
+// inverted diagonal stored with FP16 factors: FP32
+template <typename V>
+using DiagT = typename std::conditional<kHalfV<V>, float, V>::type;
 
 // number of set bits of b in the lanes below this one
 __device__ __forceinline__ int lanes_below(unsigned long long b) {
@@ -126,14 +141,17 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(int nslices, const int4 *_
     else if (wide) *err = 2;
 }
 
-// Scatter one factor's values (and, once, the external columns) into the layout.
-template <typename T, int BS, bool UPPER>
+// Scatter one factor's values (and, once, the external columns) into the layout.  FP16
+// storage: the row exponent e is the binary exponent of the row's largest magnitude
+// (the U diagonal slot excluded), so every stored value lies in (-1, 1).
+template <typename S, typename V, int BS, bool UPPER>
 __global__ __launch_bounds__(kBlock) void k_tri_fill(int nslices, const int4 *__restrict__ sinfo,
                                                      const int *__restrict__ blk_list, const int *__restrict__ rptr,
-                                                     const int *__restrict__ cind, const T *__restrict__ val,
+                                                     const int *__restrict__ cind, const S *__restrict__ val,
                                                      const typename RowWord<BS>::W *__restrict__ rw,
                                                      const int *__restrict__ next, int *__restrict__ scol,
-                                                     T *__restrict__ ve, T *__restrict__ vi, T *__restrict__ dinv) {
+                                                     V *__restrict__ ve, V *__restrict__ vi,
+                                                     DiagT<V> *__restrict__ dinv, signed char *__restrict__ sx) {
     const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (w >= nslices) return;
     const int4 si = sinfo[w];
@@ -147,6 +165,17 @@ __global__ __launch_bounds__(kBlock) void k_tri_fill(int nslices, const int4 *__
         s = rptr[lr.row];
         e = rptr[lr.row + 1];
     }
+    int ex = 0;
+    if constexpr (kHalfV<V>) {
+        double m = 0;
+        for (int j = UPPER ? s + 1 : s; j < e; j++) m = fmax(m, fabs((double)val[j]));
+        if (m > 0) (void)frexp(m, &ex);
+        sx[slot] = (signed char)ex;
+    }
+    auto enc = [&](S v) -> V {
+        if constexpr (kHalfV<V>) return __float2half((float)ldexp((double)v, -ex));
+        else return (V)v;
+    };
     int pos = si.x;
     for (int k = 0;; k++) {
         const unsigned long long b = __ballot(k < ne);
@@ -155,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_fill(int nslices, const int4 *__
             const int j = UPPER ? e - 1 - k : s + k;
             const int p = pos + lanes_below(b);
             if (scol) scol[p] = cind[j];
-            ve[p] = val[j];
+            ve[p] = enc(val[j]);
         }
         pos += __popcll(b);
     }
@@ -166,12 +195,12 @@ __global__ __launch_bounds__(kBlock) void k_tri_fill(int nslices, const int4 *__
         const bool on = (mk >> cc) & 1;
         const unsigned long long b = __ballot(on);
         if (on) {
-            vi[ipos + lanes_below(b)] = val[q];
+            vi[ipos + lanes_below(b)] = enc(val[q]);
             q += UPPER ? -1 : 1;
         }
         ipos += __popcll(b);
     }
-    if (UPPER) dinv[slot] = lr.active ? val[s] : T(0);
+    if (UPPER) dinv[slot] = lr.active ? (DiagT<V>)val[s] : DiagT<V>(0);
 }
 
 // ------------------------------------------------------------ single-launch sweep
@@ -218,13 +247,14 @@ constexpr int kSweepSpinLimit = 1 << 22;
 // column base+c subtract it.  SWEEP: gathers wait on the sentinel, the row is stored
 // device-coherent, and reset[row] (when given) receives the sentinel once x[row] has
 // been read.
-template <typename T, int BS, bool UPPER, bool SWEEP>
+template <typename T, typename V, int BS, bool UPPER, bool SWEEP>
 __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
                                             const typename RowWord<BS>::W *__restrict__ rw,
                                             const int *__restrict__ next, const int *__restrict__ scol,
-                                            const T *__restrict__ ve, const T *__restrict__ vi,
-                                            const T *__restrict__ dinv, const T *x, T *y,
-                                            const double *__restrict__ in_scale, T *reset, unsigned *err) {
+                                            const V *__restrict__ ve, const V *__restrict__ vi,
+                                            const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx,
+                                            const T *x, T *y, const double *__restrict__ in_scale, T *reset,
+                                            unsigned *err) {
     using U = typename Bits<T>::U;
     const int4 si = sinfo[slice];
     const LaneRow<BS> lr(si, blk_list);
@@ -232,7 +262,8 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
     int ne;
     MaskT<BS> mk;
     RowWord<BS>::unpack(rw[slot], next, slot, ne, mk);
-    const T dv = UPPER ? dinv[slot] : T(0);
+    const T dv = UPPER ? (T)dinv[slot] : T(0);
+    const int ex = kHalfV<V> ? (int)sx[slot] : 0;
     T t = 0;
     if (lr.active) {
         t = x[lr.row];
@@ -259,7 +290,7 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
         for (int q = 0; q < CH; q++)
             if (on[q]) {
                 cc[q] = scol[idx[q]];
-                vv[q] = ve[idx[q]];
+                vv[q] = decode<T>(ve[idx[q]], ex);
             }
     };
     if (more) load_chunk();
@@ -271,7 +302,7 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
         const int c = UPPER ? BS - 1 - q : q;
         on0[q] = (mk >> c) & 1;
         const unsigned long long b = __ballot(on0[q]);
-        iv0[q] = on0[q] ? vi[ipos + lanes_below(b)] : T(0);
+        iv0[q] = on0[q] ? decode<T>(vi[ipos + lanes_below(b)], ex) : T(0);
         ipos += __popcll(b);
     }
     while (more) {
@@ -326,7 +357,7 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
                 const int c = UPPER ? BS - 1 - (i0 + q) : i0 + q;
                 onq[q] = (mk >> c) & 1;
                 const unsigned long long b = __ballot(onq[q]);
-                iv[q] = onq[q] ? vi[ipos + lanes_below(b)] : T(0);
+                iv[q] = onq[q] ? decode<T>(vi[ipos + lanes_below(b)], ex) : T(0);
                 ipos += __popcll(b);
             }
         }
@@ -347,20 +378,21 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
 }
 
 // One launch = one schedule level (slices slice0 .. slice0 + nsl).
-template <typename T, int BS, bool UPPER>
+template <typename T, typename V, int BS, bool UPPER>
 __global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, const int4 *__restrict__ sinfo,
                                                         const int *__restrict__ blk_list,
                                                         const typename RowWord<BS>::W *__restrict__ rw,
                                                         const int *__restrict__ next, const int *__restrict__ scol,
-                                                        const T *__restrict__ ve, const T *__restrict__ vi,
-                                                        const T *__restrict__ dinv, const T *x, T *y,
+                                                        const V *__restrict__ ve, const V *__restrict__ vi,
+                                                        const DiagT<V> *__restrict__ dinv,
+                                                        const signed char *__restrict__ sx, const T *x, T *y,
                                                         const double *__restrict__ in_scale,
                                                         const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) return;
     const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (w >= nsl) return;
-    solve_slice<T, BS, UPPER, false>(slice0 + w, sinfo, blk_list, rw, next, scol, ve, vi, dinv, x, y, in_scale,
-                                     nullptr, nullptr);
+    solve_slice<T, V, BS, UPPER, false>(slice0 + w, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y,
+                                        in_scale, nullptr, nullptr);
 }
 
 // The whole triangle in one cooperative launch: every wave of the grid is resident at
@@ -369,19 +401,21 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, con
 // unfinished slice therefore waits on nothing unfinished and its wave, being resident
 // and done with its own smaller slices, works on it: the sweep always drains.  y must
 // hold the sentinel in every row on entry.
-template <typename T, int BS, bool UPPER>
+template <typename T, typename V, int BS, bool UPPER>
 __global__ __launch_bounds__(kBlock) void k_sptrsv_sweep(int nslices, const int4 *__restrict__ sinfo,
                                                          const int *__restrict__ blk_list,
                                                          const typename RowWord<BS>::W *__restrict__ rw,
                                                          const int *__restrict__ next, const int *__restrict__ scol,
-                                                         const T *__restrict__ ve, const T *__restrict__ vi,
-                                                         const T *__restrict__ dinv, const T *x, T *y,
+                                                         const V *__restrict__ ve, const V *__restrict__ vi,
+                                                         const DiagT<V> *__restrict__ dinv,
+                                                         const signed char *__restrict__ sx, const T *x, T *y,
                                                          const double *__restrict__ in_scale, T *reset,
                                                          unsigned *err, const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) return;
     const int nw = (int)gridDim.x * kWavesPerBlock;
     for (int t = (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6); t < nslices; t += nw)
-        solve_slice<T, BS, UPPER, true>(t, sinfo, blk_list, rw, next, scol, ve, vi, dinv, x, y, in_scale, reset, err);
+        solve_slice<T, V, BS, UPPER, true>(t, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
+                                           reset, err);
 }
 
 template <typename T>
@@ -410,59 +444,69 @@ int count_rows(Ctx *c, const Schedule &s, const TriLayout &L, const int *rptr, c
     return SENK_OK;
 }
 
-template <typename T, int BS>
-int fill_rows(Ctx *c, const Schedule &s, const TriLayout &L, const int *rptr, const int *cind, const T *val, int *scol,
-              T *ve, T *vi, T *dinv) {
+template <typename S, typename V, int BS>
+int fill_rows(Ctx *c, const Schedule &s, const TriLayout &L, const int *rptr, const int *cind, const S *val, int *scol,
+              V *ve, V *vi, DiagT<V> *dinv, signed char *sx) {
     using W = typename RowWord<BS>::W;
     if (L.upper)
-        k_tri_fill<T, BS, true><<<waves_grid(L.nslices), kBlock, 0, c->stream>>>(
-            L.nslices, L.sinfo, s.dblk, rptr, cind, val, (const W *)L.rw, L.next, scol, ve, vi, dinv);
+        k_tri_fill<S, V, BS, true><<<waves_grid(L.nslices), kBlock, 0, c->stream>>>(
+            L.nslices, L.sinfo, s.dblk, rptr, cind, val, (const W *)L.rw, L.next, scol, ve, vi, dinv, sx);
     else
-        k_tri_fill<T, BS, false><<<waves_grid(L.nslices), kBlock, 0, c->stream>>>(
-            L.nslices, L.sinfo, s.dblk, rptr, cind, val, (const W *)L.rw, L.next, scol, ve, vi, dinv);
+        k_tri_fill<S, V, BS, false><<<waves_grid(L.nslices), kBlock, 0, c->stream>>>(
+            L.nslices, L.sinfo, s.dblk, rptr, cind, val, (const W *)L.rw, L.next, scol, ve, vi, dinv, sx);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
 
-template <typename T, int BS>
-int solve_levels(Ctx *c, const Schedule &s, const TriLayout &L, const T *ve, const T *vi, const T *dinv, const T *x,
-                 T *y, const double *in_scale, const int *stop) {
+template <typename T, typename V, int BS>
+int solve_levels(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
+                 const signed char *sx, const T *x, T *y, const double *in_scale, const int *stop) {
     using W = typename RowWord<BS>::W;
     for (int q = 0; q < L.nsteps; q++) {
         const int sl0 = L.step_slice[q], nsl = L.step_slice[q + 1] - sl0;
         if (L.upper)
-            k_sptrsv_sell<T, BS, true><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, x, y, nullptr, stop);
+            k_sptrsv_sell<T, V, BS, true><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
+                sl0, nsl, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y, nullptr, stop);
         else
-            k_sptrsv_sell<T, BS, false><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, x, y, in_scale, stop);
+            k_sptrsv_sell<T, V, BS, false><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
+                sl0, nsl, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y, in_scale, stop);
         SENK_LAUNCH_CHECK();
     }
     return SENK_OK;
 }
 
-// Resident workgroups of a sweep kernel on the current device (the cooperative grid).
-template <typename T, int BS, bool UPPER>
+// Cooperative grid of a sweep kernel on the current device: half of the workgroups
+// that are resident at once by the runtime's occupancy and by a conservative count of
+// our own (VGPRs rounded up to 16 of the 512 per SIMD lane, 4 SIMDs per CU, at most 8
+// waves each).  Every wave must be resident for the sweep to drain; the margin keeps it
+// so whatever the register allocation granule is.
+template <typename T, typename V, int BS, bool UPPER>
 int sweep_grid() {
     static int cached[64] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
     if (!cached[dev]) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sptrsv_sweep<T, BS, UPPER>, kBlock, 0) !=
+        hipFuncAttributes fa{};
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sptrsv_sweep<T, V, BS, UPPER>, kBlock, 0) !=
                 hipSuccess ||
+            hipFuncGetAttributes(&fa, (const void *)k_sptrsv_sweep<T, V, BS, UPPER>) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0;
-        cached[dev] = per_cu * cus;
+        const int vgpr = std::max(16, (fa.numRegs + 15) / 16 * 16);
+        const int waves_per_simd = std::min(8, 512 / vgpr);
+        const int own = waves_per_simd * 4 / kWavesPerBlock;  // workgroups per CU
+        if (fa.localSizeBytes > 0 || fa.sharedSizeBytes > 0) return 0;
+        cached[dev] = std::max(1, std::min(per_cu, own) * cus / 2);
     }
     return cached[dev];
 }
 
-template <typename T, int BS, bool UPPER>
-int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const T *ve, const T *vi, const T *dinv, const T *x,
-                 T *y, const double *in_scale, T *reset, const int *stop) {
+template <typename T, typename V, int BS, bool UPPER>
+int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
+                 const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, const int *stop) {
     using W = typename RowWord<BS>::W;
-    int grid = sweep_grid<T, BS, UPPER>();
+    int grid = sweep_grid<T, V, BS, UPPER>();
     if (grid <= 0) return SENK_ERR_HIP;
     grid = std::min(grid, waves_grid(L.nslices));
     int nslices = L.nslices;
@@ -471,17 +515,17 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const T *ve, con
     const W *rw = (const W *)L.rw;
     unsigned *err = L.sync + kSweepErr;
     const double *sc = UPPER ? nullptr : in_scale;
-    void *args[] = {&nslices, &sinfo, &blk, &rw, &next, &scol, &ve, &vi, &dinv, &x, &y, &sc, &reset, &err, &stop};
-    SENK_HIP(hipLaunchCooperativeKernel((const void *)k_sptrsv_sweep<T, BS, UPPER>, dim3(grid), dim3(kBlock), args, 0,
-                                        c->stream));
+    void *args[] = {&nslices, &sinfo, &blk, &rw, &next, &scol, &ve, &vi, &dinv, &sx, &x, &y, &sc, &reset, &err, &stop};
+    SENK_HIP(hipLaunchCooperativeKernel((const void *)k_sptrsv_sweep<T, V, BS, UPPER>, dim3(grid), dim3(kBlock), args,
+                                        0, c->stream));
     return SENK_OK;
 }
 
-template <typename T, int BS>
-int solve_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const T *ve, const T *vi, const T *dinv, const T *x,
-                T *y, const double *in_scale, T *reset, const int *stop) {
-    if (L.upper) return launch_sweep<T, BS, true>(c, s, L, ve, vi, dinv, x, y, in_scale, reset, stop);
-    return launch_sweep<T, BS, false>(c, s, L, ve, vi, dinv, x, y, in_scale, reset, stop);
+template <typename T, typename V, int BS>
+int solve_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
+                const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, const int *stop) {
+    if (L.upper) return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
+    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
 }
 
 template <typename T>
@@ -509,7 +553,7 @@ size_t row_word_bytes(int bs) { return bs <= 16 ? sizeof(unsigned) : sizeof(unsi
 void tri_layout_free(TriLayout *L) {
     for (void *p : {(void *)L->sinfo, (void *)L->next, L->rw, (void *)L->scol, (void *)L->ve64, (void *)L->vi64,
                     (void *)L->d64, (void *)L->ve32, (void *)L->vi32, (void *)L->d32, (void *)L->sync, L->mid64,
-                    L->mid32})
+                    L->mid32, L->ve16, L->vi16, (void *)L->d16, (void *)L->sx16})
         if (p) (void)hipFree(p);
     *L = TriLayout{};
 }
@@ -604,7 +648,9 @@ int tri_layout_fill(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, co
         SENK_TRY(fill_sentinel<T>(c, L->n, (T *)*mid));
     }
     int *scol = L->cols_filled ? nullptr : L->scol;
-    auto run = [&]() -> int { SENK_BS_DISPATCH(L->bsize, (fill_rows<T, B>(c, s, *L, rptr, cind, val, scol, *ve, *vi, *dv))) };
+    auto run = [&]() -> int {
+        SENK_BS_DISPATCH(L->bsize, (fill_rows<T, T, B>(c, s, *L, rptr, cind, val, scol, *ve, *vi, *dv, nullptr)))
+    };
     SENK_TRY(run());
     L->cols_filled = true;
     return SENK_OK;
@@ -612,45 +658,86 @@ int tri_layout_fill(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, co
 template int tri_layout_fill<double>(Ctx *, const Schedule &, TriLayout *, const int *, const int *, const double *);
 template int tri_layout_fill<float>(Ctx *, const Schedule &, TriLayout *, const int *, const int *, const float *);
 
+int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, const int *cind, const double *val) {
+    if (L->nslices == 0) return SENK_OK;
+    if (!L->ve16) SENK_HIP(hipMalloc(&L->ve16, (size_t)(L->n_ext ? L->n_ext : 1) * sizeof(__half)));
+    if (!L->vi16) SENK_HIP(hipMalloc(&L->vi16, (size_t)(L->n_int ? L->n_int : 1) * sizeof(__half)));
+    if (!L->sx16) SENK_HIP(hipMalloc(&L->sx16, (size_t)L->nslices * 64));
+    if (L->upper && !L->d16) SENK_HIP(hipMalloc(&L->d16, (size_t)L->nslices * 64 * sizeof(float)));
+    if (!L->upper && !L->mid32) {
+        SENK_HIP(hipMalloc(&L->mid32, (size_t)L->n * sizeof(float)));
+        SENK_TRY(fill_sentinel<float>(c, L->n, (float *)L->mid32));
+    }
+    int *scol = L->cols_filled ? nullptr : L->scol;
+    auto run = [&]() -> int {
+        SENK_BS_DISPATCH(L->bsize, (fill_rows<double, __half, B>(c, s, *L, rptr, cind, val, scol, (__half *)L->ve16,
+                                                                 (__half *)L->vi16, L->d16, L->sx16)))
+    };
+    SENK_TRY(run());
+    L->cols_filled = true;
+    return SENK_OK;
+}
+
+namespace {
+template <typename T, typename V>
+int ilu_layout_apply(Ctx *c, const Schedule &ls, const TriLayout &lt, const Schedule &us, const TriLayout &ut,
+                     const V *lve, const V *lvi, const V *uve, const V *uvi, const DiagT<V> *udv,
+                     const signed char *lsx, const signed char *usx, const T *in, T *out, const double *in_scale,
+                     const int *stop) {
+    T *mid = (T *)(std::is_same<T, double>::value ? lt.mid64 : lt.mid32);
+    if (lt.mode == kTrsvSweep && mid && lt.sync && ut.sync) {
+        // L: in -> mid (every row of out becomes the sentinel for the U sweep);
+        // U: mid -> out (mid returns to the sentinel as its rows are read)
+        auto l = [&]() -> int {
+            SENK_BS_DISPATCH(lt.bsize, (solve_sweep<T, V, B>(c, ls, lt, lve, lvi, nullptr, lsx, in, mid, in_scale, out,
+                                                             stop)))
+        };
+        auto u = [&]() -> int {
+            SENK_BS_DISPATCH(ut.bsize,
+                             (solve_sweep<T, V, B>(c, us, ut, uve, uvi, udv, usx, mid, out, nullptr, mid, stop)))
+        };
+        SENK_TRY(l());
+        return u();
+    }
+    auto l = [&]() -> int {
+        SENK_BS_DISPATCH(lt.bsize, (solve_levels<T, V, B>(c, ls, lt, lve, lvi, nullptr, lsx, in, out, in_scale, stop)))
+    };
+    auto u = [&]() -> int {
+        SENK_BS_DISPATCH(ut.bsize, (solve_levels<T, V, B>(c, us, ut, uve, uvi, udv, usx, out, out, nullptr, stop)))
+    };
+    SENK_TRY(l());
+    return u();
+}
+}  // namespace
+
 template <typename T>
 int launch_ilu_layout(Ctx *c, const Schedule &ls, const TriLayout &lt, const Schedule &us, const TriLayout &ut,
-                      const T *in, T *out, const double *in_scale, const int *stop) {
+                      const T *in, T *out, const double *in_scale, const int *stop, bool f16) {
+    if (lt.nslices == 0 || ut.nslices == 0) return lt.nslices == ut.nslices ? SENK_OK : SENK_ERR_ARG;
+    if (lt.bsize != ls.bsize || ut.bsize != us.bsize || lt.upper || !ut.upper || lt.n != ut.n) return SENK_ERR_ARG;
+    if constexpr (std::is_same<T, float>::value) {
+        if (f16) {
+            if (!lt.ve16 || !lt.vi16 || !ut.ve16 || !ut.vi16 || !ut.d16 || !lt.sx16 || !ut.sx16) return SENK_ERR_ARG;
+            return ilu_layout_apply<float, __half>(c, ls, lt, us, ut, (const __half *)lt.ve16, (const __half *)lt.vi16,
+                                                   (const __half *)ut.ve16, (const __half *)ut.vi16, ut.d16, lt.sx16,
+                                                   ut.sx16, in, out, in_scale, stop);
+        }
+    } else {
+        if (f16) return SENK_ERR_ARG;
+    }
     const bool f64 = std::is_same<T, double>::value;
     const T *lve = (const T *)(f64 ? (void *)lt.ve64 : (void *)lt.ve32);
     const T *lvi = (const T *)(f64 ? (void *)lt.vi64 : (void *)lt.vi32);
     const T *uve = (const T *)(f64 ? (void *)ut.ve64 : (void *)ut.ve32);
     const T *uvi = (const T *)(f64 ? (void *)ut.vi64 : (void *)ut.vi32);
     const T *udv = (const T *)(f64 ? (void *)ut.d64 : (void *)ut.d32);
-    T *mid = (T *)(f64 ? lt.mid64 : lt.mid32);
-    if (lt.nslices == 0 || ut.nslices == 0) return lt.nslices == ut.nslices ? SENK_OK : SENK_ERR_ARG;
-    if (!lve || !lvi || !uve || !uvi || !udv || lt.bsize != ls.bsize || ut.bsize != us.bsize || lt.upper ||
-        !ut.upper || lt.n != ut.n)
-        return SENK_ERR_ARG;
-    if (lt.mode == kTrsvSweep && mid && lt.sync && ut.sync) {
-        // L: in -> mid (every row of out becomes the sentinel for the U sweep);
-        // U: mid -> out (mid returns to the sentinel as its rows are read)
-        auto l = [&]() -> int {
-            SENK_BS_DISPATCH(lt.bsize, (solve_sweep<T, B>(c, ls, lt, lve, lvi, nullptr, in, mid, in_scale, out, stop)))
-        };
-        auto u = [&]() -> int {
-            SENK_BS_DISPATCH(ut.bsize, (solve_sweep<T, B>(c, us, ut, uve, uvi, udv, mid, out, nullptr, mid, stop)))
-        };
-        SENK_TRY(l());
-        return u();
-    }
-    auto l = [&]() -> int {
-        SENK_BS_DISPATCH(lt.bsize, (solve_levels<T, B>(c, ls, lt, lve, lvi, nullptr, in, out, in_scale, stop)))
-    };
-    auto u = [&]() -> int {
-        SENK_BS_DISPATCH(ut.bsize, (solve_levels<T, B>(c, us, ut, uve, uvi, udv, out, out, nullptr, stop)))
-    };
-    SENK_TRY(l());
-    return u();
+    if (!lve || !lvi || !uve || !uvi || !udv) return SENK_ERR_ARG;
+    return ilu_layout_apply<T, T>(c, ls, lt, us, ut, lve, lvi, uve, uvi, udv, nullptr, nullptr, in, out, in_scale, stop);
 }
 template int launch_ilu_layout<double>(Ctx *, const Schedule &, const TriLayout &, const Schedule &,
-                                       const TriLayout &, const double *, double *, const double *, const int *);
+                                       const TriLayout &, const double *, double *, const double *, const int *, bool);
 template int launch_ilu_layout<float>(Ctx *, const Schedule &, const TriLayout &, const Schedule &, const TriLayout &,
-                                      const float *, float *, const double *, const int *);
+                                      const float *, float *, const double *, const int *, bool);
 
 int tri_layout_check(Ctx *c, TriLayout *L) {
     if (!L->sync) return SENK_OK;
