@@ -28,6 +28,7 @@ struct senk_mat_s {
     float *v32 = nullptr;
     bool has_sell = false;
     senk_sellcs sell{};
+    int *sell_order = nullptr;  // device: slice order of the SpMV (null: slice order)
     senk_amd::MatView view;
 };
 struct senk_prec_s {
@@ -1060,6 +1061,7 @@ int senk_mat_destroy(senk_mat A) {
     (void)hipFree(A->v64);
     (void)hipFree(A->v32);
     if (A->has_sell) senk_sellcs_free_device(A->ctx, &A->sell);
+    (void)hipFree(A->sell_order);
     delete A;
     return SENK_OK;
 }
@@ -1074,6 +1076,41 @@ int senk_mat_spmv_f32(senk_mat A, const float *dx, float *dy) {
 int senk_mat_nnz(senk_mat A, int *nnz) {
     if (!A || !nnz) return SENK_ERR_ARG;
     *nnz = A->nnz;
+    return SENK_OK;
+}
+
+// With a color ordering every color spans the whole domain, in the same spatial order.
+// The SELL SpMV hands its slices to the workgroups so that the j-th eighth of every
+// color comes in the j-th eighth of the grid, which the workgroup numbering gives to
+// XCD j: an XCD then works on one region of the domain and finds the gathered entries
+// of x in its own L2.  Each row is still computed alone, so results do not change.
+static int set_sell_order(senk_mat A, const int *cptr, int cnum, int bsize) {
+    if (!A->has_sell || !cptr || cnum <= 0) return SENK_OK;
+    const senk_sellcs &d = A->sell;
+    const int ns = d.nslices, C = d.C;
+    std::vector<int> perm((size_t)ns * C);
+    Ctx *c = &A->ctx->c;
+    SENK_HIP(hipMemcpyAsync(perm.data(), d.perm, perm.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> eighth(ns);
+    for (int q = 0; q < ns; q++) {
+        const int row = perm[(size_t)q * C];
+        int col = (int)(std::upper_bound(cptr, cptr + cnum + 1, row / bsize) - cptr) - 1;
+        col = std::max(0, std::min(cnum - 1, col));
+        const long long r0 = (long long)cptr[col] * bsize, r1 = (long long)cptr[col + 1] * bsize;
+        const long long e = r1 > r0 ? (8 * (row - r0)) / (r1 - r0) : 0;
+        eighth[q] = (int)std::max(0LL, std::min(7LL, e));
+    }
+    std::vector<int> order;
+    order.reserve(ns);
+    for (int e = 0; e < 8; e++)
+        for (int q = 0; q < ns; q++)
+            if (eighth[q] == e) order.push_back(q);
+    if (!A->sell_order) SENK_TRY(dalloc(&A->sell_order, (size_t)ns));
+    SENK_HIP(hipMemcpyAsync(A->sell_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
+                            c->stream));
+    SENK_HIP(hipStreamSynchronize(c->stream));
+    A->view.sell_order = A->sell_order;
     return SENK_OK;
 }
 
@@ -1095,6 +1132,10 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
     if (opts->kind == SENK_ILUP && opts->p > 0) st = senk_alloc_level_zero(&fv, &fc, &fr, N, opts->p);
     else if (opts->kind == SENK_ILUB) st = senk_alloc_block_zero(&fv, &fc, &fr, N, opts->bnl, opts->bnw);
     if (st) {
+        std::free(fv); std::free(fc); std::free(fr);
+        return st;
+    }
+    if ((st = set_sell_order(A, cptr, cnum, bsize))) {
         std::free(fv); std::free(fc); std::free(fr);
         return st;
     }

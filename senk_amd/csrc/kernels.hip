@@ -104,11 +104,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
                                                       const int *__restrict__ cind, const T *__restrict__ val,
                                                       const int *__restrict__ perm, const T *__restrict__ x,
                                                       T *__restrict__ y, int last_short,
-                                                      const int *__restrict__ stop) {
+                                                      const int *__restrict__ stop,
+                                                      const int *__restrict__ order) {
     if (stop && *stop) return;
     const long long gid = (long long)xcd_block() * kBlock + threadIdx.x;
-    const int s = (int)(gid / C), k = (int)(gid % C);
-    if (s >= nslices) return;
+    const int q = (int)(gid / C), k = (int)(gid % C);
+    if (q >= nslices) return;
+    const int s = order ? order[q] : q;
     const int h = (last_short && s == nslices - 1) ? N - s * C : C;
     if (k >= h) return;
     const size_t base = (size_t)wid[s] * C + k;
@@ -144,7 +146,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
 }
 template <typename T>
 int launch_spmv_sell(Ctx *c, int N, int C, int nslices, const int *wid, const int *cind, const T *val,
-                     const int *perm, const T *x, T *y, bool last_short, const int *stop, double avg_width) {
+                     const int *perm, const T *x, T *y, bool last_short, const int *stop, double avg_width,
+                     const int *order) {
     if (N <= 0 || nslices <= 0) return SENK_OK;
     long long threads = (long long)nslices * C;
     int grid = (int)((threads + kBlock - 1) / kBlock);
@@ -152,17 +155,17 @@ int launch_spmv_sell(Ctx *c, int N, int C, int nslices, const int *wid, const in
     // take chunks of 16
     if (avg_width > 0 && avg_width <= 8.5)
         k_spmv_sell<T, 8><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y,
-                                                          last_short ? 1 : 0, stop);
+                                                          last_short ? 1 : 0, stop, order);
     else
         k_spmv_sell<T, 16><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y,
-                                                           last_short ? 1 : 0, stop);
+                                                           last_short ? 1 : 0, stop, order);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
 template int launch_spmv_sell<double>(Ctx *, int, int, int, const int *, const int *, const double *, const int *,
-                                      const double *, double *, bool, const int *, double);
+                                      const double *, double *, bool, const int *, double, const int *);
 template int launch_spmv_sell<float>(Ctx *, int, int, int, const int *, const int *, const float *, const int *,
-                                     const float *, float *, bool, const int *, double);
+                                     const float *, float *, bool, const int *, double, const int *);
 
 // =============================================================== SpMV BCSR
 // G lanes per block row; lane q takes blocks brptr[g]+q, +G, ...; each block is

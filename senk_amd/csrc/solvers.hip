@@ -451,7 +451,8 @@ int spmv(Ctx *c, const MatView &A, const T *x, T *y, const int *stop) {
         if (sv)
             return launch_spmv_sell<T>(c, A.N, A.sell->C, A.sell->nslices, A.sell->slice_ptr, A.sell->cind, sv,
                                        A.sell->perm, x, y, false, stop,
-                                       (double)A.sell->nnz_padded / ((double)A.sell->nslices * A.sell->C));
+                                       (double)A.sell->nnz_padded / ((double)A.sell->nslices * A.sell->C),
+                                       A.sell_order);
     }
     if (A.bnl) {
         const T *bv = bvals<T>(A);

@@ -13,8 +13,10 @@ struct MatView {
     const double *v64 = nullptr;
     const float *v32 = nullptr;
     double avg_row = 0;
-    // optional SELL-C-sigma copy (device)
+    // optional SELL-C-sigma copy (device) and the order in which its slices are handed
+    // to the workgroups (null: slice order)
     const senk_sellcs *sell = nullptr;
+    const int *sell_order = nullptr;
     // optional BCSR form used for SpMV (device); bnl == 0 when absent
     int bnl = 0, bnw = 0;
     const int *brptr = nullptr, *bcind = nullptr;

@@ -389,7 +389,9 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, con
                                                         const double *__restrict__ in_scale,
                                                         const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) return;
-    const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    // each XCD takes a contiguous part of the level: within one color, neighbouring
+    // slices cover neighbouring parts of the domain, so their gathers share that L2
+    const int w = xcd_block() * kWavesPerBlock + (threadIdx.x >> 6);
     if (w >= nsl) return;
     solve_slice<T, V, BS, UPPER, false>(slice0 + w, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y,
                                         in_scale, nullptr, nullptr);

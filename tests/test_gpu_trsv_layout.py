@@ -187,3 +187,37 @@ def test_sweep_modes_vardiff64(ctx, bsize):
         for M in (Ml, Mc):
             S.lib().senk_prec_destroy(M)
         S.lib().senk_mat_destroy(A)
+
+
+def test_spmv_slice_order_keeps_bits(ctx):
+    # creating a colored preconditioner reorders how the matrix's SELL slices are handed
+    # to the workgroups (by region of the domain); every row is still computed alone, so
+    # the products keep their bits, in FP64 and FP32
+    rptr, cind, val = S.gen_vardiff7(40, 2, 8.0, 1.5)
+    sy = abmc_system(rptr, cind, val, 8)
+    N = sy["N"]
+    A = C.c_void_p()
+    S.call("senk_mat_create", ctx.h, N, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]), 1 | 2, C.byref(A))
+    x = np.random.default_rng(9).uniform(-1, 1, N)
+    dx, dx32 = ctx.array(x), ctx.array(x.astype(np.float32))
+
+    def products():
+        dy, dy32 = ctx.zeros(N), ctx.array(np.zeros(N, np.float32))
+        S.call("senk_mat_spmv_f64", A, dx.p, dy.p)
+        S.call("senk_mat_spmv_f32", A, dx32.p, dy32.p)
+        return dy.download(), dy32.download()
+
+    before = products()
+    opts = S.senk_ilu_opts(S.SENK_ILU0, 0, 0, 0, 1 << S.SENK_FACTOR_F64, 0)
+    M = C.c_void_p()
+    S.call("senk_prec_create_ilu", ctx.h, A, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]),
+           sy["cptr"].ctypes.data_as(C.c_void_p), sy["nc"], 8, C.byref(opts), C.byref(M))
+    try:
+        after = products()
+        assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+        rows = np.repeat(np.arange(N), np.diff(sy["rptr"]))
+        want = np.bincount(rows, weights=sy["val"] * x[sy["cind"]], minlength=N)
+        assert np.max(np.abs(after[0] - want)) <= 1e-12 * np.max(np.abs(want))
+    finally:
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
