@@ -150,9 +150,9 @@ def kernel_rates(ctx, dev, reps=50, sweep_modes=False):
            "colors": dev.sy["nc"], "bsize": dev.sy["bsize"]}
     if not sweep_modes:
         return out
-    # triangular-solve launch modes: one cooperative sweep per triangle, then the default
-    # one launch per level
-    for mode in (1, 0):
+    # triangular-solve launch modes: one cooperative sweep per triangle, one ticketed
+    # sweep per triangle, then the default one launch per level
+    for mode in (1, 2, 0):
         S.call("senk_prec_set_trsv_mode", dev.M, mode)
         out[f"ilu_apply_f64_us_mode{mode}"] = time_kernel(
             ctx, lambda: S.call("senk_prec_apply_f64", dev.M, dx.p, dy.p), 20) * 1e3

@@ -396,9 +396,11 @@ int senk_prec_apply_f32(senk_prec M, const float *dx, float *dy);
 int senk_prec_apply_f16(senk_prec M, const float *dx, float *dy);
 /* Triangular-solve launch mode of the sliced layout (no reference counterpart; the
  * reference runs one OpenMP parallel loop per color, senk_sparse.hpp:207-246):
- * 0 (default) one launch per level; 1 one cooperative launch per triangle, each
- * gather waiting until the row it reads is final.  Results are bit-identical in both
- * modes.  Applications of one preconditioner must not overlap (one stream). */
+ * 0 one launch per level; 1 one cooperative launch per triangle, each gather waiting
+ * until the row it reads is final; 2 (default) one launch per triangle of workgroups
+ * that draw tickets in arrival order (no residency assumption), gathers waiting
+ * likewise.  Results are bit-identical in every mode.  Applications of one
+ * preconditioner must not overlap (one stream). */
 int senk_prec_set_trsv_mode(senk_prec M, int mode);
 /* Synchronise and report SENK_ERR_TIMEOUT if a single-launch solve since the last check
  * gave up a wait (never expected; senk_solve checks by itself). */

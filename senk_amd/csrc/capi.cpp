@@ -1287,7 +1287,7 @@ int senk_prec_apply_f16(senk_prec M, const float *dx, float *dy) {
     return prec_apply<float>(&M->ctx->c, v, dx, nullptr, dy, nullptr, nullptr);
 }
 int senk_prec_set_trsv_mode(senk_prec M, int mode) {
-    if (!M || mode < senk_amd::kTrsvLevels || mode > senk_amd::kTrsvSweep) return SENK_ERR_ARG;
+    if (!M || mode < senk_amd::kTrsvLevels || mode > senk_amd::kTrsvTicket) return SENK_ERR_ARG;
     SENK_HIP(hipStreamSynchronize(M->ctx->c.stream));
     M->lt.mode = mode;
     M->ut.mode = mode;

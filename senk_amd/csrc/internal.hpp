@@ -83,8 +83,9 @@ struct Schedule {
 // entries (value; column = bit of `mask`), each group stored packed and in the order
 // the solve consumes it.  Built once per pattern, filled per precision.
 // Launch modes of the layout solve: one launch per level, or one launch per triangle
-// whose gathers wait for the rows they read (sentinel-initialised output).
-constexpr int kTrsvLevels = 0, kTrsvSweep = 1;
+// whose gathers wait for the rows they read (sentinel-initialised output), with every
+// wave resident (cooperative) or with workgroups drawing tickets in arrival order.
+constexpr int kTrsvLevels = 0, kTrsvSweep = 1, kTrsvTicket = 2;
 struct TriLayout {
     int bsize = 0;
     bool upper = false;
@@ -106,7 +107,8 @@ struct TriLayout {
     void *ve16 = nullptr, *vi16 = nullptr;
     float *d16 = nullptr;
     signed char *sx16 = nullptr;
-    int mode = kTrsvLevels;
+    int mode = kTrsvTicket;
+    unsigned ticket_base = 0;  // ticketed sweep: workgroups launched so far (mod 2^32)
 };
 int tri_layout_check(Ctx *c, TriLayout *L);
 int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rptr, const int *cind,
@@ -117,7 +119,7 @@ void tri_layout_free(TriLayout *L);
 // ILU application out = U^-1 L^-1 (in_scale * in) with the two layouts; in may equal out.
 // f16 (T = float only): the FP16 copies of the factor values.
 template <typename T>
-int launch_ilu_layout(Ctx *c, const Schedule &ls, const TriLayout &lt, const Schedule &us, const TriLayout &ut,
+int launch_ilu_layout(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule &us, TriLayout &ut,
                       const T *in, T *out, const double *in_scale, const int *stop, bool f16 = false);
 int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, const int *cind, const double *val);
 

@@ -40,7 +40,7 @@ struct PrecView {
     const double *l64 = nullptr, *u64 = nullptr;
     const float *l32 = nullptr, *u32 = nullptr;
     // optional solve layouts of the CSR factors (used when they hold the precision)
-    const TriLayout *lt = nullptr, *ut = nullptr;
+    TriLayout *lt = nullptr, *ut = nullptr;  // the ticketed sweep advances their counters
     // FP32 applications read the layouts' FP16 factor copies (per-row scaled)
     bool f16 = false;
     // BCSR factors

@@ -209,7 +209,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_fill(int nslices, const int4 *__
 // input holding the pattern cannot leak through either).  A gather of an earlier row
 // that still reads the sentinel polls again; a row's value is stored once, so the
 // first non-sentinel value read is the final one.  Stores of the output and the polls
-// go to the device-coherent level (every XCD has its own L2).
+// go to the device-coherent level (every XCD has its own L2, which a row written on
+// another XCD does not update).
 template <typename T>
 struct Bits {
     using U = unsigned;
@@ -225,6 +226,16 @@ __device__ __forceinline__ typename Bits<T>::U load_agent(const T *p) {
     using U = typename Bits<T>::U;
     return __hip_atomic_load((const U *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// First read of a gathered row: through the XCD's L2 (workgroup scope).  At the start of
+// a launch no cache holds an older value of any row (the sentinels were written by an
+// earlier launch), and within the launch a row changes once, from the sentinel to its
+// final value; so this read gives the final value or the sentinel, and only a sentinel
+// is polled again at device scope.
+template <typename T>
+__device__ __forceinline__ typename Bits<T>::U load_local(const T *p) {
+    using U = typename Bits<T>::U;
+    return __hip_atomic_load((const U *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 template <typename T>
 __device__ __forceinline__ void store_agent(T *p, T v) {
     using U = typename Bits<T>::U;
@@ -233,8 +244,8 @@ __device__ __forceinline__ void store_agent(T *p, T v) {
 __device__ __forceinline__ float canonical(float v) { return __builtin_canonicalizef(v); }
 __device__ __forceinline__ double canonical(double v) { return __builtin_canonicalize(v); }
 
-// Sweep words (unsigned, device): time-out flag, spares.
-constexpr int kSweepErr = 0, kSweepWords = 4;
+// Sweep words (unsigned, device): time-out flag, ticket counter, spares.
+constexpr int kSweepErr = 0, kSweepTicket = 1, kSweepWords = 4;
 // polls of one pending gather before the wave gives up and raises the time-out flag;
 // a poll takes about a microsecond, so this bounds a wait at a few seconds
 constexpr int kSweepSpinLimit = 1 << 22;
@@ -312,7 +323,7 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
             bool pend = false;
 #pragma unroll
             for (int q = 0; q < CH; q++)
-                if (on[q]) yb[q] = load_agent(y + cc[q]);
+                if (on[q]) yb[q] = load_local(y + cc[q]);
 #pragma unroll
             for (int q = 0; q < CH; q++) pend |= on[q] && yb[q] == Bits<T>::kSentinel;
             for (int spin = 0; __ballot(pend); spin++) {
@@ -420,6 +431,41 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sweep(int nslices, const int4
                                            reset, err);
 }
 
+// The whole triangle in one launch of 16-wave workgroups, each solving 16 consecutive
+// slices.  A workgroup draws a ticket when it starts; tickets follow arrival, and ticket
+// q solves slices 16q .. 16q+15.  A gather only waits on rows of earlier levels, i.e. on
+// smaller slice numbers, held by workgroups that drew smaller tickets and are running or
+// done: the launch drains without assuming that all of it is resident.  The ticket
+// counter is never reset: every launch advances it by exactly its workgroup count (a
+// stopped launch in one step), and the host passes the count of earlier launches.
+constexpr int kTicketWaves = 16;
+template <typename T, typename V, int BS, bool UPPER>
+__global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
+    int nslices, unsigned base, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
+    const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
+    const V *__restrict__ ve, const V *__restrict__ vi, const DiagT<V> *__restrict__ dinv,
+    const signed char *__restrict__ sx, const T *x, T *y, const double *__restrict__ in_scale, T *reset,
+    unsigned *sync, const int *__restrict__ stop_flag) {
+    if (stop_flag && *stop_flag) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            __hip_atomic_fetch_add(sync + kSweepTicket, gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    __shared__ unsigned ticket;
+    if (threadIdx.x == 0)
+        ticket = __hip_atomic_fetch_add(sync + kSweepTicket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base;
+    __syncthreads();
+    const unsigned q = ticket;
+    if (q >= gridDim.x) {  // launches out of step with the host's count
+        if (threadIdx.x == 0) __hip_atomic_store(sync + kSweepErr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const int slice = (int)q * kTicketWaves + (int)(threadIdx.x >> 6);
+    if (slice < nslices)
+        solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
+                                           reset, sync + kSweepErr);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill_sentinel(int n, T *p) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -523,9 +569,27 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
     return SENK_OK;
 }
 
+template <typename T, typename V, int BS, bool UPPER>
+int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
+                  const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, const int *stop) {
+    using W = typename RowWord<BS>::W;
+    const unsigned grid = (unsigned)((L.nslices + kTicketWaves - 1) / kTicketWaves);
+    const unsigned base = L.ticket_base;
+    k_sptrsv_ticket<T, V, BS, UPPER><<<grid, 64 * kTicketWaves, 0, c->stream>>>(
+        L.nslices, base, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
+        UPPER ? nullptr : in_scale, reset, L.sync, stop);
+    SENK_LAUNCH_CHECK();
+    L.ticket_base = base + grid;
+    return SENK_OK;
+}
+
 template <typename T, typename V, int BS>
-int solve_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
+int solve_sweep(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                 const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, const int *stop) {
+    if (L.mode == kTrsvTicket) {
+        if (L.upper) return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
+        return launch_ticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
+    }
     if (L.upper) return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
     return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
 }
@@ -682,12 +746,12 @@ int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr
 
 namespace {
 template <typename T, typename V>
-int ilu_layout_apply(Ctx *c, const Schedule &ls, const TriLayout &lt, const Schedule &us, const TriLayout &ut,
+int ilu_layout_apply(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule &us, TriLayout &ut,
                      const V *lve, const V *lvi, const V *uve, const V *uvi, const DiagT<V> *udv,
                      const signed char *lsx, const signed char *usx, const T *in, T *out, const double *in_scale,
                      const int *stop) {
     T *mid = (T *)(std::is_same<T, double>::value ? lt.mid64 : lt.mid32);
-    if (lt.mode == kTrsvSweep && mid && lt.sync && ut.sync) {
+    if (lt.mode != kTrsvLevels && mid && lt.sync && ut.sync) {
         // L: in -> mid (every row of out becomes the sentinel for the U sweep);
         // U: mid -> out (mid returns to the sentinel as its rows are read)
         auto l = [&]() -> int {
@@ -713,7 +777,7 @@ int ilu_layout_apply(Ctx *c, const Schedule &ls, const TriLayout &lt, const Sche
 }  // namespace
 
 template <typename T>
-int launch_ilu_layout(Ctx *c, const Schedule &ls, const TriLayout &lt, const Schedule &us, const TriLayout &ut,
+int launch_ilu_layout(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule &us, TriLayout &ut,
                       const T *in, T *out, const double *in_scale, const int *stop, bool f16) {
     if (lt.nslices == 0 || ut.nslices == 0) return lt.nslices == ut.nslices ? SENK_OK : SENK_ERR_ARG;
     if (lt.bsize != ls.bsize || ut.bsize != us.bsize || lt.upper || !ut.upper || lt.n != ut.n) return SENK_ERR_ARG;
@@ -736,9 +800,9 @@ int launch_ilu_layout(Ctx *c, const Schedule &ls, const TriLayout &lt, const Sch
     if (!lve || !lvi || !uve || !uvi || !udv) return SENK_ERR_ARG;
     return ilu_layout_apply<T, T>(c, ls, lt, us, ut, lve, lvi, uve, uvi, udv, nullptr, nullptr, in, out, in_scale, stop);
 }
-template int launch_ilu_layout<double>(Ctx *, const Schedule &, const TriLayout &, const Schedule &,
-                                       const TriLayout &, const double *, double *, const double *, const int *, bool);
-template int launch_ilu_layout<float>(Ctx *, const Schedule &, const TriLayout &, const Schedule &, const TriLayout &,
+template int launch_ilu_layout<double>(Ctx *, const Schedule &, TriLayout &, const Schedule &, TriLayout &,
+                                       const double *, double *, const double *, const int *, bool);
+template int launch_ilu_layout<float>(Ctx *, const Schedule &, TriLayout &, const Schedule &, TriLayout &,
                                       const float *, float *, const double *, const int *, bool);
 
 int tri_layout_check(Ctx *c, TriLayout *L) {
@@ -750,6 +814,7 @@ int tri_layout_check(Ctx *c, TriLayout *L) {
     // a wait that gave up leaves sentinels or stale rows in the intermediate vector:
     // start the next application from a clean state
     SENK_HIP(hipMemsetAsync(L->sync, 0, kSweepWords * sizeof(unsigned), c->stream));
+    L->ticket_base = 0;
     if (L->mid64) SENK_TRY(fill_sentinel<double>(c, L->n, (double *)L->mid64));
     if (L->mid32) SENK_TRY(fill_sentinel<float>(c, L->n, (float *)L->mid32));
     SENK_HIP(hipStreamSynchronize(c->stream));

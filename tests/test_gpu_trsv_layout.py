@@ -6,9 +6,10 @@ L-then-U solve bit for bit in FP64 (same operations in the same order per row), 
 the same bits as each other in FP32, over every block size the layout takes, for
 color schedules and for dependency-level schedules (ILU(p) fill, natural ordering).
 The layout solve runs in each launch mode (one launch per level; one launch per
-triangle whose gathers wait for the rows they read): both give the same bits, also
-when applied many times in a row (the sweep's intermediate vector and ticket counters
-must return to their initial state after every application) and in place.
+triangle whose gathers wait for the rows they read, cooperative or with workgroup
+tickets): all give the same bits, also when applied many times in a row (the sweep's
+intermediate vector must return to the sentinel and the ticket count stay in step with
+the host after every application) and in place.
 """
 import ctypes as C
 
@@ -20,7 +21,7 @@ from oracle import oracle as O
 from tests.helpers import abmc_system, oracle_ilu0
 
 pytestmark = pytest.mark.gpu
-MODES = (0, 1)  # per-level launches, single-launch sweep
+MODES = (0, 1, 2)  # per-level launches, cooperative sweep, ticketed sweep
 
 
 def _expected(sy, kind, p):
@@ -141,14 +142,17 @@ def test_sweep_modes_vardiff64(ctx, bsize):
     try:
         rng = np.random.default_rng(11)
         xs = [rng.uniform(-1, 1, N) for _ in range(3)]
+        dxs = [(ctx.array(x), ctx.array(x.astype(np.float32))) for x in xs]
+        # one output vector per precision for every application, inputs changing in
+        # between: a value left over from an earlier application would show
+        dy, dy32 = ctx.zeros(N), ctx.array(np.zeros(N, np.float32))
         ref = {}
         for mode in MODES:
             S.call("senk_prec_set_trsv_mode", Ml, mode)
             for rep in range(4):
                 for i, x in enumerate(xs):
-                    dx, dy = ctx.array(x), ctx.zeros(N)
+                    dx, dx32 = dxs[i]
                     S.call("senk_prec_apply_f64", Ml, dx.p, dy.p)
-                    dx32, dy32 = ctx.array(x.astype(np.float32)), ctx.array(np.zeros(N, np.float32))
                     S.call("senk_prec_apply_f32", Ml, dx32.p, dy32.p)
                     got = (dy.download(), dy32.download())
                     if (i, 0) not in ref:
