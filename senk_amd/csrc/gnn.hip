@@ -61,6 +61,10 @@ struct GnnGraph {
     float *s = nullptr;                            // owned
     int *own_trptr = nullptr, *own_tcind = nullptr;
     float *own_tval = nullptr;
+    // order in which the layer kernels take their tiles of 32 nodes (null: index order):
+    // with a color ordering, the j-th eighth of every color comes in the j-th eighth of
+    // the grid, i.e. on XCD j, so that neighbours' features are found in that XCD's L2
+    int *torder = nullptr;                         // owned
 };
 struct GnnWork {
     int N = 0;
@@ -148,8 +152,11 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
                                                    const float *__restrict__ val, const float *__restrict__ s,
                                                    const float2 *__restrict__ xin, const float2 *__restrict__ xs,
                                                    const double *sc, const float *__restrict__ th,
-                                                   unsigned short *h1, float2 *xkeep, float2 *m1keep) {
-    const int i = xcd_block() * kBlock + threadIdx.x;
+                                                   unsigned short *h1, float2 *xkeep, float2 *m1keep,
+                                                   const int *__restrict__ torder) {
+    const int tl = (xcd_block() * kBlock + (int)threadIdx.x) >> 5;
+    if (tl >= (N + 31) / 32) return;
+    const int i = (torder ? torder[tl] : tl) * 32 + (threadIdx.x & 31);
     if (i >= N) return;
     const float inv = (float)sc[1];
     float a0 = 0, a1 = 0;
@@ -195,10 +202,14 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
                                                    const unsigned short *__restrict__ hin, const bf16x4 *__restrict__ wb,
                                                    const float *__restrict__ bias, unsigned short *hout,
                                                    unsigned short *mkeep, const float *__restrict__ wo,
-                                                   const float *__restrict__ bo, float *chat) {
-    __shared__ float red[kWavesPerBlock][32][33];
+                                                   const float *__restrict__ bo, float *chat,
+                                                   const int *__restrict__ torder) {
+    __shared__ float red[LAST ? kWavesPerBlock : 1][32][33];
+    __shared__ __align__(16) unsigned short tileh[kWavesPerBlock][32 * W];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n0 = (xcd_block() * kWavesPerBlock + wave) * 32;
+    const int ntiles = (N + 31) / 32;
+    const int tl = xcd_block() * kWavesPerBlock + wave;
+    const int n0 = (tl < ntiles ? (torder ? torder[tl] : tl) : ntiles) * 32;
     const int half = lane >> 5, col = lane & 31;
     const int node = n0 + col;
     const bool act = node < N;
@@ -266,10 +277,21 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int row = 8 * (r >> 2) + 4 * half + (r & 3);
-        const int nd = n0 + row;
         const unsigned short hv = f2bf(fmaxf(acc[r] + b, 0.f));
-        if (hout && nd < N) hout[(size_t)nd * W + fpos(col)] = hv;
+        if (hout) tileh[wave][row * W + fpos(col)] = hv;
         if (LAST) red[wave][row][col] = bf2f(hv) * wo[col];
+    }
+    if (hout) {
+        // the tile's 32 node records (64 bytes each, stored order) leave the LDS as two
+        // 16-byte stores per lane, consecutive lanes writing consecutive bytes
+        __syncthreads();
+        const int row = lane >> 1, hh = lane & 1;
+        if (n0 + row < N) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(&tileh[wave][row * W + 16 * hh]);
+            uint4 *dst = reinterpret_cast<uint4 *>(hout + (size_t)(n0 + row) * W + 16 * hh);
+            dst[0] = src[0];
+            dst[1] = src[1];
+        }
     }
     if (LAST) {
         __syncthreads();
@@ -476,6 +498,8 @@ __global__ void k_zero_d(int n, double *p) {
 
 inline int node_grid(int N) { return (N + kBlock - 1) / kBlock; }
 inline int tile_grid(int N) { return (N + 32 * kWavesPerBlock - 1) / (32 * kWavesPerBlock); }
+// one thread per node of whole 32-node tiles
+inline int tile_threads_grid(int N) { return (int)(((long long)(N + 31) / 32 * 32 + kBlock - 1) / kBlock); }
 
 template <typename T>
 int dalloc(T **p, size_t n) {
@@ -486,7 +510,7 @@ int dalloc(T **p, size_t n) {
 
 // ----------------------------------------------------------------- host helpers
 void gnn_graph_free(GnnGraph *g) {
-    for (void *p : {(void *)g->s, (void *)g->own_trptr, (void *)g->own_tcind, (void *)g->own_tval})
+    for (void *p : {(void *)g->s, (void *)g->own_trptr, (void *)g->own_tcind, (void *)g->own_tval, (void *)g->torder})
         if (p) (void)hipFree(p);
     *g = GnnGraph{};
 }
@@ -611,15 +635,17 @@ int gnn_forward(Ctx *c, senk_gnn_s *net, const GnnGraph &g, GnnWork &w, const T 
     k_gnn_sigma<<<1, kBlock, 0, c->stream>>>(P, B, N, w.sc);
     SENK_LAUNCH_CHECK();
     const float *th = net->theta_r;
-    k_gnn_l1<<<node_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.xin, w.xs, w.sc, th, w.h1,
-                                                       keep ? w.x : nullptr, keep ? w.m1 : nullptr);
+    k_gnn_l1<<<tile_threads_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.xin, w.xs, w.sc, th,
+                                                               w.h1, keep ? w.x : nullptr, keep ? w.m1 : nullptr,
+                                                               g.torder);
     SENK_LAUNCH_CHECK();
     k_gnn_mc<false><<<tile_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.h1, net->wb, th + ob2,
-                                                             w.h2, keep ? w.m2 : nullptr, nullptr, nullptr, nullptr);
+                                                             w.h2, keep ? w.m2 : nullptr, nullptr, nullptr, nullptr,
+                                                             g.torder);
     SENK_LAUNCH_CHECK();
     k_gnn_mc<true><<<tile_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.h2, net->wb + 512,
                                                             th + ob3, keep ? w.h3 : nullptr, keep ? w.m3 : nullptr,
-                                                            th + owo, th + obo, w.chat);
+                                                            th + owo, th + obo, w.chat, g.torder);
     SENK_LAUNCH_CHECK();
     k_gnn_out<T><<<node_grid(N), kBlock, 0, c->stream>>>(N, g.s, w.chat, w.sc, z);
     SENK_LAUNCH_CHECK();
@@ -741,6 +767,35 @@ int gnn_bind(senk_gnn_s *net, int N, const int *drptr, const int *dcind, const f
     return gnn_work_ensure(&net->w, N, net->w.train);
 }
 LearnedPrec *gnn_learned(senk_gnn_s *net) { return &net->lp; }
+// Tile order of the bound graph from a color schedule (levels = colors of consecutive
+// blocks); any other schedule, or a graph of another size, keeps the index order.
+int gnn_set_tile_order(senk_gnn_s *net, const Schedule &sch) {
+    GnnGraph &g = net->g;
+    (void)hipFree(g.torder);
+    g.torder = nullptr;
+    const int N = g.N;
+    if (N <= 0 || sch.dblk || sch.nlev <= 0 || sch.bsize <= 0 || (long long)sch.lev_ptr[sch.nlev] * sch.bsize != N)
+        return SENK_OK;
+    const int nt = (N + 31) / 32;
+    std::vector<int> region(nt);
+    for (int t = 0; t < nt; t++) {
+        const int blk = 32 * t / sch.bsize;
+        int l = (int)(std::upper_bound(sch.lev_ptr.begin(), sch.lev_ptr.begin() + sch.nlev + 1, blk) -
+                      sch.lev_ptr.begin()) - 1;
+        l = std::max(0, std::min(sch.nlev - 1, l));
+        const long long b0 = sch.lev_ptr[l], b1 = sch.lev_ptr[l + 1];
+        region[t] = b1 > b0 ? (int)std::min(7LL, 8 * (blk - b0) / (b1 - b0)) : 0;
+    }
+    std::vector<int> order;
+    order.reserve(nt);
+    for (int e = 0; e < 8; e++)
+        for (int t = 0; t < nt; t++)
+            if (region[t] == e) order.push_back(t);
+    SENK_TRY(dalloc(&g.torder, (size_t)nt));
+    SENK_HIP(hipMemcpyAsync(g.torder, order.data(), sizeof(int) * nt, hipMemcpyHostToDevice, net->c->stream));
+    SENK_HIP(hipStreamSynchronize(net->c->stream));
+    return SENK_OK;
+}
 int gnn_apply_f32(senk_gnn_s *net, const float *r, float *z) {
     if (net->g.N <= 0) return SENK_ERR_ARG;
     return gnn_forward<float>(net->c, net, net->g, net->w, r, nullptr, z, false);
