@@ -657,7 +657,7 @@ template int launch_sptrsv_bcsr<float>(Ctx *, const Schedule &, bool, int, int, 
 // row's upper part.  Entries thus receive their updates in increasing pivot column,
 // the reference's IKJ order (senk_matrix.hpp:418-448).  Pivot rows are final: earlier
 // levels were written by earlier launches, earlier rows of the same block by this
-// wave followed by a device-scope fence.
+// wave followed by a workgroup-scope fence.
 constexpr int kIluRegs = 4;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_ilu0(int nblk, int blk0, const int *__restrict__ blk_list, int BS,
@@ -728,7 +728,10 @@ __global__ __launch_bounds__(kBlock) void k_ilu0(int nblk, int blk0, const int *
             const int j = s + lane + 64 * q;
             if (j < e) val[j] = v[q];
         }
-        __threadfence();
+        // the next rows of the block read this one from other lanes of the same wave: a
+        // workgroup-scope fence orders it (later launches see it through the kernel
+        // boundary; a device-scope fence here would write back the XCD's whole L2)
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
 }
 template <typename T>
