@@ -102,7 +102,7 @@ struct TriLayout {
     double *ve64 = nullptr, *vi64 = nullptr, *d64 = nullptr;  // values; d = inverted diagonal (U)
     float *ve32 = nullptr, *vi32 = nullptr, *d32 = nullptr;
     unsigned *sync = nullptr;    // sweep time-out flag, spares
-    void *mid64 = nullptr, *mid32 = nullptr;  // lower layout: sweep intermediate vector
+    void *mid64 = nullptr, *mid32 = nullptr;  // coherent output vector of the sweep
     // FP16 copy: values scaled by 2^-e per row (e in sx16), FP32 inverted diagonal (U)
     void *ve16 = nullptr, *vi16 = nullptr;
     float *d16 = nullptr;

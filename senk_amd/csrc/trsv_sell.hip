@@ -265,7 +265,7 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
                                             const V *__restrict__ ve, const V *__restrict__ vi,
                                             const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx,
                                             const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-                                            unsigned *err) {
+                                            unsigned *err, T *copy = nullptr) {
     using U = typename Bits<T>::U;
     const int4 si = sinfo[slice];
     const LaneRow<BS> lr(si, blk_list);
@@ -383,8 +383,12 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
         }
     }
     if (lr.active) {
-        if (SWEEP) store_agent(y + lr.row, canonical(t));
-        else y[lr.row] = t;
+        if (SWEEP) {
+            store_agent(y + lr.row, canonical(t));
+            if (copy) copy[lr.row] = t;
+        } else {
+            y[lr.row] = t;
+        }
     }
 }
 
@@ -423,12 +427,12 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sweep(int nslices, const int4
                                                          const DiagT<V> *__restrict__ dinv,
                                                          const signed char *__restrict__ sx, const T *x, T *y,
                                                          const double *__restrict__ in_scale, T *reset,
-                                                         unsigned *err, const int *__restrict__ stop_flag) {
+                                                         unsigned *err, T *copy, const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) return;
     const int nw = (int)gridDim.x * kWavesPerBlock;
     for (int t = (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6); t < nslices; t += nw)
         solve_slice<T, V, BS, UPPER, true>(t, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, err);
+                                           reset, err, copy);
 }
 
 // The whole triangle in one launch of 16-wave workgroups, each solving 16 consecutive
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
     const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
     const V *__restrict__ ve, const V *__restrict__ vi, const DiagT<V> *__restrict__ dinv,
     const signed char *__restrict__ sx, const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-    unsigned *sync, const int *__restrict__ stop_flag) {
+    unsigned *sync, T *copy, const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             __hip_atomic_fetch_add(sync + kSweepTicket, gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -463,7 +467,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
     const int slice = (int)q * kTicketWaves + (int)(threadIdx.x >> 6);
     if (slice < nslices)
         solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, sync + kSweepErr);
+                                           reset, sync + kSweepErr, copy);
 }
 
 template <typename T>
@@ -552,7 +556,8 @@ int sweep_grid() {
 
 template <typename T, typename V, int BS, bool UPPER>
 int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
-                 const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, const int *stop) {
+                 const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
+                 const int *stop) {
     using W = typename RowWord<BS>::W;
     int grid = sweep_grid<T, V, BS, UPPER>();
     if (grid <= 0) return SENK_ERR_HIP;
@@ -563,7 +568,8 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
     const W *rw = (const W *)L.rw;
     unsigned *err = L.sync + kSweepErr;
     const double *sc = UPPER ? nullptr : in_scale;
-    void *args[] = {&nslices, &sinfo, &blk, &rw, &next, &scol, &ve, &vi, &dinv, &sx, &x, &y, &sc, &reset, &err, &stop};
+    void *args[] = {&nslices, &sinfo, &blk, &rw,  &next,  &scol, &ve,   &vi,  &dinv,
+                    &sx,      &x,     &y,   &sc, &reset, &err, &copy, &stop};
     SENK_HIP(hipLaunchCooperativeKernel((const void *)k_sptrsv_sweep<T, V, BS, UPPER>, dim3(grid), dim3(kBlock), args,
                                         0, c->stream));
     return SENK_OK;
@@ -571,13 +577,14 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
 
 template <typename T, typename V, int BS, bool UPPER>
 int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
-                  const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, const int *stop) {
+                  const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
+                  const int *stop) {
     using W = typename RowWord<BS>::W;
     const unsigned grid = (unsigned)((L.nslices + kTicketWaves - 1) / kTicketWaves);
     const unsigned base = L.ticket_base;
     k_sptrsv_ticket<T, V, BS, UPPER><<<grid, 64 * kTicketWaves, 0, c->stream>>>(
         L.nslices, base, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-        UPPER ? nullptr : in_scale, reset, L.sync, stop);
+        UPPER ? nullptr : in_scale, reset, L.sync, copy, stop);
     SENK_LAUNCH_CHECK();
     L.ticket_base = base + grid;
     return SENK_OK;
@@ -585,13 +592,15 @@ int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V 
 
 template <typename T, typename V, int BS>
 int solve_sweep(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
-                const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, const int *stop) {
+                const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
+                const int *stop) {
     if (L.mode == kTrsvTicket) {
-        if (L.upper) return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
-        return launch_ticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
+        if (L.upper)
+            return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, stop);
+        return launch_ticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, stop);
     }
-    if (L.upper) return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
-    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, stop);
+    if (L.upper) return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, stop);
+    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, stop);
 }
 
 template <typename T>
@@ -706,10 +715,11 @@ int tri_layout_fill(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, co
     if (!*ve) SENK_HIP(hipMalloc(ve, (size_t)(L->n_ext ? L->n_ext : 1) * sizeof(T)));
     if (!*vi) SENK_HIP(hipMalloc(vi, (size_t)(L->n_int ? L->n_int : 1) * sizeof(T)));
     if (L->upper && !*dv) SENK_HIP(hipMalloc(dv, (size_t)L->nslices * 64 * sizeof(T)));
-    // the lower layout owns the intermediate vector of the sweep (L output, U input),
-    // kept all-sentinel between applications
+    // each layout owns the coherent output vector of its sweep (L: the intermediate
+    // vector, U: the final rows before they are copied out), kept all-sentinel between
+    // applications
     void **mid = std::is_same<T, double>::value ? &L->mid64 : &L->mid32;
-    if (!L->upper && !*mid) {
+    if (!*mid) {
         SENK_HIP(hipMalloc(mid, (size_t)L->n * sizeof(T)));
         SENK_TRY(fill_sentinel<T>(c, L->n, (T *)*mid));
     }
@@ -730,7 +740,7 @@ int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr
     if (!L->vi16) SENK_HIP(hipMalloc(&L->vi16, (size_t)(L->n_int ? L->n_int : 1) * sizeof(__half)));
     if (!L->sx16) SENK_HIP(hipMalloc(&L->sx16, (size_t)L->nslices * 64));
     if (L->upper && !L->d16) SENK_HIP(hipMalloc(&L->d16, (size_t)L->nslices * 64 * sizeof(float)));
-    if (!L->upper && !L->mid32) {
+    if (!L->mid32) {
         SENK_HIP(hipMalloc(&L->mid32, (size_t)L->n * sizeof(float)));
         SENK_TRY(fill_sentinel<float>(c, L->n, (float *)L->mid32));
     }
@@ -751,16 +761,18 @@ int ilu_layout_apply(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule &
                      const signed char *lsx, const signed char *usx, const T *in, T *out, const double *in_scale,
                      const int *stop) {
     T *mid = (T *)(std::is_same<T, double>::value ? lt.mid64 : lt.mid32);
-    if (lt.mode != kTrsvLevels && mid && lt.sync && ut.sync) {
-        // L: in -> mid (every row of out becomes the sentinel for the U sweep);
-        // U: mid -> out (mid returns to the sentinel as its rows are read)
+    T *up = (T *)(std::is_same<T, double>::value ? ut.mid64 : ut.mid32);
+    if (lt.mode != kTrsvLevels && mid && up && lt.sync && ut.sync) {
+        // L: in -> mid, every row of up (the U sweep's coherent output) becomes the
+        // sentinel; U: mid -> up, mid returns to the sentinel as its rows are read, and
+        // each final row is also written to out with an ordinary (cached) store
         auto l = [&]() -> int {
-            SENK_BS_DISPATCH(lt.bsize, (solve_sweep<T, V, B>(c, ls, lt, lve, lvi, nullptr, lsx, in, mid, in_scale, out,
-                                                             stop)))
+            SENK_BS_DISPATCH(lt.bsize, (solve_sweep<T, V, B>(c, ls, lt, lve, lvi, nullptr, lsx, in, mid, in_scale, up,
+                                                             (T *)nullptr, stop)))
         };
         auto u = [&]() -> int {
             SENK_BS_DISPATCH(ut.bsize,
-                             (solve_sweep<T, V, B>(c, us, ut, uve, uvi, udv, usx, mid, out, nullptr, mid, stop)))
+                             (solve_sweep<T, V, B>(c, us, ut, uve, uvi, udv, usx, mid, up, nullptr, mid, out, stop)))
         };
         SENK_TRY(l());
         return u();
