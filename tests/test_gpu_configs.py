@@ -1,0 +1,105 @@
+"""The mixed-precision workloads beside the headline system, at test sizes.
+
+* Unstructured nonsymmetric matrix (lognormal row lengths, banded-random columns,
+  diagonal 1.1x the row's absolute sum): FP32 values, FP32 ILU(0) factors, FP32 SpMV and
+  an inner GMRES(50), FP64 iterative refinement to 1e-10; the error is reported against
+  an FP64 solve of the same system to 1e-14 (plain GMRES on the device, whose own
+  parity with the oracle drivers is tested elsewhere).
+* Block path: the 3-dof elasticity-like system (3x3-block 7-point stencil, lognormal
+  moduli) with ILUB (3x3 block fill) and with ILU(1) over ABMC colors, FP32 factors, FP64
+  refinement and the learned correction inside FGMRES(50): converges to 1e-10 in FP64,
+  the correction either helps or is switched off by the stagnation monitor, never
+  breaking the solve.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import senk_amd as S
+from oracle import oracle as O
+from senk_amd import gnn as G
+from tests.helpers import abmc_system, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(rptr, cind, val, bsize, seed):
+    N = len(rptr) - 1
+    xs = S.gen_uniform(seed, N)
+    b = np.zeros(N)
+    O.orc().or_spmv_csr_d(O.P(val), O.P(cind), O.P(rptr), O.P(xs), O.P(b), N)
+    sy = abmc_system(rptr, cind, val, bsize)
+    sy["b"], sy["xs"] = b[sy["LP"]].copy(), xs[sy["LP"]].copy()
+    return sy
+
+
+def _handles(ctx, sy, kind, p=0, bnl=0, bnw=0):
+    N = sy["N"]
+    A = C.c_void_p()
+    S.call("senk_mat_create", ctx.h, N, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]), 1 | 2, C.byref(A))
+    opts = S.senk_ilu_opts(kind, p, bnl, bnw, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32), 0)
+    M = C.c_void_p()
+    S.call("senk_prec_create_ilu", ctx.h, A, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]),
+           sy["cptr"].ctypes.data_as(C.c_void_p), sy["nc"], sy["bsize"], C.byref(opts), C.byref(M))
+    return A, M
+
+
+def _solve(ctx, A, M, b, N, **kw):
+    db, dx = ctx.array(b), ctx.zeros(N)
+    res = S.senk_result()
+    o = S.solve_opts(**kw)
+    S.call("senk_solve", ctx.h, A, M, db.p, dx.p, C.byref(o), C.byref(res), None, 0)
+    return res, dx.download()
+
+
+def test_unstructured_mixed_ir_gmres50(ctx):
+    rptr, cind, val = S.gen_unstructured(65536, 3)
+    sy = _system(rptr, cind, val, 16, 3)
+    N = sy["N"]
+    A, M = _handles(ctx, sy, S.SENK_ILU0)
+    try:
+        r64, x64 = _solve(ctx, A, M, sy["b"], N, solver=S.SENK_GMRES, m=50, eps=1e-14, max_iter=20000)
+        assert r64.converged and r64.true_relres <= 1e-13
+        res, x = _solve(ctx, A, M, sy["b"], N, solver=S.SENK_GMRES, m=50, eps=1e-10, max_iter=20000, mixed=1,
+                        inner_eps=1e-5, inner_max=500)
+        assert res.converged == 1 and res.status == 0
+        assert res.true_relres <= 1e-10
+        # error against the FP64 solve at the level the residual allows
+        assert relerr(x, x64) <= 1e-8, relerr(x, x64)
+        assert relerr(x, sy["xs"]) <= 1e-8
+    finally:
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
+
+
+@pytest.mark.parametrize("kind", ["ilub", "ilu1"])
+def test_block_path_mixed_learned_fgmres50(ctx, kind):
+    rptr, cind, val = S.gen_elasticity3(12, 1)
+    sy = _system(rptr, cind, val, 48, 1)
+    N = sy["N"]
+    if kind == "ilub":
+        A, M = _handles(ctx, sy, S.SENK_ILUB, bnl=3, bnw=3)
+    else:
+        A, M = _handles(ctx, sy, S.SENK_ILUP, p=1)
+    theta, _ = G.load()
+    net = G.Gnn(ctx, theta)
+    try:
+        net.bind(A)
+        net.attach(M)
+        r64, x64 = _solve(ctx, A, M, sy["b"], N, solver=S.SENK_GMRES, m=50, eps=1e-14, max_iter=20000)
+        assert r64.converged
+        plain, xp = _solve(ctx, A, M, sy["b"], N, solver=S.SENK_FGMRES, m=50, eps=1e-10, max_iter=20000, mixed=1,
+                           inner_eps=1e-5, inner_max=500)
+        learned, xl = _solve(ctx, A, M, sy["b"], N, solver=S.SENK_FGMRES, m=50, eps=1e-10, max_iter=20000,
+                             mixed=1, inner_eps=1e-5, inner_max=500, use_learned=1)
+        for res, x in ((plain, xp), (learned, xl)):
+            assert res.converged == 1 and res.status == 0
+            assert res.true_relres <= 1e-10
+            assert relerr(x, x64) <= 1e-7, relerr(x, x64)
+        assert plain.fell_back == 0 and learned.fell_back in (0, 1)
+    finally:
+        S.call("senk_prec_attach_gnn", M, None)
+        net.close()
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
