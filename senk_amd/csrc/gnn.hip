@@ -225,30 +225,42 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
         a[1] = bf16x4{(short)(u0.z & 0xFFFF), (short)(u0.z >> 16), (short)(u0.w & 0xFFFF), (short)(u0.w >> 16)};
         a[2] = bf16x4{(short)(u1.x & 0xFFFF), (short)(u1.x >> 16), (short)(u1.y & 0xFFFF), (short)(u1.y >> 16)};
         a[3] = bf16x4{(short)(u1.z & 0xFFFF), (short)(u1.z >> 16), (short)(u1.w & 0xFFFF), (short)(u1.w >> 16)};
-        // aggregation: four neighbours' loads in flight at a time
+        // aggregation: the columns and values of up to 8 neighbours are requested at
+        // once, then their features four at a time (fewer dependent round trips)
         const int e = rptr[node + 1];
-        for (int j0 = rptr[node]; j0 < e; j0 += 4) {
-            float wgt[4];
-            uint4 hv[4][2];
+        for (int j8 = rptr[node]; j8 < e; j8 += 8) {
+            int cjs[8];
+            float vjs[8];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (j0 + q < e) {
-                    const int cj = cind[j0 + q];
-                    wgt[q] = val[j0 + q] * s[cj];
-                    const uint4 *hj = reinterpret_cast<const uint4 *>(hin + (size_t)cj * W + 16 * half);
-                    hv[q][0] = hj[0];
-                    hv[q][1] = hj[1];
+            for (int q = 0; q < 8; q++)
+                if (j8 + q < e) {
+                    cjs[q] = cind[j8 + q];
+                    vjs[q] = val[j8 + q];
                 }
-            }
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (j0 + q < e) {
-                    const unsigned wd[8] = {hv[q][0].x, hv[q][0].y, hv[q][0].z, hv[q][0].w,
-                                            hv[q][1].x, hv[q][1].y, hv[q][1].z, hv[q][1].w};
+            for (int q4 = 0; q4 < 8; q4 += 4) {
+                float wgt[4];
+                uint4 hv[4][2];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        m[2 * u] += wgt[q] * bf2f((unsigned short)(wd[u] & 0xFFFF));
-                        m[2 * u + 1] += wgt[q] * bf2f((unsigned short)(wd[u] >> 16));
+                for (int q = 0; q < 4; q++) {
+                    if (j8 + q4 + q < e) {
+                        const int cj = cjs[q4 + q];
+                        wgt[q] = vjs[q4 + q] * s[cj];
+                        const uint4 *hj = reinterpret_cast<const uint4 *>(hin + (size_t)cj * W + 16 * half);
+                        hv[q][0] = hj[0];
+                        hv[q][1] = hj[1];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (j8 + q4 + q < e) {
+                        const unsigned wd[8] = {hv[q][0].x, hv[q][0].y, hv[q][0].z, hv[q][0].w,
+                                                hv[q][1].x, hv[q][1].y, hv[q][1].z, hv[q][1].w};
+#pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            m[2 * u] += wgt[q] * bf2f((unsigned short)(wd[u] & 0xFFFF));
+                            m[2 * u + 1] += wgt[q] * bf2f((unsigned short)(wd[u] >> 16));
+                        }
                     }
                 }
             }
