@@ -12,7 +12,9 @@ driver times only the solve).
                  outer iteration around FP32 inner Krylov solves with FP32 SpMV and FP32
                  ILU(0) factors (ms per solve, median of --steps)
   fp64_gmres30   the plain FP64 ILU(0)-GMRES(30) solve it is compared with (the
-                 reference's AbmcIluGmresm algorithm), at its fastest ABMC block size
+                 reference's AbmcIluGmresm algorithm, modified Gram-Schmidt), at its
+                 fastest ABMC block size; fp64_gmres30_cgs2 the same with two-pass
+                 classical Gram-Schmidt (reported beside it, not the baseline)
   learned_*      the same with the trained message-passing correction of the ILU
                  application (weights from senk_amd/data/gnn_vardiff.npz)
   plain_fp64_c2  the 27-point convection-diffusion 128^3 system (Pe 10, ABMC blocks of
@@ -324,6 +326,9 @@ def main():
     for tag, dev in (("b8", d8), ("b4", d4)):
         runs[f"fp64_gmres30_{tag}"] = timed(dev, K, W, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000)
         log(f"[bench] fp64_gmres30_{tag}: {json.dumps(runs[f'fp64_gmres30_{tag}'])}")
+    # the same GMRES(30) with two-pass classical Gram-Schmidt (not the reference's MGS)
+    runs["fp64_gmres30_cgs2_b8"] = timed(d8, K, W, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000,
+                                         ortho=S.SENK_CGS2)
     runs["fp64_bicgstab_b8"] = timed(d8, K, W, solver=S.SENK_BICGSTAB, m=30, eps=1e-10, max_iter=6000)
     for tag, dev in (("b8", d8), ("b4", d4)):
         runs[f"mixed_bicgstab_{tag}"] = timed(dev, K, W, solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)
