@@ -4,7 +4,7 @@
 // of 64 rows, one wave per slice, one lane per row; a slice therefore holds 64/BS whole
 // blocks.  Slices are numbered in solve order (L: levels ascending, U: descending).
 // Every row's entries are stored in the order the sequential solve consumes them
-// (senk_sptrsv.hpp: L ascending, U descending after the inverted diagonal), split in
+// (senk_sparse.hpp:133-167: L ascending, U descending after the inverted diagonal), split in
 // two groups:
 //   external entries  column outside the row's block, solved in an earlier level;
 //                     value + column, k-th entry of every lane stored together;
