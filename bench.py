@@ -314,9 +314,22 @@ def main():
         # one configuration alone (for a kernel trace of exactly that solve)
         cfgs = {"fp64_gmres30_b8": (d8, dict(solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000)),
                 "mixed_bicgstab_b4": (d4, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)),
-                "mixed_bicgstab_b8": (d8, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED))}
+                "mixed_bicgstab_b8": (d8, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)),
+                "learned_fgcr30_fp64_b8": (d8, dict(solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
+                                                    check_every=1, use_learned=1)),
+                "learned_mixed_fgcr_b8": (d8, dict(solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED))}
         dev, kw = cfgs[args.run]
+        net = None
+        if kw.get("use_learned"):
+            net, _ = load_learned(ctx)
+            if net is None:
+                raise SystemExit("no trained network for " + args.run)
+            net.bind(dev.A)
+            net.attach(dev.M)
         r = timed(dev, K, W, **kw)
+        if net is not None:
+            S.call("senk_prec_attach_gnn", dev.M, None)
+            net.close()
         print(json.dumps({"run": args.run, **r}), flush=True)
         d8.close()
         d4.close()

@@ -448,7 +448,9 @@ int senk_gnn_create(senk_ctx ctx, senk_gnn *out);   /* all parameters zero: no c
 int senk_gnn_destroy(senk_gnn g);
 int senk_gnn_set_params(senk_gnn g, const float *theta);   /* host arrays of SENK_GNN_PARAMS */
 int senk_gnn_get_params(senk_gnn g, float *theta);
-/* Bind the matrix the correction runs on (needs its FP32 values: senk_mat_create flag bit0). */
+/* Bind the matrix the correction runs on (needs its FP32 values: senk_mat_create flag bit0).
+ * Builds, on the host, the layers' tile order (breadth-first over tiles of 32 rows) and
+ * their tiled copy of the graph (8 bytes per entry, widths padded to multiples of 8). */
 int senk_gnn_bind(senk_gnn g, senk_mat A);
 /* dz = dz + correction(dr, dz) on the bound matrix (dz holds the ILU output on entry). */
 int senk_gnn_apply_f32(senk_gnn g, const float *dr, float *dz);

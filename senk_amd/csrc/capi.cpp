@@ -1441,8 +1441,6 @@ int senk_gnn_collect(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, co
 int senk_prec_attach_gnn(senk_prec M, senk_gnn g) {
     if (!M) return SENK_ERR_ARG;
     M->gnn = g;
-    // the layers take their node tiles by region of each color (per-XCD caches)
-    if (g) return senk_amd::gnn_set_tile_order(g, M->view.ls);
     return SENK_OK;
 }
 

@@ -9,9 +9,20 @@
 //   h3 = relu(h2 W3s + (Ahat h2) W3n + b3)
 //   z  = z0 + sigma * s * (h3 wo + bo)
 // The network is homogeneous (scaling r scales the correction) and with wo = bo = 0 it
-// is plain ILU.  Edge aggregation reads the matrix's own FP32 CSR arrays.  Activations
-// are stored in BF16 (64 bytes per node), the aggregated inputs of the matrix-core
-// layers are rounded to BF16 as their operands, and accumulation is FP32.
+// is plain ILU.  Activations are stored in BF16 (64 bytes per node), the aggregated
+// inputs of the matrix-core layers are rounded to BF16 as their operands, and
+// accumulation is FP32.
+//
+// The layers take the nodes in tiles of 32 consecutive matrix rows, and for inference
+// the tiles in a breadth-first (Cuthill-McKee) order of the graph between tiles: the
+// records of a node's neighbours then lie near its own and are gathered from the L2 of
+// the XCD that handles that band of tiles (a color ordering puts every neighbour in
+// another color, far away), while the inputs and the correction still move as whole
+// 128-byte lines of the matrix-ordered vectors.  Training keeps the matrix order.  A
+// tile's graph is stored slot-major (entry k of its 32 nodes side by side), each entry
+// the neighbour's position in the layers' order and the weight a_ij s_j; tile widths
+// are padded to multiples of 8 with entries (own position, 0) so that every load of the
+// aggregation is unconditional.
 //
 // Parameter vector (FP32, kParams values): W1s[2][32] W1n[2][32] b1[32] W2s[32][32]
 // W2n[32][32] b2[32] W3s[32][32] W3n[32][32] b3[32] wo[32] bo.
@@ -52,24 +63,29 @@ __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float
 __device__ __forceinline__ int fpos(int f) { return ((f >> 2) & 1) * 16 + (f >> 3) * 4 + (f & 3); }
 
 // ----------------------------------------------------------------- state
+// one entry of the layers' graph: neighbour position in the layers' node order and the
+// weight a_ij s_j
+struct __align__(8) GnnEdge {
+    int col;
+    float w;
+};
 struct GnnGraph {
     int N = 0;
-    const int *rptr = nullptr, *cind = nullptr;
+    const int *rptr = nullptr, *cind = nullptr;    // A in matrix order (training's products)
     const float *val = nullptr;
     const int *trptr = nullptr, *tcind = nullptr;  // A^T (== A's arrays when A is symmetric)
     const float *tval = nullptr;
-    float *s = nullptr;                            // owned
+    float *s = nullptr;                            // owned; |d|^-1/2 in the layers' order
     int *own_trptr = nullptr, *own_tcind = nullptr;
     float *own_tval = nullptr;
-    // order in which the layer kernels take their tiles of 32 nodes (null: index order):
-    // with a color ordering, the j-th eighth of every color comes in the j-th eighth of
-    // the grid, i.e. on XCD j, so that neighbours' features are found in that XCD's L2
-    int *torder = nullptr;                         // owned
+    int *tperm = nullptr;     // owned; layers' tile T = matrix rows 32 tperm[T] + [0, 32) (null: T)
+    int *tptr = nullptr;      // owned; first slot of each tile of 32 nodes (ntiles + 1, multiples of 8)
+    GnnEdge *ent = nullptr;   // owned; entry (tptr[t] + k) * 32 + l: k-th entry of node 32t + l
 };
 struct GnnWork {
     int N = 0;
     bool train = false;
-    float2 *xin = nullptr, *xs = nullptr;          // [s r, z0/s] and s*[s r, z0/s]
+    float2 *xin = nullptr;                         // [s r, z0/s] in the layers' order
     unsigned short *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *m2 = nullptr, *m3 = nullptr;
     float *chat = nullptr;
     float2 *x = nullptr, *m1 = nullptr;            // training: normalised inputs and Ahat x
@@ -122,18 +138,22 @@ __global__ void k_pack_params(const float *__restrict__ theta, float *theta_r, b
 }
 
 // ----------------------------------------------------------------- forward
-// xin = [s r, z0/s], xs = s * xin, partials of (s r)^2
+// matrix row of the layers' node u
+__device__ __forceinline__ long long gnn_row(const int *__restrict__ tperm, long long u) {
+    return tperm ? (long long)tperm[u >> 5] * 32 + (u & 31) : u;
+}
+// xin[u] = [s r, z0/s] at the matrix row of u; partials of (s r)^2
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gnn_in(int N, const T *__restrict__ r, const double *in_scale,
                                                    const T *__restrict__ z0, const float *__restrict__ s,
-                                                   float2 *xin, float2 *xs, double *partials) {
+                                                   const int *__restrict__ tperm, float2 *xin, double *partials) {
     const double sc = in_scale ? *in_scale : 1.0;
     double ss = 0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
-        const float si = s[i];
+    for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long long)gridDim.x * kBlock) {
+        const long long i = gnn_row(tperm, u);
+        const float si = s[u];
         const float rh = si * (float)((double)r[i] * sc), zh = (float)z0[i] / si;
-        xin[i] = make_float2(rh, zh);
-        xs[i] = make_float2(si * rh, si * zh);
+        xin[u] = make_float2(rh, zh);
         ss += (double)rh * (double)rh;
     }
     const double v[1] = {ss};
@@ -147,31 +167,48 @@ __global__ __launch_bounds__(kBlock) void k_gnn_sigma(const double *P, int B, in
         sc[1] = sg > 0 ? 1.0 / sg : 0.0;
     }
 }
-// layer 1, one thread per node
-__global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict__ rptr, const int *__restrict__ cind,
-                                                   const float *__restrict__ val, const float *__restrict__ s,
-                                                   const float2 *__restrict__ xin, const float2 *__restrict__ xs,
-                                                   const double *sc, const float *__restrict__ th,
-                                                   unsigned short *h1, float2 *xkeep, float2 *m1keep,
-                                                   const int *__restrict__ torder) {
-    const int tl = (xcd_block() * kBlock + (int)threadIdx.x) >> 5;
-    if (tl >= (N + 31) / 32) return;
-    const int i = (torder ? torder[tl] : tl) * 32 + (threadIdx.x & 31);
-    if (i >= N) return;
+// Orders this wave's LDS writes before its later LDS reads of other lanes' data (the
+// buffers below are per wave, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// layer 1, one thread per node (two tiles per wave)
+__global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict__ tptr,
+                                                   const GnnEdge *__restrict__ ent, const float *__restrict__ s,
+                                                   const float2 *__restrict__ xin, const double *sc,
+                                                   const float *__restrict__ th, unsigned short *h1, float2 *xkeep,
+                                                   float2 *m1keep) {
+    const int u = xcd_block() * kBlock + (int)threadIdx.x;
+    if (u >= N) return;
+    const int t = u >> 5;
     const float inv = (float)sc[1];
+    const int p0 = tptr[t], w = tptr[t + 1] - p0;
+    const GnnEdge *e = ent + (size_t)p0 * 32 + (u & 31);
     float a0 = 0, a1 = 0;
-    for (int j = rptr[i]; j < rptr[i + 1]; j++) {
-        const float2 v = xs[cind[j]];
-        a0 += val[j] * v.x;
-        a1 += val[j] * v.y;
+    // widths are multiples of 8: the entries of 8 neighbours are requested at once, then
+    // their inputs
+    for (int k0 = 0; k0 < w; k0 += 8) {
+        GnnEdge ev[8];
+        float2 xv[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) ev[q] = e[(size_t)(k0 + q) * 32];
+#pragma unroll
+        for (int q = 0; q < 8; q++) xv[q] = xin[ev[q].col];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            a0 += ev[q].w * xv[q].x;
+            a1 += ev[q].w * xv[q].y;
+        }
     }
-    const float f = s[i] * inv;
+    const float f = s[u] * inv;
     const float m0 = a0 * f, m1 = a1 * f;
-    const float2 xv = xin[i];
+    const float2 xv = xin[u];
     const float x0 = xv.x * inv, x1 = xv.y * inv;
     if (xkeep) {
-        xkeep[i] = make_float2(x0, x1);
-        m1keep[i] = make_float2(m0, m1);
+        xkeep[u] = make_float2(x0, x1);
+        m1keep[u] = make_float2(m0, m1);
     }
     unsigned short out[W];  // in stored order
 #pragma unroll
@@ -181,97 +218,95 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
         v += m0 * th[oW1n + o] + m1 * th[oW1n + W + o];
         out[((o >> 2) & 1) * 16 + (o >> 3) * 4 + (o & 3)] = f2bf(fmaxf(v, 0.f));
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(h1 + (size_t)i * W);
+    uint4 *dst = reinterpret_cast<uint4 *>(h1 + (size_t)u * W);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        uint4 u;
-        u.x = out[8 * q + 0] | ((unsigned)out[8 * q + 1] << 16);
-        u.y = out[8 * q + 2] | ((unsigned)out[8 * q + 3] << 16);
-        u.z = out[8 * q + 4] | ((unsigned)out[8 * q + 5] << 16);
-        u.w = out[8 * q + 6] | ((unsigned)out[8 * q + 7] << 16);
-        dst[q] = u;
+        uint4 o4;
+        o4.x = out[8 * q + 0] | ((unsigned)out[8 * q + 1] << 16);
+        o4.y = out[8 * q + 2] | ((unsigned)out[8 * q + 3] << 16);
+        o4.z = out[8 * q + 4] | ((unsigned)out[8 * q + 5] << 16);
+        o4.w = out[8 * q + 6] | ((unsigned)out[8 * q + 7] << 16);
+        dst[q] = o4;
     }
+}
+// four BF16 values from two packed words
+__device__ __forceinline__ bf16x4 bf16x4_of(uint2 v) {
+    return bf16x4{(short)(v.x & 0xFFFF), (short)(v.x >> 16), (short)(v.y & 0xFFFF), (short)(v.y >> 16)};
 }
 // Layers 2 and 3 on the matrix cores: one wave per tile of 32 nodes.  Lane l feeds node
 // n0 + l%32; its A operand for k-step t (t < 4: own features, t >= 4: aggregated) is
-// features 8t + 4*(l/32) + [0,4) of that node (stored contiguously, see fpos).  D element (row, col) sits in lane
-// col + 32*((row/4)%2), register (row%4) + 4*(row/8).
+// features 8t + 4*(l/32) + [0,4) of that node (stored contiguously, see fpos).  D element
+// (row, col) sits in lane col + 32*((row/4)%2), register (row%4) + 4*(row/8).  Lanes of
+// the last tile past N read node N-1 and the tile's padding entries and store nothing.
 template <bool LAST>
-__global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict__ rptr, const int *__restrict__ cind,
-                                                   const float *__restrict__ val, const float *__restrict__ s,
+__global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict__ tptr,
+                                                   const GnnEdge *__restrict__ ent, const float *__restrict__ s,
                                                    const unsigned short *__restrict__ hin, const bf16x4 *__restrict__ wb,
                                                    const float *__restrict__ bias, unsigned short *hout,
                                                    unsigned short *mkeep, const float *__restrict__ wo,
-                                                   const float *__restrict__ bo, float *chat,
-                                                   const int *__restrict__ torder) {
+                                                   const float *__restrict__ bo, float *chat) {
     __shared__ float red[LAST ? kWavesPerBlock : 1][32][33];
     __shared__ __align__(16) unsigned short tileh[kWavesPerBlock][32 * W];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ntiles = (N + 31) / 32;
-    const int tl = xcd_block() * kWavesPerBlock + wave;
-    const int n0 = (tl < ntiles ? (torder ? torder[tl] : tl) : ntiles) * 32;
+    const int tl = __builtin_amdgcn_readfirstlane(xcd_block() * kWavesPerBlock + wave);
+    if (tl >= ntiles) return;
+    const int n0 = tl * 32;
     const int half = lane >> 5, col = lane & 31;
     const int node = n0 + col;
-    const bool act = node < N;
+    const int nodec = min(node, N - 1);
+    // matrix-core B operands (the same for every tile) before the aggregation's loads
+    bf16x4 bop[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) bop[t] = wb[t * 64 + lane];
+    const float b = bias[col];
     bf16x4 a[8];
     float m[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) m[q] = 0;
-    if (act) {
-        // own features: 32 contiguous bytes (stored order), k-steps 0..3
-        const uint4 *hs = reinterpret_cast<const uint4 *>(hin + (size_t)node * W + 16 * half);
+    // own features: 32 contiguous bytes (stored order), k-steps 0..3
+    {
+        const uint4 *hs = reinterpret_cast<const uint4 *>(hin + (size_t)nodec * W + 16 * half);
         const uint4 u0 = hs[0], u1 = hs[1];
-        a[0] = bf16x4{(short)(u0.x & 0xFFFF), (short)(u0.x >> 16), (short)(u0.y & 0xFFFF), (short)(u0.y >> 16)};
-        a[1] = bf16x4{(short)(u0.z & 0xFFFF), (short)(u0.z >> 16), (short)(u0.w & 0xFFFF), (short)(u0.w >> 16)};
-        a[2] = bf16x4{(short)(u1.x & 0xFFFF), (short)(u1.x >> 16), (short)(u1.y & 0xFFFF), (short)(u1.y >> 16)};
-        a[3] = bf16x4{(short)(u1.z & 0xFFFF), (short)(u1.z >> 16), (short)(u1.w & 0xFFFF), (short)(u1.w >> 16)};
-        // aggregation: the columns and values of up to 8 neighbours are requested at
-        // once, then their features four at a time (fewer dependent round trips)
-        const int e = rptr[node + 1];
-        for (int j8 = rptr[node]; j8 < e; j8 += 8) {
-            int cjs[8];
-            float vjs[8];
+        a[0] = bf16x4_of(make_uint2(u0.x, u0.y));
+        a[1] = bf16x4_of(make_uint2(u0.z, u0.w));
+        a[2] = bf16x4_of(make_uint2(u1.x, u1.y));
+        a[3] = bf16x4_of(make_uint2(u1.z, u1.w));
+    }
+    // aggregation: the entries of 8 neighbours are requested at once, then their features
+    // four at a time (fewer dependent round trips)
+    const int p0 = tptr[tl], w = tptr[tl + 1] - p0;
+    const GnnEdge *e = ent + (size_t)p0 * 32 + col;
+    for (int k8 = 0; k8 < w; k8 += 8) {
+        GnnEdge ev[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-                if (j8 + q < e) {
-                    cjs[q] = cind[j8 + q];
-                    vjs[q] = val[j8 + q];
-                }
+        for (int q = 0; q < 8; q++) ev[q] = e[(size_t)(k8 + q) * 32];
 #pragma unroll
-            for (int q4 = 0; q4 < 8; q4 += 4) {
-                float wgt[4];
-                uint4 hv[4][2];
+        for (int q4 = 0; q4 < 8; q4 += 4) {
+            uint4 hv[4][2];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    if (j8 + q4 + q < e) {
-                        const int cj = cjs[q4 + q];
-                        wgt[q] = vjs[q4 + q] * s[cj];
-                        const uint4 *hj = reinterpret_cast<const uint4 *>(hin + (size_t)cj * W + 16 * half);
-                        hv[q][0] = hj[0];
-                        hv[q][1] = hj[1];
-                    }
-                }
+            for (int q = 0; q < 4; q++) {
+                const uint4 *hj = reinterpret_cast<const uint4 *>(hin + (size_t)ev[q4 + q].col * W + 16 * half);
+                hv[q][0] = hj[0];
+                hv[q][1] = hj[1];
+            }
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    if (j8 + q4 + q < e) {
-                        const unsigned wd[8] = {hv[q][0].x, hv[q][0].y, hv[q][0].z, hv[q][0].w,
-                                                hv[q][1].x, hv[q][1].y, hv[q][1].z, hv[q][1].w};
+            for (int q = 0; q < 4; q++) {
+                const float wgt = ev[q4 + q].w;
+                const unsigned wd[8] = {hv[q][0].x, hv[q][0].y, hv[q][0].z, hv[q][0].w,
+                                        hv[q][1].x, hv[q][1].y, hv[q][1].z, hv[q][1].w};
 #pragma unroll
-                        for (int u = 0; u < 8; u++) {
-                            m[2 * u] += wgt[q] * bf2f((unsigned short)(wd[u] & 0xFFFF));
-                            m[2 * u + 1] += wgt[q] * bf2f((unsigned short)(wd[u] >> 16));
-                        }
-                    }
+                for (int v = 0; v < 8; v++) {
+                    m[2 * v] += wgt * bf2f((unsigned short)(wd[v] & 0xFFFF));
+                    m[2 * v + 1] += wgt * bf2f((unsigned short)(wd[v] >> 16));
                 }
             }
         }
-        const float si = s[node];
-#pragma unroll
-        for (int q = 0; q < 16; q++) m[q] *= si;
-    } else {
-#pragma unroll
-        for (int t = 0; t < 4; t++) a[t] = bf16x4{0, 0, 0, 0};
     }
+    const float si = s[nodec];
+#pragma unroll
+    for (int q = 0; q < 16; q++) m[q] *= si;
+    const bool act = node < N;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         bf16x4 v;
@@ -284,8 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = 0;
 #pragma unroll
-    for (int t = 0; t < 8; t++) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[t], wb[t * 64 + lane], acc, 0, 0, 0);
-    const float b = bias[col];
+    for (int t = 0; t < 8; t++) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[t], bop[t], acc, 0, 0, 0);
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int row = 8 * (r >> 2) + 4 * half + (r & 3);
@@ -293,10 +327,10 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
         if (hout) tileh[wave][row * W + fpos(col)] = hv;
         if (LAST) red[wave][row][col] = bf2f(hv) * wo[col];
     }
+    wave_lds_sync();
     if (hout) {
         // the tile's 32 node records (64 bytes each, stored order) leave the LDS as two
         // 16-byte stores per lane, consecutive lanes writing consecutive bytes
-        __syncthreads();
         const int row = lane >> 1, hh = lane & 1;
         if (n0 + row < N) {
             const uint4 *src = reinterpret_cast<const uint4 *>(&tileh[wave][row * W + 16 * hh]);
@@ -306,7 +340,6 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
         }
     }
     if (LAST) {
-        __syncthreads();
         if (lane < 32 && n0 + lane < N) {
             float sum = 0;
 #pragma unroll 8
@@ -315,13 +348,15 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
         }
     }
 }
-// z += sigma * s * chat
+// z at the matrix row of u += sigma * s[u] * chat[u]
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gnn_out(int N, const float *__restrict__ s, const float *__restrict__ chat,
-                                                    const double *sc, T *z) {
+__global__ __launch_bounds__(kBlock) void k_gnn_out(int N, const float *__restrict__ s, const int *__restrict__ tperm,
+                                                    const float *__restrict__ chat, const double *sc, T *z) {
     const float sg = (float)sc[0];
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
-        z[i] = (T)((double)z[i] + (double)(sg * s[i] * chat[i]));
+    for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long long)gridDim.x * kBlock) {
+        const long long i = gnn_row(tperm, u);
+        z[i] = (T)((double)z[i] + (double)(sg * s[u] * chat[u]));
+    }
 }
 
 // ----------------------------------------------------------------- backward
@@ -510,8 +545,6 @@ __global__ void k_zero_d(int n, double *p) {
 
 inline int node_grid(int N) { return (N + kBlock - 1) / kBlock; }
 inline int tile_grid(int N) { return (N + 32 * kWavesPerBlock - 1) / (32 * kWavesPerBlock); }
-// one thread per node of whole 32-node tiles
-inline int tile_threads_grid(int N) { return (int)(((long long)(N + 31) / 32 * 32 + kBlock - 1) / kBlock); }
 
 template <typename T>
 int dalloc(T **p, size_t n) {
@@ -522,12 +555,13 @@ int dalloc(T **p, size_t n) {
 
 // ----------------------------------------------------------------- host helpers
 void gnn_graph_free(GnnGraph *g) {
-    for (void *p : {(void *)g->s, (void *)g->own_trptr, (void *)g->own_tcind, (void *)g->own_tval, (void *)g->torder})
+    for (void *p : {(void *)g->s, (void *)g->own_trptr, (void *)g->own_tcind, (void *)g->own_tval, (void *)g->tperm,
+                    (void *)g->tptr, (void *)g->ent})
         if (p) (void)hipFree(p);
     *g = GnnGraph{};
 }
 void gnn_work_free(GnnWork *w) {
-    for (void *p : {(void *)w->xin, (void *)w->xs, (void *)w->h1, (void *)w->h2, (void *)w->h3, (void *)w->m2,
+    for (void *p : {(void *)w->xin, (void *)w->h1, (void *)w->h2, (void *)w->h3, (void *)w->m2,
                     (void *)w->m3, (void *)w->chat, (void *)w->x, (void *)w->m1, (void *)w->G, (void *)w->G2,
                     (void *)w->P, (void *)w->Q, (void *)w->dchat, (void *)w->z, (void *)w->e, (void *)w->t,
                     (void *)w->tmp, (void *)w->sc, (void *)w->gpart})
@@ -541,7 +575,6 @@ int gnn_work_ensure(GnnWork *w, int N, bool train) {
     w->train = train;
     const size_t nf = (size_t)N * gnn::W;
     SENK_TRY(dalloc(&w->xin, N));
-    SENK_TRY(dalloc(&w->xs, N));
     SENK_TRY(dalloc(&w->h1, nf));
     SENK_TRY(dalloc(&w->h2, nf));
     SENK_TRY(dalloc(&w->chat, N));
@@ -566,10 +599,66 @@ int gnn_work_ensure(GnnWork *w, int N, bool train) {
     return SENK_OK;
 }
 
-// s = |diag|^-1/2 on the device; the transpose (training) on the host, or A's own arrays
+// Breadth-first (Cuthill-McKee) order of the tiles of 32 consecutive rows, two tiles
+// being neighbours when a row of one has a column in the other; one component after the
+// other, each from a pseudo-peripheral tile (two George-Liu steps: restart from a tile
+// with fewest entries in the last level while that lengthens the search).  A partial
+// last tile (N not a multiple of 32) stays last.
+static std::vector<int> tile_bfs_order(int N, const std::vector<int> &rp, const std::vector<int> &ci) {
+    const int nt = (N + 31) / 32;
+    const int last = (N % 32) ? nt - 1 : -1;
+    std::vector<int> order, q, dist((size_t)nt, -1);
+    std::vector<char> done((size_t)nt, 0);
+    order.reserve((size_t)nt);
+    q.reserve((size_t)nt);
+    auto entries = [&](int t) { return rp[std::min(N, 32 * t + 32)] - rp[32 * t]; };
+    if (last >= 0) done[last] = 1;
+    // search from root over tiles not yet ordered; returns the depth, q the visit order
+    auto search = [&](int root) {
+        q.clear();
+        q.push_back(root);
+        dist[root] = 0;
+        for (size_t h = 0; h < q.size(); h++) {
+            const int t = q[h];
+            for (int j = rp[32 * t]; j < rp[std::min(N, 32 * t + 32)]; j++) {
+                const int c = ci[j] >> 5;
+                if (!done[c] && dist[c] < 0) {
+                    dist[c] = dist[t] + 1;
+                    q.push_back(c);
+                }
+            }
+        }
+        return dist[q.back()];
+    };
+    for (int s0 = 0; s0 < nt; s0++) {
+        if (done[s0]) continue;
+        int root = s0, depth = -1;
+        for (int step = 0; step < 2; step++) {
+            const int d = search(root);
+            int best = q.back();
+            for (size_t h = q.size(); h-- > 0 && dist[q[h]] == d;)
+                if (entries(q[h]) < entries(best)) best = q[h];
+            for (int v : q) dist[v] = -1;
+            if (d <= depth) break;
+            depth = d;
+            root = best;
+        }
+        search(root);
+        for (int v : q) {
+            dist[v] = -1;
+            done[v] = 1;
+            order.push_back(v);
+        }
+    }
+    if (last >= 0) order.push_back(last);
+    return order;
+}
+
+// s = |diag|^-1/2, the layers' tile order (breadth-first when `reorder`, else the matrix
+// order) and their tiled graph; the transpose (training) on the host, or A's own arrays
 // when A is symmetric.
 int gnn_graph_build(Ctx *c, int N, const int *drptr, const int *dcind, const float *dval, bool transpose,
-                    GnnGraph *g) {
+                    bool reorder, GnnGraph *g) {
     gnn_graph_free(g);
     g->N = N;
     g->rptr = drptr;
@@ -587,6 +676,11 @@ int gnn_graph_build(Ctx *c, int N, const int *drptr, const int *dcind, const flo
     SENK_HIP(hipMemcpyAsync(ci.data(), dcind, sizeof(int) * nnz, hipMemcpyDeviceToHost, c->stream));
     SENK_HIP(hipMemcpyAsync(va.data(), dval, sizeof(float) * nnz, hipMemcpyDeviceToHost, c->stream));
     SENK_HIP(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < nnz; j++)
+        if (ci[j] < 0 || ci[j] >= N) {
+            record_error("learned preconditioner: column index out of range");
+            return SENK_ERR_ARG;
+        }
     std::vector<float> s((size_t)N);
     for (int i = 0; i < N; i++) {
         double d = 0;
@@ -598,8 +692,47 @@ int gnn_graph_build(Ctx *c, int N, const int *drptr, const int *dcind, const flo
         }
         s[i] = (float)(1.0 / std::sqrt(std::fabs(d)));
     }
+    // layers' tile T holds matrix rows 32 P[T] + [0, 32); tile of row i: iP[i / 32]
+    const int nt = (N + 31) / 32;
+    std::vector<int> P, iP;
+    if (reorder) {
+        P = tile_bfs_order(N, rp, ci);
+        iP.assign((size_t)nt, 0);
+        for (int t = 0; t < nt; t++) iP[P[t]] = t;
+    }
+    auto row = [&](int u) { return reorder ? 32 * P[u >> 5] + (u & 31) : u; };
+    auto pos = [&](int i) { return reorder ? 32 * iP[i >> 5] + (i & 31) : i; };
+    std::vector<int> tb((size_t)nt + 1, 0);
+    for (int t = 0; t < nt; t++) {
+        int w = 0;
+        for (int u = 32 * t; u < std::min(N, 32 * t + 32); u++) w = std::max(w, rp[row(u) + 1] - rp[row(u)]);
+        tb[t + 1] = tb[t] + (w + 7) / 8 * 8;
+    }
+    // padding: the node itself (lanes past N: the tile's first node) with weight 0
+    std::vector<GnnEdge> en((size_t)tb[nt] * 32);
+    std::vector<float> sn((size_t)N);
+    for (int t = 0; t < nt; t++)
+        for (int l = 0; l < 32; l++) {
+            const int u = 32 * t + l, self = u < N ? u : 32 * t;
+            int k = 0;
+            if (u < N) {
+                const int i = row(u);
+                sn[u] = s[i];
+                for (int j = rp[i]; j < rp[i + 1]; j++, k++)
+                    en[((size_t)tb[t] + k) * 32 + l] = GnnEdge{pos(ci[j]), va[j] * s[ci[j]]};
+            }
+            for (; k < tb[t + 1] - tb[t]; k++) en[((size_t)tb[t] + k) * 32 + l] = GnnEdge{self, 0.f};
+        }
     SENK_TRY(dalloc(&g->s, N));
-    SENK_HIP(hipMemcpy(g->s, s.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    SENK_TRY(dalloc(&g->tptr, (size_t)nt + 1));
+    SENK_TRY(dalloc(&g->ent, en.size()));
+    SENK_HIP(hipMemcpy(g->s, sn.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    SENK_HIP(hipMemcpy(g->tptr, tb.data(), sizeof(int) * (nt + 1), hipMemcpyHostToDevice));
+    SENK_HIP(hipMemcpy(g->ent, en.data(), sizeof(GnnEdge) * en.size(), hipMemcpyHostToDevice));
+    if (reorder) {
+        SENK_TRY(dalloc(&g->tperm, nt));
+        SENK_HIP(hipMemcpy(g->tperm, P.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+    }
     if (!transpose) return SENK_OK;
     // transpose by counting sort; keep A's arrays when the result is identical
     std::vector<int> tp((size_t)N + 1, 0), tc((size_t)std::max(nnz, 1));
@@ -642,24 +775,22 @@ int gnn_forward(Ctx *c, senk_gnn_s *net, const GnnGraph &g, GnnWork &w, const T 
     const int N = g.N;
     const int B = red_grid(N);
     double *P = red_slot(c, kSlot);
-    k_gnn_in<T><<<B, kBlock, 0, c->stream>>>(N, r, in_scale, z, g.s, w.xin, w.xs, P);
+    k_gnn_in<T><<<B, kBlock, 0, c->stream>>>(N, r, in_scale, z, g.s, g.tperm, w.xin, P);
     SENK_LAUNCH_CHECK();
     k_gnn_sigma<<<1, kBlock, 0, c->stream>>>(P, B, N, w.sc);
     SENK_LAUNCH_CHECK();
     const float *th = net->theta_r;
-    k_gnn_l1<<<tile_threads_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.xin, w.xs, w.sc, th,
-                                                               w.h1, keep ? w.x : nullptr, keep ? w.m1 : nullptr,
-                                                               g.torder);
+    k_gnn_l1<<<node_grid(N), kBlock, 0, c->stream>>>(N, g.tptr, g.ent, g.s, w.xin, w.sc, th, w.h1,
+                                                       keep ? w.x : nullptr, keep ? w.m1 : nullptr);
     SENK_LAUNCH_CHECK();
-    k_gnn_mc<false><<<tile_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.h1, net->wb, th + ob2,
-                                                             w.h2, keep ? w.m2 : nullptr, nullptr, nullptr, nullptr,
-                                                             g.torder);
+    k_gnn_mc<false><<<tile_grid(N), kBlock, 0, c->stream>>>(N, g.tptr, g.ent, g.s, w.h1, net->wb, th + ob2, w.h2,
+                                                             keep ? w.m2 : nullptr, nullptr, nullptr, nullptr);
     SENK_LAUNCH_CHECK();
-    k_gnn_mc<true><<<tile_grid(N), kBlock, 0, c->stream>>>(N, g.rptr, g.cind, g.val, g.s, w.h2, net->wb + 512,
-                                                            th + ob3, keep ? w.h3 : nullptr, keep ? w.m3 : nullptr,
-                                                            th + owo, th + obo, w.chat, g.torder);
+    k_gnn_mc<true><<<tile_grid(N), kBlock, 0, c->stream>>>(N, g.tptr, g.ent, g.s, w.h2, net->wb + 512, th + ob3,
+                                                            keep ? w.h3 : nullptr, keep ? w.m3 : nullptr, th + owo,
+                                                            th + obo, w.chat);
     SENK_LAUNCH_CHECK();
-    k_gnn_out<T><<<node_grid(N), kBlock, 0, c->stream>>>(N, g.s, w.chat, w.sc, z);
+    k_gnn_out<T><<<node_grid(N), kBlock, 0, c->stream>>>(N, g.s, g.tperm, w.chat, w.sc, z);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
@@ -775,39 +906,10 @@ int gnn_get_params(senk_gnn_s *net, float *h) {
 }
 int gnn_bind(senk_gnn_s *net, int N, const int *drptr, const int *dcind, const float *dval) {
     if (!dval) return SENK_ERR_ARG;  // the network aggregates over the FP32 values
-    SENK_TRY(gnn_graph_build(net->c, N, drptr, dcind, dval, false, &net->g));
+    SENK_TRY(gnn_graph_build(net->c, N, drptr, dcind, dval, false, true, &net->g));
     return gnn_work_ensure(&net->w, N, net->w.train);
 }
 LearnedPrec *gnn_learned(senk_gnn_s *net) { return &net->lp; }
-// Tile order of the bound graph from a color schedule (levels = colors of consecutive
-// blocks); any other schedule, or a graph of another size, keeps the index order.
-int gnn_set_tile_order(senk_gnn_s *net, const Schedule &sch) {
-    GnnGraph &g = net->g;
-    (void)hipFree(g.torder);
-    g.torder = nullptr;
-    const int N = g.N;
-    if (N <= 0 || sch.dblk || sch.nlev <= 0 || sch.bsize <= 0 || (long long)sch.lev_ptr[sch.nlev] * sch.bsize != N)
-        return SENK_OK;
-    const int nt = (N + 31) / 32;
-    std::vector<int> region(nt);
-    for (int t = 0; t < nt; t++) {
-        const int blk = 32 * t / sch.bsize;
-        int l = (int)(std::upper_bound(sch.lev_ptr.begin(), sch.lev_ptr.begin() + sch.nlev + 1, blk) -
-                      sch.lev_ptr.begin()) - 1;
-        l = std::max(0, std::min(sch.nlev - 1, l));
-        const long long b0 = sch.lev_ptr[l], b1 = sch.lev_ptr[l + 1];
-        region[t] = b1 > b0 ? (int)std::min(7LL, 8 * (blk - b0) / (b1 - b0)) : 0;
-    }
-    std::vector<int> order;
-    order.reserve(nt);
-    for (int e = 0; e < 8; e++)
-        for (int t = 0; t < nt; t++)
-            if (region[t] == e) order.push_back(t);
-    SENK_TRY(dalloc(&g.torder, (size_t)nt));
-    SENK_HIP(hipMemcpyAsync(g.torder, order.data(), sizeof(int) * nt, hipMemcpyHostToDevice, net->c->stream));
-    SENK_HIP(hipStreamSynchronize(net->c->stream));
-    return SENK_OK;
-}
 int gnn_apply_f32(senk_gnn_s *net, const float *r, float *z) {
     if (net->g.N <= 0) return SENK_ERR_ARG;
     return gnn_forward<float>(net->c, net, net->g, net->w, r, nullptr, z, false);
@@ -819,7 +921,7 @@ int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const fl
     Ctx *c = net->c;
     if (!A.v32 || nvec <= 0) return SENK_ERR_ARG;
     GnnGraph g;
-    int st = gnn_graph_build(c, A.N, A.rptr, A.cind, A.v32, true, &g);
+    int st = gnn_graph_build(c, A.N, A.rptr, A.cind, A.v32, true, false, &g);
     if (st == SENK_OK) st = gnn_work_ensure(&net->w, A.N, true);
     if (st == SENK_OK) {
         k_zero_d<<<(gnn::kParams + 255) / 256, 256, 0, c->stream>>>(gnn::kParams, net->grad);
@@ -848,7 +950,7 @@ int gnn_train(senk_gnn_s *net, int nmat, const MatView *As, const PrecView *Ms, 
     int st = SENK_OK, maxN = 0;
     for (int k = 0; k < nmat && st == SENK_OK; k++) {
         if (!As[k].v32 || !Ms[k].l32) st = SENK_ERR_ARG;
-        else st = gnn_graph_build(c, As[k].N, As[k].rptr, As[k].cind, As[k].v32, true, &gs[k]);
+        else st = gnn_graph_build(c, As[k].N, As[k].rptr, As[k].cind, As[k].v32, true, false, &gs[k]);
         maxN = std::max(maxN, As[k].N);
     }
     if (st == SENK_OK) st = gnn_work_ensure(&net->w, maxN, true);

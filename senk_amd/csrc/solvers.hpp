@@ -89,7 +89,6 @@ void gnn_destroy(senk_gnn_s *net);
 int gnn_set_params(senk_gnn_s *net, const float *host_theta);
 int gnn_get_params(senk_gnn_s *net, float *host_theta);
 int gnn_bind(senk_gnn_s *net, int N, const int *drptr, const int *dcind, const float *dval);
-int gnn_set_tile_order(senk_gnn_s *net, const Schedule &sch);
 LearnedPrec *gnn_learned(senk_gnn_s *net);
 int gnn_apply_f32(senk_gnn_s *net, const float *r, float *z);
 int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const float *R, int nvec, double *loss,
