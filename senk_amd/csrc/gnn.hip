@@ -174,15 +174,19 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-// layer 1, one thread per node (two tiles per wave)
+// layer 1, one thread per node (two tiles per wave); a wave's 64 node records go out
+// through the LDS so that consecutive lanes store consecutive 16 bytes
 __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict__ tptr,
                                                    const GnnEdge *__restrict__ ent, const float *__restrict__ s,
                                                    const float2 *__restrict__ xin, const double *sc,
                                                    const float *__restrict__ th, unsigned short *h1, float2 *xkeep,
                                                    float2 *m1keep) {
-    const int u = xcd_block() * kBlock + (int)threadIdx.x;
-    if (u >= N) return;
-    const int t = u >> 5;
+    __shared__ __align__(16) uint4 stage[kWavesPerBlock][64 * 4];
+    const int u0 = xcd_block() * kBlock + (int)(threadIdx.x & ~63);
+    const int u = u0 + (int)(threadIdx.x & 63);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (u0 >= N) return;
+    const int t = min(u, N - 1) >> 5;
     const float inv = (float)sc[1];
     const int p0 = tptr[t], w = tptr[t + 1] - p0;
     const GnnEdge *e = ent + (size_t)p0 * 32 + (u & 31);
@@ -202,11 +206,12 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
             a1 += ev[q].w * xv[q].y;
         }
     }
-    const float f = s[u] * inv;
+    const int uc = min(u, N - 1);
+    const float f = s[uc] * inv;
     const float m0 = a0 * f, m1 = a1 * f;
-    const float2 xv = xin[u];
+    const float2 xv = xin[uc];
     const float x0 = xv.x * inv, x1 = xv.y * inv;
-    if (xkeep) {
+    if (xkeep && u < N) {
         xkeep[u] = make_float2(x0, x1);
         m1keep[u] = make_float2(m0, m1);
     }
@@ -218,7 +223,6 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
         v += m0 * th[oW1n + o] + m1 * th[oW1n + W + o];
         out[((o >> 2) & 1) * 16 + (o >> 3) * 4 + (o & 3)] = f2bf(fmaxf(v, 0.f));
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(h1 + (size_t)u * W);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         uint4 o4;
@@ -226,8 +230,14 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
         o4.y = out[8 * q + 2] | ((unsigned)out[8 * q + 3] << 16);
         o4.z = out[8 * q + 4] | ((unsigned)out[8 * q + 5] << 16);
         o4.w = out[8 * q + 6] | ((unsigned)out[8 * q + 7] << 16);
-        dst[q] = o4;
+        stage[wave][lane * 4 + q] = o4;
     }
+    wave_lds_sync();
+    uint4 *dst = reinterpret_cast<uint4 *>(h1 + (size_t)u0 * W);
+    const int nq = min(64, N - u0) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (k * 64 + lane < nq) dst[k * 64 + lane] = stage[wave][k * 64 + lane];
 }
 // four BF16 values from two packed words
 __device__ __forceinline__ bf16x4 bf16x4_of(uint2 v) {
@@ -238,6 +248,10 @@ __device__ __forceinline__ bf16x4 bf16x4_of(uint2 v) {
 // features 8t + 4*(l/32) + [0,4) of that node (stored contiguously, see fpos).  D element
 // (row, col) sits in lane col + 32*((row/4)%2), register (row%4) + 4*(row/8).  Lanes of
 // the last tile past N read node N-1 and the tile's padding entries and store nothing.
+// The aggregation runs in another lane layout (lanes 2k, 2k+1: node k) and its sums are
+// permuted across lanes into this one: two lanes reading the halves of one record touch
+// one cache line per instruction where the matrix-core layout (lanes k, k+32) touches it
+// twice; measured, 159 against 170 us for layer 2 at 128^3.
 template <bool LAST>
 __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict__ tptr,
                                                    const GnnEdge *__restrict__ ent, const float *__restrict__ s,
@@ -273,10 +287,12 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
         a[2] = bf16x4_of(make_uint2(u1.x, u1.y));
         a[3] = bf16x4_of(make_uint2(u1.z, u1.w));
     }
-    // aggregation: the entries of 8 neighbours are requested at once, then their features
-    // four at a time (fewer dependent round trips)
+    // aggregation in the gather layout: lanes 2k and 2k+1 take the two halves of node k's
+    // neighbour records (one cache line per record and instruction), then the sums move to
+    // the matrix-core layout (lane l: node l%32, half l/32)
+    const int gnode = lane >> 1, ghalf = lane & 1;
     const int p0 = tptr[tl], w = tptr[tl + 1] - p0;
-    const GnnEdge *e = ent + (size_t)p0 * 32 + col;
+    const GnnEdge *e = ent + (size_t)p0 * 32 + gnode;
     for (int k8 = 0; k8 < w; k8 += 8) {
         GnnEdge ev[8];
 #pragma unroll
@@ -286,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
             uint4 hv[4][2];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const uint4 *hj = reinterpret_cast<const uint4 *>(hin + (size_t)ev[q4 + q].col * W + 16 * half);
+                const uint4 *hj = reinterpret_cast<const uint4 *>(hin + (size_t)ev[q4 + q].col * W + 16 * ghalf);
                 hv[q][0] = hj[0];
                 hv[q][1] = hj[1];
             }
@@ -302,6 +318,11 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
                 }
             }
         }
+    }
+    {
+        const int src = (2 * col + half) * 4;
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(m[q])));
     }
     const float si = s[nodec];
 #pragma unroll
