@@ -25,6 +25,7 @@ driver times only the solve).
                  the full FP64 solve of the same system and factors
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--kernels-only]
+       [--sweep-modes] [--learned-only] [--run NAME]
 """
 from __future__ import annotations
 
@@ -152,14 +153,15 @@ def kernel_rates(ctx, dev, reps=50, sweep_modes=False):
            "colors": dev.sy["nc"], "bsize": dev.sy["bsize"]}
     if not sweep_modes:
         return out
-    # triangular-solve launch modes: one cooperative sweep per triangle, one ticketed
-    # sweep per triangle, then the default one launch per level
+    # triangular-solve launch modes: 1 one cooperative sweep per triangle, 2 (the default)
+    # one ticketed sweep per triangle, 0 one launch per level; the default is restored after
     for mode in (1, 2, 0):
         S.call("senk_prec_set_trsv_mode", dev.M, mode)
         out[f"ilu_apply_f64_us_mode{mode}"] = time_kernel(
             ctx, lambda: S.call("senk_prec_apply_f64", dev.M, dx.p, dy.p), 20) * 1e3
         out[f"ilu_apply_f32_us_mode{mode}"] = time_kernel(
             ctx, lambda: S.call("senk_prec_apply_f32", dev.M, dx32.p, dy32.p), 20) * 1e3
+    S.call("senk_prec_set_trsv_mode", dev.M, 2)
     S.call("senk_prec_check", dev.M)
     return out
 
@@ -289,6 +291,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--kernels-only", action="store_true", help="kernel rates and the copy calibration only")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="time the learned correction's application alone (FP32, 10 applications)")
     ap.add_argument("--run", default=None, help="time only the solve configuration of this name (profiling)")
     ap.add_argument("--sweep-modes", action="store_true",
                     help="also time the ILU application with one cooperative launch per triangle")
@@ -301,6 +305,21 @@ def main():
     log(f"[bench] building the {n}^3 variable-coefficient diffusion system (ABMC blocks of 8 and 4)")
     sy8 = build_system_kind("vardiff", n, 8)
     d8 = Device(ctx, sy8)
+    if args.learned_only:
+        # one layer pass per application: its kernels at this size for a trace or counters
+        net, _ = load_learned(ctx)
+        if net is None:
+            raise SystemExit("no trained network")
+        net.bind(d8.A)
+        rng = np.random.default_rng(0)
+        dr = ctx.array(rng.uniform(-1, 1, d8.N).astype(np.float32))
+        dz = ctx.zeros(d8.N, np.float32)
+        S.call("senk_prec_apply_f32", d8.M, dr.p, dz.p)
+        us = 1e3 * time_kernel(ctx, lambda: S.call("senk_gnn_apply_f32", net.h, dr.p, dz.p), 10)
+        print(json.dumps({"learned_apply_f32_us": us, "n": d8.N}), flush=True)
+        net.close()
+        d8.close()
+        return
     kern = kernel_rates(ctx, d8, sweep_modes=args.sweep_modes)
     cal = copy_rate(ctx)
     log(f"[bench] kernels: {json.dumps(kern)} copy: {json.dumps(cal)}")

@@ -50,11 +50,13 @@ constexpr int kGradBlocks = 256;  // workgroups of the weight-gradient reduction
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// BF16 with round-to-nearest-even (finite inputs).
-__device__ __forceinline__ unsigned short f2bf(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
+// BF16 with round-to-nearest-even: the conversion instruction of gfx950
+// (v_cvt_pk_bf16_f32), two values per instruction in bf16x2.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+__device__ __forceinline__ unsigned bf16x2(float lo, float hi) {
+    const bf16x2_t v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(unsigned, v);
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 // Stored position of feature f (= 8t + 4h + i) inside a node's 32 BF16 features:
@@ -215,21 +217,21 @@ __global__ __launch_bounds__(kBlock) void k_gnn_l1(int N, const int *__restrict_
         xkeep[u] = make_float2(x0, x1);
         m1keep[u] = make_float2(m0, m1);
     }
-    unsigned short out[W];  // in stored order
+    float out[W];  // in stored order
 #pragma unroll
     for (int o = 0; o < W; o++) {
         float v = th[ob1 + o];
         v += x0 * th[oW1s + o] + x1 * th[oW1s + W + o];
         v += m0 * th[oW1n + o] + m1 * th[oW1n + W + o];
-        out[((o >> 2) & 1) * 16 + (o >> 3) * 4 + (o & 3)] = f2bf(fmaxf(v, 0.f));
+        out[((o >> 2) & 1) * 16 + (o >> 3) * 4 + (o & 3)] = fmaxf(v, 0.f);
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         uint4 o4;
-        o4.x = out[8 * q + 0] | ((unsigned)out[8 * q + 1] << 16);
-        o4.y = out[8 * q + 2] | ((unsigned)out[8 * q + 3] << 16);
-        o4.z = out[8 * q + 4] | ((unsigned)out[8 * q + 5] << 16);
-        o4.w = out[8 * q + 6] | ((unsigned)out[8 * q + 7] << 16);
+        o4.x = bf16x2(out[8 * q + 0], out[8 * q + 1]);
+        o4.y = bf16x2(out[8 * q + 2], out[8 * q + 3]);
+        o4.z = bf16x2(out[8 * q + 4], out[8 * q + 5]);
+        o4.w = bf16x2(out[8 * q + 6], out[8 * q + 7]);
         stage[wave][lane * 4 + q] = o4;
     }
     wave_lds_sync();
@@ -330,9 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_mc(int N, const int *__restrict_
     const bool act = node < N;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        bf16x4 v;
-#pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = (short)f2bf(m[4 * t + i]);
+        const bf16x4 v = bf16x4_of(make_uint2(bf16x2(m[4 * t], m[4 * t + 1]), bf16x2(m[4 * t + 2], m[4 * t + 3])));
         a[4 + t] = v;
         if (mkeep && act) *reinterpret_cast<bf16x4 *>(mkeep + (size_t)node * W + 16 * half + 4 * t) = v;
     }
