@@ -277,8 +277,9 @@ def load_learned(ctx):
 
 
 MIXED = dict(m=30, eps=1e-10, max_iter=6000, mixed=2, inner_eps=1e-3, outer_m=20)
-# inner BiCGStab steps per outer step of the nested solve (a sweep over 40..120 steps,
-# tolerances 1e-2..1e-4 and block sizes 4 / 8 on this system gave 46-93 ms; this is its best)
+# inner BiCGStab steps per outer step of the nested solve (sweeps over 40..120 steps,
+# tolerances 3e-3..1e-4 and block sizes 4 / 8 / 16 on this system, with the split inner
+# form and without: this setting is within noise of the best of either)
 INNER_BICGSTAB = 60
 
 

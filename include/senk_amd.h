@@ -402,6 +402,16 @@ int senk_prec_apply_f16(senk_prec M, const float *dx, float *dy);
  * likewise.  Results are bit-identical in every mode.  Applications of one
  * preconditioner must not overlap (one stream). */
 int senk_prec_set_trsv_mode(senk_prec M, int mode);
+/* Split form of the mixed forms' inner BiCGStab (no reference counterpart; the
+ * reference's BiCGStab is right-preconditioned, senk_bicgstab.hpp:129-161).  When the
+ * factorization is an ILU(0) that changes no off-diagonal entry (no row i has entries k
+ * < i and j != i with (k, j) in the pattern: a triangle-free graph such as a 5- or
+ * 7-point stencil in any ordering), A = L D + U - E with D = diag(U), E = 2D - diag(A),
+ * and the inner FP32 BiCGStab runs on L^-1 A U^-1 y = L^-1 r, each product being one U
+ * and one L solve plus two vector passes and no SpMV (Eisenstat's trick); d = U^-1 y.
+ * On by default where it applies; enable = 1 where it does not is SENK_ERR_UNSUPPORTED.
+ * The FP64 solvers and the other inner solvers are not affected. */
+int senk_prec_set_split(senk_prec M, int enable);
 /* Synchronise and report SENK_ERR_TIMEOUT if a single-launch solve since the last check
  * gave up a wait (never expected; senk_solve checks by itself). */
 int senk_prec_check(senk_prec M);

@@ -35,6 +35,7 @@ struct senk_prec_s {
     senk_ctx ctx = nullptr;
     senk_amd::PrecView view;
     senk_amd::TriLayout lt, ut;
+    float *dilu_d = nullptr, *dilu_e = nullptr;  // D-ILU vectors (owned; null: not D-ILU)
     senk_gnn gnn = nullptr;  // attached learned correction (used with use_learned)
     int lnnz = 0, unnz = 0;
     std::vector<void *> owned;
@@ -94,6 +95,26 @@ int nnz_of(senk_ctx ctx, const int *drptr, int N, int *nnz) {
 }
 double avg_row(int nnz, int N) { return N > 0 ? (double)nnz / N : 0.0; }
 
+// True when ILU(0) on this pattern changes no off-diagonal entry: no row i has an entry
+// k < i whose row holds an entry j > k, j != i, that row i holds as well (the IKJ
+// elimination of senk_matrix.hpp updates exactly those), so the factors' off-diagonal
+// entries are A's own and only the pivots change (a triangle-free graph, e.g. a 5- or
+// 7-point stencil in any ordering).
+bool dilu_pattern(int N, const int *rp, const int *ci) {
+    std::vector<int> stamp((size_t)N, -1);
+    for (int i = 0; i < N; i++) {
+        for (int a = rp[i]; a < rp[i + 1]; a++) stamp[ci[a]] = i;
+        for (int a = rp[i]; a < rp[i + 1]; a++) {
+            const int k = ci[a];
+            if (k >= i) continue;
+            for (int b = rp[k]; b < rp[k + 1]; b++) {
+                const int j = ci[b];
+                if (j > k && j != i && stamp[j] == i) return false;
+            }
+        }
+    }
+    return true;
+}
 Schedule colors(const int *cptr, int cnum, int bsize) {
     Schedule s;
     s.bsize = bsize;
@@ -1239,6 +1260,17 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
             }
             v.lt = &M->lt;
             v.ut = &M->ut;
+            // ILU(0) with no elimination update of an off-diagonal entry: the split form of
+            // the mixed forms' inner BiCGStab
+            if (v.l32 && A->v64 && fnnz == nnzA && !(opts->precision & (1 << SENK_FACTOR_F16)) &&
+                dilu_pattern(N, fr, fc)) {
+                if ((st = dalloc(&M->dilu_d, (size_t)N))) break;
+                if ((st = dalloc(&M->dilu_e, (size_t)N))) break;
+                M->owned.insert(M->owned.end(), {M->dilu_d, M->dilu_e});
+                if ((st = launch_dilu_diag(c, N, ur, u64, A->rptr, A->cind, A->v64, M->dilu_d, M->dilu_e))) break;
+                v.dilu_d = M->dilu_d;
+                v.dilu_e = M->dilu_e;
+            }
         } else {
             tri_layout_free(&M->lt);
             tri_layout_free(&M->ut);
@@ -1291,6 +1323,13 @@ int senk_prec_set_trsv_mode(senk_prec M, int mode) {
     SENK_HIP(hipStreamSynchronize(M->ctx->c.stream));
     M->lt.mode = mode;
     M->ut.mode = mode;
+    return SENK_OK;
+}
+int senk_prec_set_split(senk_prec M, int enable) {
+    if (!M) return SENK_ERR_ARG;
+    if (enable && !M->dilu_d) return SENK_ERR_UNSUPPORTED;
+    M->view.dilu_d = enable ? M->dilu_d : nullptr;
+    M->view.dilu_e = enable ? M->dilu_e : nullptr;
     return SENK_OK;
 }
 int senk_prec_check(senk_prec M) {

@@ -122,6 +122,12 @@ template <typename T>
 int launch_ilu_layout(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule &us, TriLayout &ut,
                       const T *in, T *out, const double *in_scale, const int *stop, bool f16 = false);
 int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr, const int *cind, const double *val);
+// out = self^-1 in for one triangle alone (self L or U, other the opposite triangle of
+// the same factorization); in may equal out.  Calls on one factorization must alternate
+// L, U, L, ... (an ILU application counts as L then U).
+template <typename T>
+int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
+                      const int *stop);
 
 // ----------------------------------------------------------------- launchers (kernels.hip)
 template <typename T>
@@ -147,6 +153,9 @@ template <typename T>
 int launch_ilu0(Ctx *c, const Schedule &s, int N, const int *rptr, const int *cind, const int *dpos, T *val,
                 int *derr);
 int launch_find_diag(Ctx *c, int N, const int *rptr, const int *cind, int *dpos);
+// d = diag(U) (from U's inverted diagonal, first in each row), e = 2 d - diag(A), FP32
+int launch_dilu_diag(Ctx *c, int N, const int *urptr, const double *uval, const int *arptr, const int *acind,
+                     const double *aval, float *d, float *e);
 template <typename T>
 int launch_split_ldu(Ctx *c, int N, const int *rptr, const int *cind, const T *val, const int *lrptr,
                      T *lval, int *lcind, const int *urptr, T *uval, int *ucind);

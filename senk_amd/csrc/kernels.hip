@@ -768,6 +768,29 @@ int launch_find_diag(Ctx *c, int N, const int *rptr, const int *cind, int *dpos)
     return SENK_OK;
 }
 
+// D-ILU vectors: d = diag(U) from U's inverted diagonal (first entry of each row),
+// e = 2 d - diag(A), both rounded to FP32
+__global__ __launch_bounds__(kBlock) void k_dilu_diag(int N, const int *__restrict__ urptr,
+                                                      const double *__restrict__ uval, const int *__restrict__ arptr,
+                                                      const int *__restrict__ acind, const double *__restrict__ aval,
+                                                      float *d, float *e) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const double di = 1.0 / uval[urptr[i]];
+    double a = 0;
+    for (int j = arptr[i]; j < arptr[i + 1]; j++)
+        if (acind[j] == i) a = aval[j];
+    d[i] = (float)di;
+    e[i] = (float)(2.0 * di - a);
+}
+int launch_dilu_diag(Ctx *c, int N, const int *urptr, const double *uval, const int *arptr, const int *acind,
+                     const double *aval, float *d, float *e) {
+    if (N <= 0) return SENK_OK;
+    k_dilu_diag<<<(N + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(N, urptr, uval, arptr, acind, aval, d, e);
+    SENK_LAUNCH_CHECK();
+    return SENK_OK;
+}
+
 // Split "L-DU" with inverted diagonal (senk_matrix.hpp:257-333) on the device; the
 // output row pointers come from the host pattern split.
 template <typename T>

@@ -1102,11 +1102,145 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(const double *P, int B, do
     state[1] = 0;
     state[2] = 0;
 }
+// Split form for a D-ILU factorization (Eisenstat's trick).  With M = L U (L unit lower,
+// U upper with diagonal D) and A = L D + U - E, E = 2 D - diag(A):
+//   L^-1 A U^-1 v = D t + L^-1 (v - E t),  t = U^-1 v,
+// one U solve, one L solve and two vector passes, no product with A.
+// w = v - E t
+__global__ __launch_bounds__(kBlock) void k_dilu_pre(int N, const float *__restrict__ v, const float *__restrict__ e,
+                                                     const float *__restrict__ t, float *w, const int *stop) {
+    if (*stop) return;
+    SENK_FOR4(i0, N) {
+        float a[4], b[4], ev[4];
+        get4(v, i0, N, a);
+        get4(e, i0, N, ev);
+        get4(t, i0, N, b);
+#pragma unroll
+        for (int u = 0; u < 4; u++) a[u] = a[u] - ev[u] * b[u];
+        put4(w, i0, N, a);
+    }
+}
+// out = D t + y
+__global__ __launch_bounds__(kBlock) void k_dilu_post(int N, const float *__restrict__ d, const float *__restrict__ t,
+                                                      const float *__restrict__ y, float *out, const int *stop) {
+    if (*stop) return;
+    SENK_FOR4(i0, N) {
+        float dv[4], tv[4], yv[4];
+        get4(d, i0, N, dv);
+        get4(t, i0, N, tv);
+        get4(y, i0, N, yv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) yv[u] = dv[u] * tv[u] + yv[u];
+        put4(out, i0, N, yv);
+    }
+}
 }  // namespace
+
+namespace {
+int read_scalar(Ctx *c, const double *d, double *h);
+}
+
+// v = L^-1 A U^-1 in with the layouts (t, w scratch; w may be v)
+static int dilu_product(Ctx *c, const PrecView &M, const float *in, float *v, float *t, float *w, const int *stop) {
+    const int N = M.N;
+    SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, in, t, stop));
+    k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, M.dilu_e, t, w, stop);
+    SENK_LAUNCH_CHECK();
+    SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, w, w, stop));
+    k_dilu_post<<<ew_grid(N), kBlock, 0, c->stream>>>(N, M.dilu_d, t, w, v, stop);
+    SENK_LAUNCH_CHECK();
+    return SENK_OK;
+}
+
+// BiCGStab on L^-1 A U^-1 y = L^-1 b from y = 0 (the BiCGStab steps of
+// senk_bicgstab.hpp:129-161 with the operator above and no further preconditioner),
+// x = U^-1 y.  x must be zero on entry; the residual it stops on is L^-1 (b - A x),
+// relative to ||L^-1 b||.  Every triangle alternates L, U as the layouts require:
+// L (start), (U, L) per product, U (end).
+static int bicgstab_split(Ctx *c, const MatView &A, const PrecView &M, const float *b, float *x,
+                          const SolveParams &p, senk_result *res, double *hist, int hist_cap) {
+    const int N = A.N;
+    const int hcap = p.max_iter + 1;
+    size_t need = 10 * bytes_of<float>(N) + bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(16);
+    void *wsp;
+    SENK_TRY(ctx_work(c, need, &wsp));
+    Carver cv{(char *)wsp, 0, need};
+    float *r = cv.take<float>(N), *rs = cv.take<float>(N), *pp = cv.take<float>(N), *v = cv.take<float>(N);
+    float *s = cv.take<float>(N), *tv = cv.take<float>(N), *y = cv.take<float>(N), *t = cv.take<float>(N);
+    float *w = cv.take<float>(N);
+    double *dhist = cv.take<double>(hcap + 1);
+    int *state = cv.take<int>(8);
+    double *sc = cv.take<double>(16);
+    Poller poll;
+    SENK_TRY(poll.init(c, state));
+    SolveTimer tm;
+    SENK_TRY(tm.start(c));
+    const int B = red_grid(N);
+    // the state words come from the shared arena: clear the stop flag before the sweeps
+    k_state_reset<<<1, 64, 0, c->stream>>>(state);
+    SENK_LAUNCH_CHECK();
+    // r = L^-1 b; rstr = r; p = r; y = 0; rr = (r, rstr)
+    SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, b, r, state));
+    SENK_TRY(launch_copy<float>(c, N, r, rs));
+    SENK_TRY(launch_copy<float>(c, N, r, pp));
+    SENK_HIP(hipMemsetAsync(y, 0, sizeof(float) * N, c->stream));
+    k_bicg_alpha<float><<<B, kBlock, 0, c->stream>>>(N, r, rs, red_slot(c, kSlotA), state);
+    SENK_LAUNCH_CHECK();
+    k_bicg_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotA), B, sc, state);
+    SENK_LAUNCH_CHECK();
+    double rr0 = 0;
+    SENK_TRY(read_scalar(c, sc + 2, &rr0));
+    const double nrm_c = std::sqrt(rr0), tol = p.eps * nrm_c;
+    poll.reset();
+    bool stop = !(nrm_c > 0);
+    for (int i = 0; i < p.max_iter && !stop; i++) {
+        const int par = i & 1;
+        SENK_TRY(dilu_product(c, M, pp, v, t, w, state));
+        k_bicg_alpha<float><<<B, kBlock, 0, c->stream>>>(N, v, rs, red_slot(c, kSlotA), state);
+        SENK_LAUNCH_CHECK();
+        k_bicg_s<float><<<B, kBlock, 0, c->stream>>>(N, v, r, red_slot(c, kSlotA), B, sc + 2 + par, sc + 0, s, state);
+        SENK_LAUNCH_CHECK();
+        SENK_TRY(dilu_product(c, M, s, tv, t, w, state));
+        k_bicg_omega<float><<<B, kBlock, 0, c->stream>>>(N, tv, s, red_slot(c, kSlotB), state);
+        SENK_LAUNCH_CHECK();
+        k_bicg_update<float><<<B, kBlock, 0, c->stream>>>(N, pp, s, tv, s, rs, y, r, sc, red_slot(c, kSlotB), B,
+                                                          red_slot(c, kSlotC), state);
+        SENK_LAUNCH_CHECK();
+        k_bicg_p<float><<<B, kBlock, 0, c->stream>>>(N, v, r, sc, par, red_slot(c, kSlotC), B, tol, nrm_c, i, state,
+                                                     dhist, hcap, pp);
+        SENK_LAUNCH_CHECK();
+        SENK_TRY(poll.mark());
+        SENK_TRY(poll.poll(kLag, &stop));
+    }
+    // x = U^-1 y (after the stop flag is set: no flag for this sweep)
+    SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, y, x, nullptr));
+    int hstate[4] = {1, 0, 0, 0};
+    if (nrm_c > 0) SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+    float ms = 0;
+    SENK_TRY(tm.stop(c, &ms));
+    const int converged = hstate[0] && !hstate[2];
+    const int iters = hstate[0] ? hstate[1] : p.max_iter;
+    std::vector<double> hh((size_t)std::max(iters, 1), 0.0);
+    if (iters > 0)
+        SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * std::min(iters, hcap), hipMemcpyDeviceToHost));
+    if (hist)
+        for (int k = 0; k < std::min(iters, hist_cap); k++) hist[k] = hh[k];
+    res->iters = iters;
+    res->converged = converged;
+    res->restarts = 0;
+    res->implicit_relres = iters > 0 ? hh[iters - 1] : 0;
+    res->t_solve_ms = ms;
+    res->status = hstate[2] ? SENK_ERR_BREAKDOWN : SENK_OK;
+    return SENK_OK;
+}
 
 template <typename T>
 static int bicgstab(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
                     senk_result *res, double *hist, int hist_cap) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (p.split && M.dilu_d && M.dilu_e && M.lt && M.ut && !M.f16)
+            return bicgstab_split(c, A, M, b, x, p, res, hist, hist_cap);
+    }
     const int N = A.N;
     const int hcap = p.max_iter + 1;
     size_t need = 10 * bytes_of<T>(N) + bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(16);
@@ -1317,6 +1451,7 @@ SolveParams inner_params(const senk_solve_opts &o) {
     ip.check_every = 1;
     ip.max_iter = o.inner_max;
     ip.outer = (o.inner_max + o.m - 1) / o.m;
+    ip.split = 1;  // both callers start the inner solve from d = 0
     return ip;
 }
 

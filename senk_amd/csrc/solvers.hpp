@@ -48,6 +48,11 @@ struct PrecView {
     const int *blrptr = nullptr, *blcind = nullptr, *burptr = nullptr, *bucind = nullptr;
     const double *bl64 = nullptr, *bu64 = nullptr;
     const float *bl32 = nullptr, *bu32 = nullptr;
+    // D-ILU (an ILU(0) whose off-diagonal factor entries are A's own: A = L D + U - E with
+    // D = diag(U), E = 2 D - diag(A)): D and E in FP32, matrix order, for the split form
+    // of the mixed forms' inner BiCGStab; null when the factorization is not D-ILU or the
+    // split form is switched off
+    const float *dilu_d = nullptr, *dilu_e = nullptr;
     // learned correction added to the ILU application (flexible solvers only)
     LearnedPrec *learned = nullptr;
     int *learned_off = nullptr;  // device flag: 1 = correction disabled by the monitor
@@ -69,6 +74,7 @@ struct SolveParams {
     int check_every = 0;
     int verbose = 0;
     int unprec_direct = 0;  // GMRES with no preconditioner: x += sum y_k v_k (senk_gmres.hpp:80-82)
+    int split = 0;          // BiCGStab from x = 0: the split form when M has D-ILU vectors
 };
 
 // x (device) is the initial guess and the output.  hist (host) optional.

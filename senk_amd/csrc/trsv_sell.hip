@@ -812,6 +812,32 @@ int launch_ilu_layout(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule 
     if (!lve || !lvi || !uve || !uvi || !udv) return SENK_ERR_ARG;
     return ilu_layout_apply<T, T>(c, ls, lt, us, ut, lve, lvi, uve, uvi, udv, nullptr, nullptr, in, out, in_scale, stop);
 }
+// One triangle on its own, in = out allowed.  The single-launch modes keep their
+// invariant as long as the triangles of one factorization alternate (L, U, L, ...):
+// each sweep writes its own coherent vector, which the other triangle's last sweep
+// left all-sentinel, and returns the other triangle's vector to the sentinel.
+template <typename T>
+int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
+                      const int *stop) {
+    if (self.nslices == 0) return SENK_OK;
+    if (self.bsize != s.bsize || self.n != other.n || self.upper == other.upper) return SENK_ERR_ARG;
+    const bool f64 = std::is_same<T, double>::value;
+    const T *ve = (const T *)(f64 ? (void *)self.ve64 : (void *)self.ve32);
+    const T *vi = (const T *)(f64 ? (void *)self.vi64 : (void *)self.vi32);
+    const T *dv = self.upper ? (const T *)(f64 ? (void *)self.d64 : (void *)self.d32) : nullptr;
+    if (!ve || !vi || (self.upper && !dv)) return SENK_ERR_ARG;
+    T *mine = (T *)(f64 ? self.mid64 : self.mid32);
+    T *theirs = (T *)(f64 ? other.mid64 : other.mid32);
+    if (self.mode != kTrsvLevels && mine && theirs && self.sync) {
+        SENK_BS_DISPATCH(self.bsize,
+                         (solve_sweep<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, mine, nullptr, theirs, out, stop)))
+    }
+    SENK_BS_DISPATCH(self.bsize, (solve_levels<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, out, nullptr, stop)))
+}
+template int launch_tri_single<double>(Ctx *, const Schedule &, TriLayout &, TriLayout &, const double *, double *,
+                                       const int *);
+template int launch_tri_single<float>(Ctx *, const Schedule &, TriLayout &, TriLayout &, const float *, float *,
+                                      const int *);
 template int launch_ilu_layout<double>(Ctx *, const Schedule &, TriLayout &, const Schedule &, TriLayout &,
                                        const double *, double *, const double *, const int *, bool);
 template int launch_ilu_layout<float>(Ctx *, const Schedule &, TriLayout &, const Schedule &, TriLayout &,
