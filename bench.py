@@ -19,8 +19,10 @@ driver times only the solve).
                  application (weights from senk_amd/data/gnn_vardiff.npz)
   plain_fp64_c2  the 27-point convection-diffusion 128^3 system (Pe 10, ABMC blocks of
                  16): FP64 ILU(0)-GMRES(30) and ILU(0)-BiCGStab, and SpMV rates
-  roofline       FP32 SELL-C-sigma SpMV of the headline solve: algorithmic bytes / average
-                 launch time (HIP events on the library stream) against 8 TB/s
+  roofline       FP32 ILU(0) application of the headline solve (ticketed L and U sweeps, the
+                 dominant work of an inner step): algorithmic bytes / average application
+                 time (HIP events on the library stream) against 8 TB/s; roofline_spmv the
+                 same for the FP32 SELL-C-sigma SpMV
   cpu_baseline   the reference's own OpenMP AbmcIluGmresm (oracle/_ref) on the host cores,
                  the full FP64 solve of the same system and factors
 
@@ -174,10 +176,11 @@ def copy_rate(ctx, nbytes=1 << 30, reps=10):
     return {"copy_bytes": 2 * n * 8, "copy_us": t * 1e3, "copy_GBps": 2 * n * 8 / t / 1e6}
 
 
-def traffic_from_profile(path=ROOT / "profiles" / "spmv_sell_f32_vardiff128.md"):
-    """HBM bytes per SpMV launch from the committed counter summary: FETCH_SIZE and
-    WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for a device copy
-    of known size."""
+def traffic_from_profile(name):
+    """HBM bytes per launch (per application for the ILU sweeps) from a committed counter
+    summary under profiles/: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what
+    the same counters read for a device copy of known size."""
+    path = ROOT / "profiles" / name
     try:
         vals = {}
         for line in path.read_text().splitlines():
@@ -419,10 +422,17 @@ def main():
             "kernels": c2["kernels"], **{k: {q: v[q] for q in keep if q in v} for k, v in c2.items() if k != "kernels"}},
         "setup_s": {"abmc": sy8["t_abmc"], "reorder": sy8["t_reorder"], "upload": d8.t_upload,
                     "device_ilu0_split_layout": d8.t_ilu},
-        "roofline": {"kernel": "spmv_sellcs_f32 (vardiff7 128^3)", "bound": "hbm",
-                     "achieved": round(kern["spmv_sell_f32_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(kern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile()},
+        # the dominant kernels of the headline solve: the FP32 ILU(0) application (ticketed L
+        # sweep + U sweep; with the split inner form they are most of an inner step), and the
+        # FP32 SELL SpMV beside it
+        "roofline": {"kernel": "ilu0_apply_f32: ticketed L + U sweeps (vardiff7 128^3, ABMC blocks of 8)",
+                     "bound": "hbm", "achieved": round(kern["ilu_apply_f32_GBps"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(kern["ilu_apply_f32_GBps"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile("ilu_apply_f32_vardiff128.md")},
+        "roofline_spmv": {"kernel": "spmv_sellcs_f32 (vardiff7 128^3)", "bound": "hbm",
+                          "achieved": round(kern["spmv_sell_f32_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(kern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4),
+                          "traffic": traffic_from_profile("spmv_sell_f32_vardiff128.md")},
         "cpu_baseline": cpu,
     }
     d8.close()
