@@ -1120,18 +1120,37 @@ __global__ __launch_bounds__(kBlock) void k_dilu_pre(int N, const float *__restr
         put4(w, i0, N, a);
     }
 }
-// out = D t + y
+// out = D t + y, with the partial sums of the BiCGStab step's dot products of out:
+// K = 1 (out, rs) (as k_bicg_alpha), K = 2 (out, s) and (out, out) (as k_bicg_omega); the
+// same grid and loop give the same partials as those kernels
+template <int K>
 __global__ __launch_bounds__(kBlock) void k_dilu_post(int N, const float *__restrict__ d, const float *__restrict__ t,
-                                                      const float *__restrict__ y, float *out, const int *stop) {
+                                                      const float *__restrict__ y, float *out,
+                                                      const float *__restrict__ other, double *Pout,
+                                                      const int *stop) {
     if (*stop) return;
+    double a0 = 0, a1 = 0;
     SENK_FOR4(i0, N) {
-        float dv[4], tv[4], yv[4];
+        float dv[4], tv[4], yv[4], ov[4];
         get4(d, i0, N, dv);
         get4(t, i0, N, tv);
         get4(y, i0, N, yv);
 #pragma unroll
         for (int u = 0; u < 4; u++) yv[u] = dv[u] * tv[u] + yv[u];
         put4(out, i0, N, yv);
+        get4(other, i0, N, ov);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a0 += (double)yv[u] * (double)ov[u];
+            if (K == 2) a1 += (double)yv[u] * (double)yv[u];
+        }
+    }
+    if (K == 1) {
+        const double v[1] = {a0};
+        store_partials<1>(v, Pout);
+    } else {
+        const double v[2] = {a0, a1};
+        store_partials<2>(v, Pout);
     }
 }
 }  // namespace
@@ -1140,14 +1159,17 @@ namespace {
 int read_scalar(Ctx *c, const double *d, double *h);
 }
 
-// v = L^-1 A U^-1 in with the layouts (t, w scratch; w may be v)
-static int dilu_product(Ctx *c, const PrecView &M, const float *in, float *v, float *t, float *w, const int *stop) {
+// v = L^-1 A U^-1 in with the layouts (t, w scratch; w may be v), and the partials of
+// (v, other) (K = 1) or of (v, other), (v, v) (K = 2) over red_grid(N) workgroups
+template <int K>
+static int dilu_product(Ctx *c, const PrecView &M, const float *in, float *v, float *t, float *w,
+                        const float *other, double *Pout, const int *stop) {
     const int N = M.N;
     SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, in, t, stop));
     k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, M.dilu_e, t, w, stop);
     SENK_LAUNCH_CHECK();
     SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, w, w, stop));
-    k_dilu_post<<<ew_grid(N), kBlock, 0, c->stream>>>(N, M.dilu_d, t, w, v, stop);
+    k_dilu_post<K><<<red_grid(N), kBlock, 0, c->stream>>>(N, M.dilu_d, t, w, v, other, Pout, stop);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
@@ -1195,14 +1217,10 @@ static int bicgstab_split(Ctx *c, const MatView &A, const PrecView &M, const flo
     bool stop = !(nrm_c > 0);
     for (int i = 0; i < p.max_iter && !stop; i++) {
         const int par = i & 1;
-        SENK_TRY(dilu_product(c, M, pp, v, t, w, state));
-        k_bicg_alpha<float><<<B, kBlock, 0, c->stream>>>(N, v, rs, red_slot(c, kSlotA), state);
-        SENK_LAUNCH_CHECK();
+        SENK_TRY(dilu_product<1>(c, M, pp, v, t, w, rs, red_slot(c, kSlotA), state));
         k_bicg_s<float><<<B, kBlock, 0, c->stream>>>(N, v, r, red_slot(c, kSlotA), B, sc + 2 + par, sc + 0, s, state);
         SENK_LAUNCH_CHECK();
-        SENK_TRY(dilu_product(c, M, s, tv, t, w, state));
-        k_bicg_omega<float><<<B, kBlock, 0, c->stream>>>(N, tv, s, red_slot(c, kSlotB), state);
-        SENK_LAUNCH_CHECK();
+        SENK_TRY(dilu_product<2>(c, M, s, tv, t, w, s, red_slot(c, kSlotB), state));
         k_bicg_update<float><<<B, kBlock, 0, c->stream>>>(N, pp, s, tv, s, rs, y, r, sc, red_slot(c, kSlotB), B,
                                                           red_slot(c, kSlotC), state);
         SENK_LAUNCH_CHECK();
