@@ -125,9 +125,11 @@ int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr
 // out = self^-1 in for one triangle alone (self L or U, other the opposite triangle of
 // the same factorization); in may equal out.  Calls on one factorization must alternate
 // L, U, L, ... (an ILU application counts as L then U).
+// With pre_w (single-launch modes only; *fused says whether it was done) the sweep also
+// writes pre_w = in - pre_e * out.
 template <typename T>
 int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
-                      const int *stop);
+                      const int *stop, const T *pre_e = nullptr, T *pre_w = nullptr, bool *fused = nullptr);
 
 // ----------------------------------------------------------------- launchers (kernels.hip)
 template <typename T>

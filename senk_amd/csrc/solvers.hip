@@ -1159,15 +1159,19 @@ namespace {
 int read_scalar(Ctx *c, const double *d, double *h);
 }
 
-// v = L^-1 A U^-1 in with the layouts (t, w scratch; w may be v), and the partials of
+// v = L^-1 A U^-1 in with the layouts (t, w scratch; w may be v; the U sweep writes
+// w = in - E t itself where it runs as one launch), and the partials of
 // (v, other) (K = 1) or of (v, other), (v, v) (K = 2) over red_grid(N) workgroups
 template <int K>
 static int dilu_product(Ctx *c, const PrecView &M, const float *in, float *v, float *t, float *w,
                         const float *other, double *Pout, const int *stop) {
     const int N = M.N;
-    SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, in, t, stop));
-    k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, M.dilu_e, t, w, stop);
-    SENK_LAUNCH_CHECK();
+    bool fused = false;
+    SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, in, t, stop, M.dilu_e, w, &fused));
+    if (!fused) {
+        k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, M.dilu_e, t, w, stop);
+        SENK_LAUNCH_CHECK();
+    }
     SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, w, w, stop));
     k_dilu_post<K><<<red_grid(N), kBlock, 0, c->stream>>>(N, M.dilu_d, t, w, v, other, Pout, stop);
     SENK_LAUNCH_CHECK();

@@ -265,7 +265,8 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
                                             const V *__restrict__ ve, const V *__restrict__ vi,
                                             const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx,
                                             const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-                                            unsigned *err, T *copy = nullptr) {
+                                            unsigned *err, T *copy = nullptr, const T *pre_e = nullptr,
+                                            T *pre_w = nullptr) {
     using U = typename Bits<T>::U;
     const int4 si = sinfo[slice];
     const LaneRow<BS> lr(si, blk_list);
@@ -281,6 +282,7 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
         if (!UPPER && in_scale) t = (T)((double)t * *in_scale);
         if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
     }
+    const T rhs = t;
     constexpr int CH = 8;
     constexpr int IC = BS < 8 ? BS : 8;
     int pos = si.x, k0 = 0;
@@ -386,6 +388,9 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
         if (SWEEP) {
             store_agent(y + lr.row, canonical(t));
             if (copy) copy[lr.row] = t;
+            // optional second output pre_w = rhs - pre_e * t (the vector pass that follows
+            // a U solve in the split form of BiCGStab)
+            if (pre_w) pre_w[lr.row] = rhs - pre_e[lr.row] * t;
         } else {
             y[lr.row] = t;
         }
@@ -427,12 +432,13 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sweep(int nslices, const int4
                                                          const DiagT<V> *__restrict__ dinv,
                                                          const signed char *__restrict__ sx, const T *x, T *y,
                                                          const double *__restrict__ in_scale, T *reset,
-                                                         unsigned *err, T *copy, const int *__restrict__ stop_flag) {
+                                                         unsigned *err, T *copy, const T *pre_e, T *pre_w,
+                                                         const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) return;
     const int nw = (int)gridDim.x * kWavesPerBlock;
     for (int t = (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6); t < nslices; t += nw)
         solve_slice<T, V, BS, UPPER, true>(t, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, err, copy);
+                                           reset, err, copy, pre_e, pre_w);
 }
 
 // The whole triangle in one launch of 16-wave workgroups, each solving 16 consecutive
@@ -449,7 +455,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
     const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
     const V *__restrict__ ve, const V *__restrict__ vi, const DiagT<V> *__restrict__ dinv,
     const signed char *__restrict__ sx, const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-    unsigned *sync, T *copy, const int *__restrict__ stop_flag) {
+    unsigned *sync, T *copy, const T *pre_e, T *pre_w, const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             __hip_atomic_fetch_add(sync + kSweepTicket, gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -467,7 +473,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
     const int slice = (int)q * kTicketWaves + (int)(threadIdx.x >> 6);
     if (slice < nslices)
         solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, sync + kSweepErr, copy);
+                                           reset, sync + kSweepErr, copy, pre_e, pre_w);
 }
 
 template <typename T>
@@ -557,7 +563,7 @@ int sweep_grid() {
 template <typename T, typename V, int BS, bool UPPER>
 int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                  const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                 const int *stop) {
+                 const T *pre_e, T *pre_w, const int *stop) {
     using W = typename RowWord<BS>::W;
     int grid = sweep_grid<T, V, BS, UPPER>();
     if (grid <= 0) return SENK_ERR_HIP;
@@ -568,8 +574,8 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
     const W *rw = (const W *)L.rw;
     unsigned *err = L.sync + kSweepErr;
     const double *sc = UPPER ? nullptr : in_scale;
-    void *args[] = {&nslices, &sinfo, &blk, &rw,  &next,  &scol, &ve,   &vi,  &dinv,
-                    &sx,      &x,     &y,   &sc, &reset, &err, &copy, &stop};
+    void *args[] = {&nslices, &sinfo, &blk,  &rw,   &next,  &scol,  &ve,    &vi,  &dinv, &sx,
+                    &x,       &y,     &sc,   &reset, &err, &copy, &pre_e, &pre_w, &stop};
     SENK_HIP(hipLaunchCooperativeKernel((const void *)k_sptrsv_sweep<T, V, BS, UPPER>, dim3(grid), dim3(kBlock), args,
                                         0, c->stream));
     return SENK_OK;
@@ -578,13 +584,13 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
 template <typename T, typename V, int BS, bool UPPER>
 int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                   const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                  const int *stop) {
+                  const T *pre_e, T *pre_w, const int *stop) {
     using W = typename RowWord<BS>::W;
     const unsigned grid = (unsigned)((L.nslices + kTicketWaves - 1) / kTicketWaves);
     const unsigned base = L.ticket_base;
     k_sptrsv_ticket<T, V, BS, UPPER><<<grid, 64 * kTicketWaves, 0, c->stream>>>(
         L.nslices, base, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-        UPPER ? nullptr : in_scale, reset, L.sync, copy, stop);
+        UPPER ? nullptr : in_scale, reset, L.sync, copy, pre_e, pre_w, stop);
     SENK_LAUNCH_CHECK();
     L.ticket_base = base + grid;
     return SENK_OK;
@@ -593,14 +599,18 @@ int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V 
 template <typename T, typename V, int BS>
 int solve_sweep(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                 const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                const int *stop) {
+                const int *stop, const T *pre_e = nullptr, T *pre_w = nullptr) {
     if (L.mode == kTrsvTicket) {
         if (L.upper)
-            return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, stop);
-        return launch_ticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, stop);
+            return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e,
+                                                 pre_w, stop);
+        return launch_ticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e, pre_w,
+                                              stop);
     }
-    if (L.upper) return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, stop);
-    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, stop);
+    if (L.upper)
+        return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e, pre_w,
+                                            stop);
+    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e, pre_w, stop);
 }
 
 template <typename T>
@@ -812,13 +822,15 @@ int launch_ilu_layout(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule 
     if (!lve || !lvi || !uve || !uvi || !udv) return SENK_ERR_ARG;
     return ilu_layout_apply<T, T>(c, ls, lt, us, ut, lve, lvi, uve, uvi, udv, nullptr, nullptr, in, out, in_scale, stop);
 }
-// One triangle on its own, in = out allowed.  The single-launch modes keep their
+// One triangle on its own, in = out allowed; in the single-launch modes, with pre_w,
+// also pre_w = in - pre_e * out (*fused tells whether it was written).  The single-launch modes keep their
 // invariant as long as the triangles of one factorization alternate (L, U, L, ...):
 // each sweep writes its own coherent vector, which the other triangle's last sweep
 // left all-sentinel, and returns the other triangle's vector to the sentinel.
 template <typename T>
 int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
-                      const int *stop) {
+                      const int *stop, const T *pre_e, T *pre_w, bool *fused) {
+    if (fused) *fused = false;
     if (self.nslices == 0) return SENK_OK;
     if (self.bsize != s.bsize || self.n != other.n || self.upper == other.upper) return SENK_ERR_ARG;
     const bool f64 = std::is_same<T, double>::value;
@@ -829,15 +841,16 @@ int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &oth
     T *mine = (T *)(f64 ? self.mid64 : self.mid32);
     T *theirs = (T *)(f64 ? other.mid64 : other.mid32);
     if (self.mode != kTrsvLevels && mine && theirs && self.sync) {
-        SENK_BS_DISPATCH(self.bsize,
-                         (solve_sweep<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, mine, nullptr, theirs, out, stop)))
+        if (fused) *fused = pre_w != nullptr;
+        SENK_BS_DISPATCH(self.bsize, (solve_sweep<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, mine, nullptr, theirs,
+                                                           out, stop, pre_e, pre_w)))
     }
     SENK_BS_DISPATCH(self.bsize, (solve_levels<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, out, nullptr, stop)))
 }
 template int launch_tri_single<double>(Ctx *, const Schedule &, TriLayout &, TriLayout &, const double *, double *,
-                                       const int *);
+                                       const int *, const double *, double *, bool *);
 template int launch_tri_single<float>(Ctx *, const Schedule &, TriLayout &, TriLayout &, const float *, float *,
-                                      const int *);
+                                      const int *, const float *, float *, bool *);
 template int launch_ilu_layout<double>(Ctx *, const Schedule &, TriLayout &, const Schedule &, TriLayout &,
                                        const double *, double *, const double *, const int *, bool);
 template int launch_ilu_layout<float>(Ctx *, const Schedule &, TriLayout &, const Schedule &, TriLayout &,
