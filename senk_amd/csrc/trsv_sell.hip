@@ -282,7 +282,7 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
         if (!UPPER && in_scale) t = (T)((double)t * *in_scale);
         if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
     }
-    const T rhs = t;
+    const T rhs = UPPER ? t : T(0);  // kept for pre_w (U only)
     constexpr int CH = 8;
     constexpr int IC = BS < 8 ? BS : 8;
     int pos = si.x, k0 = 0;
@@ -388,9 +388,11 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
         if (SWEEP) {
             store_agent(y + lr.row, canonical(t));
             if (copy) copy[lr.row] = t;
-            // optional second output pre_w = rhs - pre_e * t (the vector pass that follows
-            // a U solve in the split form of BiCGStab)
-            if (pre_w) pre_w[lr.row] = rhs - pre_e[lr.row] * t;
+            // optional second output of a U sweep, pre_w = rhs - pre_e * t (the vector pass
+            // that follows a U solve in the split form of BiCGStab)
+            if constexpr (UPPER) {
+                if (pre_w) pre_w[lr.row] = rhs - pre_e[lr.row] * t;
+            }
         } else {
             y[lr.row] = t;
         }
@@ -841,6 +843,7 @@ int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &oth
     T *mine = (T *)(f64 ? self.mid64 : self.mid32);
     T *theirs = (T *)(f64 ? other.mid64 : other.mid32);
     if (self.mode != kTrsvLevels && mine && theirs && self.sync) {
+        if (!self.upper) pre_w = nullptr;
         if (fused) *fused = pre_w != nullptr;
         SENK_BS_DISPATCH(self.bsize, (solve_sweep<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, mine, nullptr, theirs,
                                                            out, stop, pre_e, pre_w)))
