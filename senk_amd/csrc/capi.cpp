@@ -72,6 +72,9 @@ int ctx_events(Ctx *c) {
         SENK_HIP(hipEventCreateWithFlags(&c->ev[i], flags));
     }
     SENK_HIP(hipHostMalloc((void **)&c->ring, sizeof(int) * 4 * 8, hipHostMallocDefault));
+    SENK_HIP(hipHostMalloc((void **)&c->dring_h, sizeof(int) * 4 * 8, hipHostMallocCoherent));
+    std::memset(c->dring_h, 0, sizeof(int) * 4 * 8);
+    SENK_HIP(hipHostGetDevicePointer((void **)&c->dring_d, c->dring_h, 0));
     return SENK_OK;
 }
 int ctx_outer(Ctx *c, size_t bytes, void **out) { return arena_get(c->outer, bytes, out); }
@@ -231,6 +234,7 @@ int senk_ctx_destroy(senk_ctx ctx) {
     for (hipEvent_t e : c.ev)
         if (e) (void)hipEventDestroy(e);
     if (c.ring) (void)hipHostFree(c.ring);
+    if (c.dring_h) (void)hipHostFree(c.dring_h);
     (void)hipFree(c.red_partials);
     (void)hipFree(c.dscalars);
     (void)hipHostFree(c.hpinned);

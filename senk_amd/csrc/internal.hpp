@@ -59,6 +59,10 @@ struct Ctx {
     // polling, [8, 10) inner solve timer, [10, 12) outer solve timer
     hipEvent_t ev[12] = {};
     int *ring = nullptr;
+    // coherent pinned ring the solvers' last kernel of an iteration writes itself (host
+    // and device addresses), 8 slots of 4 ints: state words and a sequence number
+    int *dring_h = nullptr, *dring_d = nullptr;
+    unsigned dseq = 0;  // last sequence number handed out for the ring (runs on across solves)
 };
 // Creates the context's reusable events and pinned ring on first use.
 int ctx_events(Ctx *c);

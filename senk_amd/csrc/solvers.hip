@@ -163,7 +163,11 @@ struct Poller {
         ev = c->ev;
         return SENK_OK;
     }
-    void reset() { issued = checked = 0; stopped = false; }
+    void reset() {
+        issued = checked = 0;
+        stopped = false;
+        devm = false;
+    }
     int mark() {
         const int slot = (int)(issued % kRing);
         SENK_HIP(hipMemcpyAsync(hring + 4 * slot, dstate, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
@@ -171,12 +175,38 @@ struct Poller {
         issued++;
         return SENK_OK;
     }
+    // A mark the iteration's last kernel writes itself (no copy, no event): it stores
+    // {sequence number, stop flag} as one 64-bit word into slot (issued % kRing) of the
+    // context's coherent ring; pass the returned pointer and *seq to that kernel.
+    unsigned long long *dev_mark(unsigned *seq) {
+        const int slot = (int)(issued % kRing);
+        *seq = ++c->dseq;
+        expect[slot] = *seq;
+        devm = true;
+        issued++;
+        return reinterpret_cast<unsigned long long *>(c->dring_d + 4 * slot);
+    }
+    unsigned expect[kRing] = {};
+    bool devm = false;
     // Wait until at most `lag` marks are unchecked; returns stop seen.
     int poll(int lag, bool *stop) {
         while (!stopped && issued - checked > lag) {
             const int slot = (int)(checked % kRing);
-            SENK_HIP(hipEventSynchronize(ev[slot]));
-            if (hring[4 * slot]) stopped = true;
+            if (devm) {
+                const unsigned long long *w = reinterpret_cast<const unsigned long long *>(c->dring_h + 4 * slot);
+                unsigned long long v;
+                for (long long spin = 0; ((v = __atomic_load_n(w, __ATOMIC_ACQUIRE)) >> 32) != expect[slot]; spin++) {
+                    if ((spin & 4095) == 4095 && hipStreamQuery(c->stream) == hipSuccess &&
+                        (__atomic_load_n(w, __ATOMIC_ACQUIRE) >> 32) != expect[slot]) {
+                        record_error("solver: an iteration's mark did not arrive");
+                        return SENK_ERR_HIP;
+                    }
+                }
+                if ((unsigned)v) stopped = true;
+            } else {
+                SENK_HIP(hipEventSynchronize(ev[slot]));
+                if (hring[4 * slot]) stopped = true;
+            }
             checked++;
         }
         *stop = stopped;
@@ -1049,12 +1079,21 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(int N, const T *__restri
 }
 // stop test on ||r||; beta = alpha/omega * rr_new/rr_old; p = beta(-omega AKp + p) + r
 // (:149-161)
+// {seq, stop} as one 64-bit word to the host ring (system scope, no fence: the host
+// reads nothing else from the slot)
+__device__ __forceinline__ void write_mark(unsigned long long *mark, unsigned seq, int stop) {
+    __hip_atomic_store(mark, ((unsigned long long)seq << 32) | (unsigned)(stop != 0), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_p(int N, const T *__restrict__ AKp, const T *__restrict__ r,
                                                    double *sc, int parity, const double *Pin, int B, double tol,
                                                    double nrm_b, int iter, int *state, double *hist, int hcap,
-                                                   T *p) {
-    if (state[0]) return;
+                                                   T *p, unsigned long long *mark, unsigned seq) {
+    if (state[0]) {
+        if (mark && blockIdx.x == 0 && threadIdx.x == 0) write_mark(mark, seq, 1);
+        return;
+    }
     const double nr = sqrt(sum_partials(Pin, B));
     const double rr_new = sum_partials(Pin + kPartLd, B);
     const bool fin = !isfinite(nr), conv = nr < tol;
@@ -1069,6 +1108,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(int N, const T *__restrict__ 
         } else {
             sc[2 + (parity ^ 1)] = rr_new;
         }
+        if (mark) write_mark(mark, seq, fin || conv);
     }
     if (fin || conv) return;
     const double rr_old = sc[2 + parity];
@@ -1228,10 +1268,11 @@ static int bicgstab_split(Ctx *c, const MatView &A, const PrecView &M, const flo
         k_bicg_update<float><<<B, kBlock, 0, c->stream>>>(N, pp, s, tv, s, rs, y, r, sc, red_slot(c, kSlotB), B,
                                                           red_slot(c, kSlotC), state);
         SENK_LAUNCH_CHECK();
+        unsigned seq = 0;
+        unsigned long long *mark = poll.dev_mark(&seq);
         k_bicg_p<float><<<B, kBlock, 0, c->stream>>>(N, v, r, sc, par, red_slot(c, kSlotC), B, tol, nrm_c, i, state,
-                                                     dhist, hcap, pp);
+                                                     dhist, hcap, pp, mark, seq);
         SENK_LAUNCH_CHECK();
-        SENK_TRY(poll.mark());
         SENK_TRY(poll.poll(kLag, &stop));
     }
     // x = U^-1 y (after the stop flag is set: no flag for this sweep)
@@ -1310,10 +1351,11 @@ static int bicgstab(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *
         k_bicg_update<T><<<B, kBlock, 0, c->stream>>>(N, Kp, Ks, AKs, s, rs, x, r, sc, red_slot(c, kSlotB), B,
                                                       red_slot(c, kSlotC), state);
         SENK_LAUNCH_CHECK();
+        unsigned seq = 0;
+        unsigned long long *mark = poll.dev_mark(&seq);
         k_bicg_p<T><<<B, kBlock, 0, c->stream>>>(N, AKp, r, sc, par, red_slot(c, kSlotC), B, tol, p.nrm_b, i, state,
-                                                 dhist, hcap, pp);
+                                                 dhist, hcap, pp, mark, seq);
         SENK_LAUNCH_CHECK();
-        SENK_TRY(poll.mark());
         SENK_TRY(poll.poll(kLag, &stop));
     }
     int hstate[4];
