@@ -66,12 +66,8 @@ static int arena_get(DevBuf &a, size_t bytes, void **out) {
 }
 int ctx_work(Ctx *c, size_t bytes, void **out) { return arena_get(c->work, bytes, out); }
 int ctx_events(Ctx *c) {
-    if (c->ring) return SENK_OK;
-    for (int i = 0; i < 12; i++) {
-        const unsigned flags = i < 8 ? hipEventDisableTiming : hipEventDefault;
-        SENK_HIP(hipEventCreateWithFlags(&c->ev[i], flags));
-    }
-    SENK_HIP(hipHostMalloc((void **)&c->ring, sizeof(int) * 4 * 8, hipHostMallocDefault));
+    if (c->dring_h) return SENK_OK;
+    for (int i = 8; i < 12; i++) SENK_HIP(hipEventCreateWithFlags(&c->ev[i], hipEventDefault));
     SENK_HIP(hipHostMalloc((void **)&c->dring_h, sizeof(int) * 4 * 8, hipHostMallocCoherent));
     std::memset(c->dring_h, 0, sizeof(int) * 4 * 8);
     SENK_HIP(hipHostGetDevicePointer((void **)&c->dring_d, c->dring_h, 0));
@@ -233,7 +229,6 @@ int senk_ctx_destroy(senk_ctx ctx) {
     if (c.outer.p) (void)hipFree(c.outer.p);
     for (hipEvent_t e : c.ev)
         if (e) (void)hipEventDestroy(e);
-    if (c.ring) (void)hipHostFree(c.ring);
     if (c.dring_h) (void)hipHostFree(c.dring_h);
     (void)hipFree(c.red_partials);
     (void)hipFree(c.dscalars);

@@ -55,12 +55,12 @@ struct Ctx {
     DevBuf work;                      // solver arena
     DevBuf outer;                     // arena of the FP64 outer loop around an inner solve
     int num_cus = 256;
-    // events and a pinned ring that solves reuse (created on first use): [0, 8) stop
-    // polling, [8, 10) inner solve timer, [10, 12) outer solve timer
+    // events that solves reuse (created on first use): [8, 10) inner solve timer, [10, 12)
+    // outer solve timer
     hipEvent_t ev[12] = {};
-    int *ring = nullptr;
     // coherent pinned ring the solvers' last kernel of an iteration writes itself (host
-    // and device addresses), 8 slots of 4 ints: state words and a sequence number
+    // and device addresses), 8 slots of 4 ints: {sequence number, stop flag} in the first 8
+    // bytes of each
     int *dring_h = nullptr, *dring_d = nullptr;
     unsigned dseq = 0;  // last sequence number handed out for the ring (runs on across solves)
 };

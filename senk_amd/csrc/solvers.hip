@@ -28,6 +28,13 @@ namespace senk_amd {
 
 namespace {
 
+// {seq, stop} as one 64-bit word to the host ring (system scope, no fence: the host
+// reads nothing else from the slot); see Poller::dev_mark
+__device__ __forceinline__ void write_mark(unsigned long long *mark, unsigned seq, int stop) {
+    __hip_atomic_store(mark, ((unsigned long long)seq << 32) | (unsigned)(stop != 0), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename T> const T *mvals(const MatView &A);
 template <> const double *mvals<double>(const MatView &A) { return A.v64; }
 template <> const float *mvals<float>(const MatView &A) { return A.v32; }
@@ -147,66 +154,48 @@ int residual(Ctx *c, const MatView &A, const T *x, const T *b, T *y, int *grid_o
 }
 
 // ----------------------------------------------------------------- host-side stop polling
+// The last kernel of an iteration writes {sequence number, stop flag} as one 64-bit word
+// into a slot of the context's coherent pinned ring (no copy, no event per iteration);
+// the host, some iterations behind, waits for each slot's number and reads the flag.
+// Sequence numbers run on across solves (Ctx::dseq), so a slot's old value never passes
+// for a new mark.
 struct Poller {
     static constexpr int kRing = 8;
-    Ctx *c;
-    const int *dstate;
-    int *hring = nullptr;  // the context's pinned ring, kRing * 4 ints
-    hipEvent_t *ev = nullptr;
+    Ctx *c = nullptr;
     long long issued = 0, checked = 0;
     bool stopped = false;
-    int init(Ctx *cc, const int *ds) {
+    unsigned expect[kRing] = {};
+    int init(Ctx *cc, const int *) {
         c = cc;
-        dstate = ds;
-        SENK_TRY(ctx_events(c));
-        hring = c->ring;
-        ev = c->ev;
-        return SENK_OK;
+        return ctx_events(c);
     }
     void reset() {
         issued = checked = 0;
         stopped = false;
-        devm = false;
     }
-    int mark() {
-        const int slot = (int)(issued % kRing);
-        SENK_HIP(hipMemcpyAsync(hring + 4 * slot, dstate, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
-        SENK_HIP(hipEventRecord(ev[slot], c->stream));
-        issued++;
-        return SENK_OK;
-    }
-    // A mark the iteration's last kernel writes itself (no copy, no event): it stores
-    // {sequence number, stop flag} as one 64-bit word into slot (issued % kRing) of the
-    // context's coherent ring; pass the returned pointer and *seq to that kernel.
+    // Slot for the next mark (device address) and its number; pass both to the kernel.
     unsigned long long *dev_mark(unsigned *seq) {
         const int slot = (int)(issued % kRing);
         *seq = ++c->dseq;
         expect[slot] = *seq;
-        devm = true;
         issued++;
         return reinterpret_cast<unsigned long long *>(c->dring_d + 4 * slot);
     }
-    unsigned expect[kRing] = {};
-    bool devm = false;
-    // Wait until at most `lag` marks are unchecked; returns stop seen.
+    // Wait until at most `lag` marks are unchecked; returns stop seen.  A mark that has
+    // not arrived when the stream has nothing left to run is an error, not a wait.
     int poll(int lag, bool *stop) {
         while (!stopped && issued - checked > lag) {
             const int slot = (int)(checked % kRing);
-            if (devm) {
-                const unsigned long long *w = reinterpret_cast<const unsigned long long *>(c->dring_h + 4 * slot);
-                unsigned long long v;
-                for (long long spin = 0; ((v = __atomic_load_n(w, __ATOMIC_ACQUIRE)) >> 32) != expect[slot]; spin++) {
-                    if ((spin & 4095) == 4095 && hipStreamQuery(c->stream) == hipSuccess &&
-                        (__atomic_load_n(w, __ATOMIC_ACQUIRE) >> 32) != expect[slot]) {
-                        record_error("solver: an iteration's mark did not arrive");
-                        return SENK_ERR_HIP;
-                    }
+            const unsigned long long *w = reinterpret_cast<const unsigned long long *>(c->dring_h + 4 * slot);
+            unsigned long long v;
+            for (long long spin = 0; ((v = __atomic_load_n(w, __ATOMIC_ACQUIRE)) >> 32) != expect[slot]; spin++) {
+                if ((spin & 4095) == 4095 && hipStreamQuery(c->stream) == hipSuccess &&
+                    (__atomic_load_n(w, __ATOMIC_ACQUIRE) >> 32) != expect[slot]) {
+                    record_error("solver: an iteration's mark did not arrive");
+                    return SENK_ERR_HIP;
                 }
-                if ((unsigned)v) stopped = true;
-            } else {
-                SENK_HIP(hipEventSynchronize(ev[slot]));
-                if (hring[4 * slot]) stopped = true;
             }
+            if ((unsigned)v) stopped = true;
             checked++;
         }
         *stop = stopped;
@@ -389,8 +378,11 @@ __global__ __launch_bounds__(kBlock) void k_cgs_fin(const double *P, int B, cons
 __global__ __launch_bounds__(kBlock) void k_gmres_tail(int j, int m, double *H, double *cs, double *sn, double *e,
                                                        double *sig, const double *Pn, int B, const double *add_h,
                                                        double tol, double nrm_b, int *state, double *hist, int hidx,
-                                                       int hcap) {
-    if (state[0]) return;
+                                                       int hcap, unsigned long long *mark, unsigned seq) {
+    if (state[0]) {
+        if (mark && threadIdx.x == 0) write_mark(mark, seq, 1);
+        return;
+    }
     const double nn = sum_partials(Pn, B);
     if (threadIdx.x != 0) return;
     double *hc = H + (size_t)j * (m + 1);
@@ -421,6 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_gmres_tail(int j, int m, double *H, 
     } else if (ae <= tol) {
         state[0] = 1;
     }
+    if (mark) write_mark(mark, seq, state[0]);
 }
 
 // Cycle end: y = Trsv(H, e) over the jdone columns (senk_blas2.hpp:24-34) and the
@@ -637,10 +630,11 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
                 }
                 Pnorm = red_slot(c, (j + 1) % 2 ? kSlotMgsB : kSlotMgsA);
             }
+            unsigned seq = 0;
+            unsigned long long *mark = poll.dev_mark(&seq);
             k_gmres_tail<<<1, kBlock, 0, c->stream>>>(j, m, H, cs, sn, e, sig, Pnorm, B, add_h, tol, p.nrm_b, state,
-                                                      dhist, i * m + j, hcap);
+                                                      dhist, i * m + j, hcap, mark, seq);
             SENK_LAUNCH_CHECK();
-            SENK_TRY(poll.mark());
             SENK_TRY(poll.poll(kLag, &stop));
         }
         // cycle end: x += M^-1 (V y)   (senk_gmres.hpp:248-255), or x += V y / Z y
@@ -795,7 +789,10 @@ __global__ __launch_bounds__(kBlock) void k_gcr_beta_fin(const double *P, int B,
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gcr_newdir(int N, int K, T *p, T *Ap, size_t ld, const T *Kr,
                                                        const T *AKr, const double *beta, const T *__restrict__ r,
-                                                       double *Palpha, const int *stop) {
+                                                       double *Palpha, const int *stop, unsigned long long *mark,
+                                                       unsigned seq) {
+    // the step's stop test ran before: its flag is this step's mark
+    if (mark && blockIdx.x == 0 && threadIdx.x == 0) write_mark(mark, seq, *stop);
     if (*stop) return;
     double a0 = 0, a1 = 0;
     SENK_FOR4(i0, N) {
@@ -937,12 +934,12 @@ static int gcr(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, co
             SENK_LAUNCH_CHECK();
             k_gcr_beta_fin<<<K, kBlock, 0, c->stream>>>(red_slot(c, kSlotB), B, dotAp, beta, state);
             SENK_LAUNCH_CHECK();
-            k_gcr_newdir<T><<<B, kBlock, 0, c->stream>>>(N, K, P, AP, ld, z, AKr, beta, r, red_slot(c, kSlotA), state);
+            unsigned seq = 0;
+            unsigned long long *mark = p.check_every ? poll.dev_mark(&seq) : nullptr;
+            k_gcr_newdir<T><<<B, kBlock, 0, c->stream>>>(N, K, P, AP, ld, z, AKr, beta, r, red_slot(c, kSlotA), state,
+                                                         mark, seq);
             SENK_LAUNCH_CHECK();
-            if (p.check_every) {
-                SENK_TRY(poll.mark());
-                SENK_TRY(poll.poll(kLag, &stop));
-            }
+            if (p.check_every) SENK_TRY(poll.poll(kLag, &stop));
         }
         // a cycle that ran to its end did m steps
         last = (i + 1) * m;
@@ -1079,12 +1076,6 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(int N, const T *__restri
 }
 // stop test on ||r||; beta = alpha/omega * rr_new/rr_old; p = beta(-omega AKp + p) + r
 // (:149-161)
-// {seq, stop} as one 64-bit word to the host ring (system scope, no fence: the host
-// reads nothing else from the slot)
-__device__ __forceinline__ void write_mark(unsigned long long *mark, unsigned seq, int stop) {
-    __hip_atomic_store(mark, ((unsigned long long)seq << 32) | (unsigned)(stop != 0), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_SYSTEM);
-}
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_p(int N, const T *__restrict__ AKp, const T *__restrict__ r,
                                                    double *sc, int parity, const double *Pin, int B, double tol,
