@@ -155,15 +155,16 @@ def kernel_rates(ctx, dev, reps=50, sweep_modes=False):
            "colors": dev.sy["nc"], "bsize": dev.sy["bsize"]}
     if not sweep_modes:
         return out
-    # triangular-solve launch modes: 1 one cooperative sweep per triangle, 2 (the default)
-    # one ticketed sweep per triangle, 0 one launch per level; the default is restored after
-    for mode in (1, 2, 0):
+    # triangular-solve launch modes: 1 one cooperative sweep per triangle, 3 one launch per
+    # triangle with a ticket stream per XCD (the FP32 default), 2 one ticketed sweep per
+    # triangle (the FP64 default), 0 one launch per level; the defaults are restored after
+    for mode in (1, 3, 2, 0):
         S.call("senk_prec_set_trsv_mode", dev.M, mode)
         out[f"ilu_apply_f64_us_mode{mode}"] = time_kernel(
             ctx, lambda: S.call("senk_prec_apply_f64", dev.M, dx.p, dy.p), 20) * 1e3
         out[f"ilu_apply_f32_us_mode{mode}"] = time_kernel(
             ctx, lambda: S.call("senk_prec_apply_f32", dev.M, dx32.p, dy32.p), 20) * 1e3
-    S.call("senk_prec_set_trsv_mode", dev.M, 2)
+    S.call("senk_prec_set_trsv_mode", dev.M, -1)
     S.call("senk_prec_check", dev.M)
     return out
 

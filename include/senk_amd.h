@@ -397,10 +397,15 @@ int senk_prec_apply_f16(senk_prec M, const float *dx, float *dy);
 /* Triangular-solve launch mode of the sliced layout (no reference counterpart; the
  * reference runs one OpenMP parallel loop per color, senk_sparse.hpp:207-246):
  * 0 one launch per level; 1 one cooperative launch per triangle, each gather waiting
- * until the row it reads is final; 2 (default) one launch per triangle of workgroups
- * that draw tickets in arrival order (no residency assumption), gathers waiting
- * likewise.  Results are bit-identical in every mode.  Applications of one
- * preconditioner must not overlap (one stream). */
+ * until the row it reads is final; 2 one launch per triangle of workgroups that draw
+ * tickets in arrival order (no residency assumption), gathers waiting likewise; 3 the
+ * same with one ticket stream per XCD, stream x taking the x-th eighth of every level
+ * (workgroup b counts as XCD b mod 8; every stream has the same number of tickets per
+ * level, so a wait is only on workgroups handed out earlier).  Defaults: 2 for FP64
+ * applications, 3 for FP32 and FP16-valued ones; this call sets both, -1 restores the
+ * defaults.  Results are
+ * bit-identical in every mode.  Applications of one preconditioner must not overlap (one
+ * stream). */
 int senk_prec_set_trsv_mode(senk_prec M, int mode);
 /* Split form of the mixed forms' inner BiCGStab (no reference counterpart; the
  * reference's BiCGStab is right-preconditioned, senk_bicgstab.hpp:129-161).  When the

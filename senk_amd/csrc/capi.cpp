@@ -1318,10 +1318,11 @@ int senk_prec_apply_f16(senk_prec M, const float *dx, float *dy) {
     return prec_apply<float>(&M->ctx->c, v, dx, nullptr, dy, nullptr, nullptr);
 }
 int senk_prec_set_trsv_mode(senk_prec M, int mode) {
-    if (!M || mode < senk_amd::kTrsvLevels || mode > senk_amd::kTrsvTicket) return SENK_ERR_ARG;
+    if (!M || mode < -1 || mode > senk_amd::kTrsvXcd) return SENK_ERR_ARG;
     SENK_HIP(hipStreamSynchronize(M->ctx->c.stream));
-    M->lt.mode = mode;
-    M->ut.mode = mode;
+    const senk_amd::TriLayout defaults;
+    M->lt.mode = M->ut.mode = mode < 0 ? defaults.mode : mode;
+    M->lt.mode32 = M->ut.mode32 = mode < 0 ? defaults.mode32 : mode;
     return SENK_OK;
 }
 int senk_prec_set_split(senk_prec M, int enable) {

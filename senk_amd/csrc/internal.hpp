@@ -89,7 +89,7 @@ struct Schedule {
 // Launch modes of the layout solve: one launch per level, or one launch per triangle
 // whose gathers wait for the rows they read (sentinel-initialised output), with every
 // wave resident (cooperative) or with workgroups drawing tickets in arrival order.
-constexpr int kTrsvLevels = 0, kTrsvSweep = 1, kTrsvTicket = 2;
+constexpr int kTrsvLevels = 0, kTrsvSweep = 1, kTrsvTicket = 2, kTrsvXcd = 3;
 struct TriLayout {
     int bsize = 0;
     bool upper = false;
@@ -111,8 +111,19 @@ struct TriLayout {
     void *ve16 = nullptr, *vi16 = nullptr;
     float *d16 = nullptr;
     signed char *sx16 = nullptr;
-    int mode = kTrsvTicket;
+    // launch mode of the FP64 and of the FP32 (and FP16-valued) applications: measured
+    // at 128^3, the per-XCD streams make the FP32 application 8% faster, while the FP64
+    // GMRES(30) ran 1% slower with them than with the single ticket stream
+    int mode = kTrsvTicket, mode32 = kTrsvXcd;
+    template <typename T>
+    int mode_of() const { return sizeof(T) == 8 ? mode : mode32; }
     unsigned ticket_base = 0;  // ticketed sweep: workgroups launched so far (mod 2^32)
+    // per-XCD ticket streams (mode 3): stream x holds the x-th eighth of every level's
+    // slices, each level padded with -1 to the same number of 16-slice tickets in every
+    // stream; xwg tickets per stream, xslice[x * 16 * xwg + k]
+    int *xslice = nullptr;
+    int xwg = 0;
+    unsigned xticket_base = 0;
 };
 int tri_layout_check(Ctx *c, TriLayout *L);
 int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rptr, const int *cind,

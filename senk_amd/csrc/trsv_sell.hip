@@ -245,7 +245,9 @@ __device__ __forceinline__ float canonical(float v) { return __builtin_canonical
 __device__ __forceinline__ double canonical(double v) { return __builtin_canonicalize(v); }
 
 // Sweep words (unsigned, device): time-out flag, ticket counter, spares.
-constexpr int kSweepErr = 0, kSweepTicket = 1, kSweepWords = 4;
+// sync words: error flag, the ticket counter of mode 2, and one counter per XCD stream of
+// mode 3 (128 bytes apart)
+constexpr int kSweepErr = 0, kSweepTicket = 1, kSweepXTicket = 32, kSweepWords = 32 + 32 * kXcds;
 // polls of one pending gather before the wave gives up and raises the time-out flag;
 // a poll takes about a microsecond, so this bounds a wait at a few seconds
 constexpr int kSweepSpinLimit = 1 << 22;
@@ -478,6 +480,42 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
                                            reset, sync + kSweepErr, copy, pre_e, pre_w);
 }
 
+// Mode 3: the same with one ticket stream per XCD.  Workgroup b belongs to stream
+// b mod 8 (the XCD the dispatcher hands it to) and draws that stream's tickets in
+// arrival order; stream x solves the x-th eighth of every level (a band of the domain),
+// so most gathers find their rows in the XCD's own L2.  Every stream has the same
+// number of tickets per level (slices padded with -1), so ticket j of any stream is in
+// the same level: a wait is only on tickets < j of the other streams, whose workgroups
+// were handed out before this one.
+template <typename T, typename V, int BS, bool UPPER>
+__global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_xticket(
+    int xwg, unsigned base, const int *__restrict__ xslice, const int4 *__restrict__ sinfo,
+    const int *__restrict__ blk_list, const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next,
+    const int *__restrict__ scol, const V *__restrict__ ve, const V *__restrict__ vi,
+    const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx, const T *x, T *y,
+    const double *__restrict__ in_scale, T *reset, unsigned *sync, T *copy, const T *pre_e, T *pre_w,
+    const int *__restrict__ stop_flag) {
+    const int xs = (int)(blockIdx.x % kXcds);
+    unsigned *ctr = sync + kSweepXTicket + 32 * xs;
+    if (stop_flag && *stop_flag) {
+        if (blockIdx.x < kXcds && threadIdx.x == 0)
+            __hip_atomic_fetch_add(ctr, (unsigned)xwg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    __shared__ unsigned ticket;
+    if (threadIdx.x == 0) ticket = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base;
+    __syncthreads();
+    const unsigned q = ticket;
+    if (q >= (unsigned)xwg) {  // launches out of step with the host's count
+        if (threadIdx.x == 0) __hip_atomic_store(sync + kSweepErr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const int slice = xslice[(size_t)xs * kTicketWaves * xwg + (size_t)q * kTicketWaves + (threadIdx.x >> 6)];
+    if (slice >= 0)
+        solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
+                                           reset, sync + kSweepErr, copy, pre_e, pre_w);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill_sentinel(int n, T *p) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -598,11 +636,34 @@ int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V 
     return SENK_OK;
 }
 
+template <typename T, typename V, int BS, bool UPPER>
+int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
+                   const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
+                   const T *pre_e, T *pre_w, const int *stop) {
+    using W = typename RowWord<BS>::W;
+    if (!L.xslice || L.xwg <= 0) return SENK_ERR_ARG;
+    const unsigned base = L.xticket_base;
+    k_sptrsv_xticket<T, V, BS, UPPER><<<kXcds * L.xwg, 64 * kTicketWaves, 0, c->stream>>>(
+        L.xwg, base, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
+        UPPER ? nullptr : in_scale, reset, L.sync, copy, pre_e, pre_w, stop);
+    SENK_LAUNCH_CHECK();
+    L.xticket_base = base + (unsigned)L.xwg;
+    return SENK_OK;
+}
+
 template <typename T, typename V, int BS>
 int solve_sweep(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                 const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
                 const int *stop, const T *pre_e = nullptr, T *pre_w = nullptr) {
-    if (L.mode == kTrsvTicket) {
+    const int mode = L.mode_of<T>();
+    if (mode == kTrsvXcd) {
+        if (L.upper)
+            return launch_xticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e,
+                                                  pre_w, stop);
+        return launch_xticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e, pre_w,
+                                               stop);
+    }
+    if (mode == kTrsvTicket) {
         if (L.upper)
             return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e,
                                                  pre_w, stop);
@@ -640,7 +701,7 @@ size_t row_word_bytes(int bs) { return bs <= 16 ? sizeof(unsigned) : sizeof(unsi
 void tri_layout_free(TriLayout *L) {
     for (void *p : {(void *)L->sinfo, (void *)L->next, L->rw, (void *)L->scol, (void *)L->ve64, (void *)L->vi64,
                     (void *)L->d64, (void *)L->ve32, (void *)L->vi32, (void *)L->d32, (void *)L->sync, L->mid64,
-                    L->mid32, L->ve16, L->vi16, (void *)L->d16, (void *)L->sx16})
+                    L->mid32, L->ve16, L->vi16, (void *)L->d16, (void *)L->sx16, (void *)L->xslice})
         if (p) (void)hipFree(p);
     *L = TriLayout{};
 }
@@ -669,6 +730,32 @@ int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rp
     L->nsteps = (int)L->step_slice.size() - 1;
     L->nslices = (int)si.size();
     if (L->nslices == 0) return SENK_OK;
+    // mode 3 streams: the x-th eighth of every step's slices, steps padded to the same
+    // number of tickets in every stream
+    {
+        std::vector<std::vector<int>> xs(kXcds);
+        for (int q = 0; q < L->nsteps; q++) {
+            const int a = L->step_slice[q], n = L->step_slice[q + 1] - a;
+            int tk = 0;
+            for (int x = 0; x < kXcds; x++) {
+                const int cnt = (int)((long long)n * (x + 1) / kXcds - (long long)n * x / kXcds);
+                tk = std::max(tk, (cnt + kTicketWaves - 1) / kTicketWaves);
+            }
+            for (int x = 0; x < kXcds; x++) {
+                const int lo = a + (int)((long long)n * x / kXcds), hi = a + (int)((long long)n * (x + 1) / kXcds);
+                for (int k = 0; k < tk * kTicketWaves; k++) xs[x].push_back(lo + k < hi ? lo + k : -1);
+            }
+        }
+        L->xwg = (int)(xs[0].size() / kTicketWaves);
+        std::vector<int> flat;
+        flat.reserve((size_t)kXcds * xs[0].size());
+        for (int x = 0; x < kXcds; x++) flat.insert(flat.end(), xs[x].begin(), xs[x].end());
+        if (hipMalloc(&L->xslice, flat.size() * sizeof(int)) != hipSuccess ||
+            hipMemcpy(L->xslice, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            tri_layout_free(L);
+            return SENK_ERR_ALLOC;
+        }
+    }
     const size_t nslots = (size_t)L->nslices * 64;
     int *ecnt = nullptr, *icnt = nullptr, *err = nullptr;
     int st = SENK_OK;
@@ -774,7 +861,7 @@ int ilu_layout_apply(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule &
                      const int *stop) {
     T *mid = (T *)(std::is_same<T, double>::value ? lt.mid64 : lt.mid32);
     T *up = (T *)(std::is_same<T, double>::value ? ut.mid64 : ut.mid32);
-    if (lt.mode != kTrsvLevels && mid && up && lt.sync && ut.sync) {
+    if (lt.mode_of<T>() != kTrsvLevels && mid && up && lt.sync && ut.sync) {
         // L: in -> mid, every row of up (the U sweep's coherent output) becomes the
         // sentinel; U: mid -> up, mid returns to the sentinel as its rows are read, and
         // each final row is also written to out with an ordinary (cached) store
@@ -842,7 +929,7 @@ int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &oth
     if (!ve || !vi || (self.upper && !dv)) return SENK_ERR_ARG;
     T *mine = (T *)(f64 ? self.mid64 : self.mid32);
     T *theirs = (T *)(f64 ? other.mid64 : other.mid32);
-    if (self.mode != kTrsvLevels && mine && theirs && self.sync) {
+    if (self.mode_of<T>() != kTrsvLevels && mine && theirs && self.sync) {
         if (!self.upper) pre_w = nullptr;
         if (fused) *fused = pre_w != nullptr;
         SENK_BS_DISPATCH(self.bsize, (solve_sweep<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, mine, nullptr, theirs,
@@ -869,6 +956,7 @@ int tri_layout_check(Ctx *c, TriLayout *L) {
     // start the next application from a clean state
     SENK_HIP(hipMemsetAsync(L->sync, 0, kSweepWords * sizeof(unsigned), c->stream));
     L->ticket_base = 0;
+    L->xticket_base = 0;
     if (L->mid64) SENK_TRY(fill_sentinel<double>(c, L->n, (double *)L->mid64));
     if (L->mid32) SENK_TRY(fill_sentinel<float>(c, L->n, (float *)L->mid32));
     SENK_HIP(hipStreamSynchronize(c->stream));

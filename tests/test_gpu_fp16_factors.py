@@ -107,7 +107,7 @@ def test_fp16_apply_matches_emulation(ctx, gen, bsize):
         L, U = _download(M, N)
         x = np.random.default_rng(1).uniform(-1, 1, N).astype(np.float32)
         want = _emulate(L, U, x)
-        for mode in (0, 1, 2):
+        for mode in (0, 1, 2, 3):
             S.call("senk_prec_set_trsv_mode", M, mode)
             got = _apply16(ctx, M, x)
             assert np.array_equal(got, want), (mode, np.max(np.abs(got - want)))

@@ -55,7 +55,7 @@ def _solve(ctx, A, M, sy, mixed):
     return res, dx.download()
 
 
-@pytest.mark.parametrize("mode", [2, 1, 0])
+@pytest.mark.parametrize("mode", [2, 3, 1, 0])
 def test_split_and_plain_inner_solves_agree(ctx, mode):
     sy = _system()
     A, M = _handles(ctx, sy)

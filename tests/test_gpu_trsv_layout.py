@@ -21,7 +21,7 @@ from oracle import oracle as O
 from tests.helpers import abmc_system, oracle_ilu0
 
 pytestmark = pytest.mark.gpu
-MODES = (0, 1, 2)  # per-level launches, cooperative sweep, ticketed sweep
+MODES = (0, 1, 2, 3)  # per-level launches, cooperative sweep, ticketed sweep, per-XCD ticket streams
 
 
 def _expected(sy, kind, p):
@@ -224,4 +224,31 @@ def test_spmv_slice_order_keeps_bits(ctx):
         assert np.max(np.abs(after[0] - want)) <= 1e-12 * np.max(np.abs(want))
     finally:
         S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
+
+
+def test_launch_mode_argument(ctx):
+    """Modes 0..3 and -1 (the defaults: 2 for FP64, 3 for FP32) are taken, others refused;
+    after -1 both precisions still give the bits of the per-level launches."""
+    rptr, cind, val = S.gen_poisson7(12)
+    sy = abmc_system(rptr, cind, val, 8)
+    N = sy["N"]
+    A, (Ml, Mc) = _handles(ctx, sy, S.SENK_ILU0, 0, 8, True)
+    try:
+        for bad in (-2, 4):
+            assert S.lib().senk_prec_set_trsv_mode(Ml, bad) == 1  # SENK_ERR_ARG
+        x = np.random.default_rng(9).uniform(-1, 1, N)
+        outs = {}
+        for mode in (0, -1):
+            S.call("senk_prec_set_trsv_mode", Ml, mode)
+            dx, dy = ctx.array(x), ctx.zeros(N)
+            S.call("senk_prec_apply_f64", Ml, dx.p, dy.p)
+            dx32, dy32 = ctx.array(x.astype(np.float32)), ctx.array(np.zeros(N, np.float32))
+            S.call("senk_prec_apply_f32", Ml, dx32.p, dy32.p)
+            outs[mode] = (dy.download(), dy32.download())
+        assert np.array_equal(outs[0][0], outs[-1][0]) and np.array_equal(outs[0][1], outs[-1][1])
+        S.call("senk_prec_check", Ml)
+    finally:
+        for M in (Ml, Mc):
+            S.lib().senk_prec_destroy(M)
         S.lib().senk_mat_destroy(A)
