@@ -156,8 +156,8 @@ def kernel_rates(ctx, dev, reps=50, sweep_modes=False):
     if not sweep_modes:
         return out
     # triangular-solve launch modes: 1 one cooperative sweep per triangle, 3 one launch per
-    # triangle with a ticket stream per XCD (the FP32 default), 2 one ticketed sweep per
-    # triangle (the FP64 default), 0 one launch per level; the defaults are restored after
+    # triangle with a ticket stream per XCD (the default), 2 one ticketed sweep per
+    # triangle, 0 one launch per level; the default is restored after
     for mode in (1, 3, 2, 0):
         S.call("senk_prec_set_trsv_mode", dev.M, mode)
         out[f"ilu_apply_f64_us_mode{mode}"] = time_kernel(

@@ -401,9 +401,8 @@ int senk_prec_apply_f16(senk_prec M, const float *dx, float *dy);
  * tickets in arrival order (no residency assumption), gathers waiting likewise; 3 the
  * same with one ticket stream per XCD, stream x taking the x-th eighth of every level
  * (workgroup b counts as XCD b mod 8; every stream has the same number of tickets per
- * level, so a wait is only on workgroups handed out earlier).  Defaults: 2 for FP64
- * applications, 3 for FP32 and FP16-valued ones; this call sets both, -1 restores the
- * defaults.  Results are
+ * level, so a wait is only on workgroups handed out earlier); the default, which -1
+ * restores.  Results are
  * bit-identical in every mode.  Applications of one preconditioner must not overlap (one
  * stream). */
 int senk_prec_set_trsv_mode(senk_prec M, int mode);

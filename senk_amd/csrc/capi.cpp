@@ -1322,7 +1322,6 @@ int senk_prec_set_trsv_mode(senk_prec M, int mode) {
     SENK_HIP(hipStreamSynchronize(M->ctx->c.stream));
     const senk_amd::TriLayout defaults;
     M->lt.mode = M->ut.mode = mode < 0 ? defaults.mode : mode;
-    M->lt.mode32 = M->ut.mode32 = mode < 0 ? defaults.mode32 : mode;
     return SENK_OK;
 }
 int senk_prec_set_split(senk_prec M, int enable) {

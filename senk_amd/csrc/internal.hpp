@@ -111,16 +111,11 @@ struct TriLayout {
     void *ve16 = nullptr, *vi16 = nullptr;
     float *d16 = nullptr;
     signed char *sx16 = nullptr;
-    // launch mode of the FP64 and of the FP32 (and FP16-valued) applications: measured
-    // at 128^3, the per-XCD streams make the FP32 application 8% faster, while the FP64
-    // GMRES(30) ran 1% slower with them than with the single ticket stream
-    int mode = kTrsvTicket, mode32 = kTrsvXcd;
-    template <typename T>
-    int mode_of() const { return sizeof(T) == 8 ? mode : mode32; }
+    int mode = kTrsvXcd;
     unsigned ticket_base = 0;  // ticketed sweep: workgroups launched so far (mod 2^32)
     // per-XCD ticket streams (mode 3): stream x holds the x-th eighth of every level's
-    // slices, each level padded with -1 to the same number of 16-slice tickets in every
-    // stream; xwg tickets per stream, xslice[x * 16 * xwg + k]
+    // slices, each level padded with -1 to the same number of 8-slice tickets in every
+    // stream; xwg tickets per stream, xslice[x * 8 * xwg + k]
     int *xslice = nullptr;
     int xwg = 0;
     unsigned xticket_base = 0;

@@ -228,8 +228,8 @@ def test_spmv_slice_order_keeps_bits(ctx):
 
 
 def test_launch_mode_argument(ctx):
-    """Modes 0..3 and -1 (the defaults: 2 for FP64, 3 for FP32) are taken, others refused;
-    after -1 both precisions still give the bits of the per-level launches."""
+    """Modes 0..3 and -1 (the default, 3) are taken, others refused; after -1 both
+    precisions still give the bits of the per-level launches."""
     rptr, cind, val = S.gen_poisson7(12)
     sy = abmc_system(rptr, cind, val, 8)
     N = sy["N"]
