@@ -19,8 +19,8 @@ driver times only the solve).
                  application (weights from senk_amd/data/gnn_vardiff.npz)
   plain_fp64_c2  the 27-point convection-diffusion 128^3 system (Pe 10, ABMC blocks of
                  16): FP64 ILU(0)-GMRES(30) and ILU(0)-BiCGStab, and SpMV rates
-  roofline       FP32 ILU(0) application of the headline solve (ticketed L and U sweeps, the
-                 dominant work of an inner step): algorithmic bytes / average application
+  roofline       FP32 ILU(0) application of the headline solve (L and U sweeps, the dominant
+                 work of an inner step): algorithmic bytes / average application
                  time (HIP events on the library stream) against 8 TB/s; roofline_spmv the
                  same for the FP32 SELL-C-sigma SpMV
   cpu_baseline   the reference's own OpenMP AbmcIluGmresm (oracle/_ref) on the host cores,
@@ -423,10 +423,10 @@ def main():
             "kernels": c2["kernels"], **{k: {q: v[q] for q in keep if q in v} for k, v in c2.items() if k != "kernels"}},
         "setup_s": {"abmc": sy8["t_abmc"], "reorder": sy8["t_reorder"], "upload": d8.t_upload,
                     "device_ilu0_split_layout": d8.t_ilu},
-        # the dominant kernels of the headline solve: the FP32 ILU(0) application (ticketed L
-        # sweep + U sweep; with the split inner form they are most of an inner step), and the
+        # the dominant kernels of the headline solve: the FP32 ILU(0) application (one L sweep
+        # and one U sweep; with the split inner form they are most of an inner step), and the
         # FP32 SELL SpMV beside it
-        "roofline": {"kernel": "ilu0_apply_f32: ticketed L + U sweeps (vardiff7 128^3, ABMC blocks of 8)",
+        "roofline": {"kernel": "ilu0_apply_f32: L + U sweeps, per-XCD ticket streams (vardiff7 128^3, ABMC blocks of 8)",
                      "bound": "hbm", "achieved": round(kern["ilu_apply_f32_GBps"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(kern["ilu_apply_f32_GBps"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profile("ilu_apply_f32_vardiff128.md")},
