@@ -586,7 +586,10 @@ static int gmres(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
     const double tol = p.nrm_b * p.eps;
     const bool prec = M.kind != 0 || M.learned;
     const bool use_cgs = p.ortho == SENK_CGS2;
-    const int B = red_grid(N);
+    // at most 512 workgroups in this driver's passes: each Gram-Schmidt step's consumer sums
+    // every partial, and with 1024 of them the FP64 GMRES(30) at 128^3 took 125.9 ms, with
+    // 512 121.8 ms (256: 132.3)
+    const int B = std::min(red_grid(N), 512);
     int total = p.outer * m, converged = 0, breakdown = 0, cycles = 0;
     int hstate[4] = {0, 0, 0, 0};
     for (int i = 0; i < p.outer; i++) {
