@@ -1051,6 +1051,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(int N, const T *__restri
     const double omega = sum_partials(Pin, B) / sum_partials(Pin + kPartLd, B);
     if (blockIdx.x == 0 && threadIdx.x == 0) sc[1] = omega;
     const T a = (T)sc[0], w = (T)omega;
+    const bool ks_is_s = Ks == s;  // the split form passes s for both: read it once
     double ss = 0, rr = 0;
     SENK_FOR4(i0, N) {
         T xv[4], t[4], rv[4];
@@ -1062,8 +1063,13 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(int N, const T *__restri
 #pragma unroll
         for (int u = 0; u < 4; u++) xv[u] = xv[u] + w * t[u];
         put4(x, i0, N, xv);
+        if (ks_is_s) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) rv[u] = t[u];
+        } else {
+            get4(s, i0, N, rv);
+        }
         get4(AKs, i0, N, t);
-        get4(s, i0, N, rv);
 #pragma unroll
         for (int u = 0; u < 4; u++) rv[u] = (-w) * t[u] + rv[u];
         put4(r, i0, N, rv);
@@ -1083,7 +1089,12 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_p(int N, const T *__restrict__ AKp, const T *__restrict__ r,
                                                    double *sc, int parity, const double *Pin, int B, double tol,
                                                    double nrm_b, int iter, int *state, double *hist, int hcap,
-                                                   T *p, unsigned long long *mark, unsigned seq) {
+                                                   T *p, unsigned long long *mark, unsigned seq,
+                                                   const double *dtol = nullptr) {
+    if (dtol) {  // tolerance and norm of the right-hand side formed on the device
+        tol = dtol[0];
+        nrm_b = dtol[1];
+    }
     if (state[0]) {
         if (mark && blockIdx.x == 0 && threadIdx.x == 0) write_mark(mark, seq, 1);
         return;
@@ -1128,13 +1139,26 @@ __global__ void k_state_reset(int *state) {
         state[2] = 0;
     }
 }
-__global__ __launch_bounds__(kBlock) void k_bicg_init(const double *P, int B, double *sc, int *state) {
+// rr = (r, rstr); with dtol: dtol = {eps ||r||, ||r||} (r = rstr); a zero r stops as
+// converged, a non-finite one as a breakdown
+__global__ __launch_bounds__(kBlock) void k_bicg_init(const double *P, int B, double *sc, int *state,
+                                                      double eps = 0, double *dtol = nullptr) {
     const double rr = sum_partials(P, B);
     if (threadIdx.x != 0) return;
     sc[2] = rr;
     state[0] = 0;
     state[1] = 0;
     state[2] = 0;
+    if (dtol) {
+        dtol[0] = eps * sqrt(rr);
+        dtol[1] = sqrt(rr);
+        if (!isfinite(rr)) {
+            state[0] = 1;
+            state[2] = 1;
+        } else if (!(rr > 0)) {
+            state[0] = 1;
+        }
+    }
 }
 // Split form for a D-ILU factorization (Eisenstat's trick).  With M = L U (L unit lower,
 // U upper with diagonal D) and A = L D + U - E, E = 2 D - diag(A):
@@ -1246,13 +1270,11 @@ static int bicgstab_split(Ctx *c, const MatView &A, const PrecView &M, const flo
     SENK_HIP(hipMemsetAsync(y, 0, sizeof(float) * N, c->stream));
     k_bicg_alpha<float><<<B, kBlock, 0, c->stream>>>(N, r, rs, red_slot(c, kSlotA), state);
     SENK_LAUNCH_CHECK();
-    k_bicg_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotA), B, sc, state);
+    double *dtol = sc + 8;  // {tolerance, ||L^-1 b||}, formed on the device (no host round trip)
+    k_bicg_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotA), B, sc, state, p.eps, dtol);
     SENK_LAUNCH_CHECK();
-    double rr0 = 0;
-    SENK_TRY(read_scalar(c, sc + 2, &rr0));
-    const double nrm_c = std::sqrt(rr0), tol = p.eps * nrm_c;
     poll.reset();
-    bool stop = !(nrm_c > 0);
+    bool stop = false;
     for (int i = 0; i < p.max_iter && !stop; i++) {
         const int par = i & 1;
         SENK_TRY(dilu_product<1>(c, M, pp, v, t, w, rs, red_slot(c, kSlotA), state));
@@ -1264,15 +1286,15 @@ static int bicgstab_split(Ctx *c, const MatView &A, const PrecView &M, const flo
         SENK_LAUNCH_CHECK();
         unsigned seq = 0;
         unsigned long long *mark = poll.dev_mark(&seq);
-        k_bicg_p<float><<<B, kBlock, 0, c->stream>>>(N, v, r, sc, par, red_slot(c, kSlotC), B, tol, nrm_c, i, state,
-                                                     dhist, hcap, pp, mark, seq);
+        k_bicg_p<float><<<B, kBlock, 0, c->stream>>>(N, v, r, sc, par, red_slot(c, kSlotC), B, 0.0, 1.0, i, state,
+                                                     dhist, hcap, pp, mark, seq, dtol);
         SENK_LAUNCH_CHECK();
         SENK_TRY(poll.poll(kLag, &stop));
     }
     // x = U^-1 y (after the stop flag is set: no flag for this sweep)
     SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, y, x, nullptr));
-    int hstate[4] = {1, 0, 0, 0};
-    if (nrm_c > 0) SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+    int hstate[4];
+    SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
     float ms = 0;
     SENK_TRY(tm.stop(c, &ms));
     const int converged = hstate[0] && !hstate[2];
