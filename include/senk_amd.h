@@ -40,8 +40,10 @@ enum senk_status {
     SENK_ERR_ALLOC = 8,          /* allocation failure (senk_utils.hpp:26,38,51)               */
     SENK_ERR_UNSUPPORTED = 9,    /* shape outside what the kernels support                     */
     SENK_ERR_SCHEDULE = 10,      /* a factor entry couples two blocks scheduled together       */
-    SENK_ERR_TIMEOUT = 11        /* a single-launch triangular solve gave up waiting on an
+    SENK_ERR_TIMEOUT = 11,       /* a single-launch triangular solve gave up waiting on an
                                     earlier level (the result of that solve is invalid)      */
+    SENK_ERR_IO = 12,            /* a file could not be opened or read                         */
+    SENK_ERR_FORMAT = 13         /* a file is not a MatrixMarket matrix the reader supports    */
 };
 const char *senk_status_string(int status);
 /* Last HIP error text recorded by the library (thread-local), for diagnostics. */
@@ -210,6 +212,27 @@ int senk_csr2bcsr(const double *val, const int *cind, const int *rptr,
 int senk_bcsr2csr(const double *bval, const int *bcind, const int *brptr,
                   double **val, int **cind, int **rptr, int N, int bnl, int bnw, int *nnz); /* :213-241 */
 int senk_padding(double **val, int **cind, int **rptr, int size, int *N);          /* :243-255 */
+/* RemoveZeros (:29-43): entries equal to zero leave, in place; val/cind reallocated. */
+int senk_remove_zeros(double **val, int **cind, int *rptr, int N);
+/* Expand (:335-371): the full matrix from a stored triangle, key "L" (lower part with the
+ * diagonal stored, as in a symmetric MatrixMarket file) or "U". */
+int senk_expand(const double *tval, const int *tcind, const int *trptr,
+                double **val, int **cind, int **rptr, int N, const char *key);
+/* Duplicate (:373-389): a copy. */
+int senk_duplicate(const double *tval, const int *tcind, const int *trptr,
+                   double **val, int **cind, int **rptr, int N);
+
+/* ------------------------------------------------------------------ MatrixMarket
+ * io::ReadMatrixMarket (senk_io.hpp:41-103).  A real or integer coordinate file,
+ * "general" (*shape = SENK_SHAPE_UNSYM) or "symmetric" (*shape = SENK_SHAPE_SYM: the
+ * stored triangle is returned; senk_expand(..., "L") completes it).  removeZeros drops
+ * every stored zero.  CSR output, rows in order and columns ascending, malloc'd.
+ * SENK_ERR_IO if the file cannot be read; SENK_ERR_FORMAT for a banner other than
+ * real/integer coordinate general/symmetric (pattern, complex, hermitian, skew-symmetric,
+ * array), a bad size line, a short or out-of-range entry list. */
+enum senk_shape { SENK_SHAPE_SYM = 0, SENK_SHAPE_UNSYM = 1 };
+int senk_read_matrix_market(const char *filename, double **val, int **cind, int **rptr,
+                            int *N, int *M, int *shape, int removeZeros);
 /* Split (:257-333): key "LD-U", "L-DU", "L-D-U" or "LU-D".  Unused outputs may be NULL. */
 int senk_split(const double *val, const int *cind, const int *rptr,
                double **lval, int **lcind, int **lrptr, double **uval, int **ucind, int **urptr,

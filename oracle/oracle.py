@@ -96,6 +96,7 @@ _REF_PROTOS = {
     "ref_bicgstab_d": (None, "pppppdiid"), "ref_ilu_bicgstab_d": (None, "ppppppppp" + "ppdiid"),
     "ref_ilub_bicgstab_d": ("i", "ppppppppp" + "ppdiid" + "ii"),
     "ref_relative_residual_error": (None, "pppppi"), "ref_flush": (None, ""),
+    "ref_read_mm": ("i", "pippppii"), "ref_expand_d": ("i", "pppippppi"), "ref_remove_zeros_d": ("i", "pppi"),
 }
 _K = {"p": C.c_void_p, "i": C.c_int, "d": C.c_double, "f": C.c_float, None: None}
 

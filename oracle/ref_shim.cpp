@@ -8,6 +8,7 @@
 // counterpart allocates outputs (T** arguments) copy them into caller buffers of a
 // stated capacity and free the reference's allocation; they return the produced
 // length, or -(needed length) when the capacity is too small.
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -167,6 +168,47 @@ EXPORT int ref_bcsr2csr_d(double *bval, int *bcind, int *brptr, double *val, int
     return res;
 }
 // In/out arrays must already have room for the padded size (N rounded up to size).
+// io::ReadMatrixMarket: dims = {N, M, shape (0 Sym, 1 Unsym)}; returns nnz, -(nnz + 1)
+// when a capacity is short, INT_MIN when the reader refused the file.
+EXPORT int ref_read_mm(const char *fn, int removeZeros, double *val, int *cind, int *rptr, int *dims, int cap,
+                       int rcap) {
+    double *v; int *c, *r; int N = 0, M = 0;
+    senk::io::Shape sh;
+    if (!senk::io::ReadMatrixMarket(fn, &v, &c, &r, &N, &M, &sh, removeZeros != 0)) return INT_MIN;
+    dims[0] = N; dims[1] = M; dims[2] = sh == senk::io::Sym ? 0 : 1;
+    const int nnz = r[N];
+    if (nnz > cap || N + 1 > rcap) { free(v); free(c); free(r); return -(nnz + 1); }
+    std::memcpy(val, v, sizeof(double) * nnz);
+    std::memcpy(cind, c, sizeof(int) * nnz);
+    std::memcpy(rptr, r, sizeof(int) * (N + 1));
+    free(v); free(c); free(r);
+    return nnz;
+}
+// Expand (key "L" or "U"): returns the entry count, -(count + 1) when cap is short.
+EXPORT int ref_expand_d(double *tval, int *tcind, int *trptr, int N, const char *key, double *val, int *cind,
+                        int *rptr, int cap) {
+    double *v; int *c, *r;
+    senk::matrix::Expand<double>(tval, tcind, trptr, &v, &c, &r, N, key);
+    const int nnz = r[N];
+    if (nnz > cap) { free(v); free(c); free(r); return -(nnz + 1); }
+    std::memcpy(val, v, sizeof(double) * nnz);
+    std::memcpy(cind, c, sizeof(int) * nnz);
+    std::memcpy(rptr, r, sizeof(int) * (N + 1));
+    free(v); free(c); free(r);
+    return nnz;
+}
+// RemoveZeros on malloc'd copies; outputs into the caller's arrays (same capacity as
+// the input); returns the new entry count.
+EXPORT int ref_remove_zeros_d(double *val, int *cind, int *rptr, int N) {
+    const int nnz = rptr[N];
+    double *v = (double *)malloc(sizeof(double) * (nnz ? nnz : 1)); std::memcpy(v, val, sizeof(double) * nnz);
+    int *c = (int *)malloc(sizeof(int) * (nnz ? nnz : 1)); std::memcpy(c, cind, sizeof(int) * nnz);
+    senk::matrix::RemoveZeros<double>(&v, &c, rptr, N);
+    std::memcpy(val, v, sizeof(double) * rptr[N]);
+    std::memcpy(cind, c, sizeof(int) * rptr[N]);
+    free(v); free(c);
+    return rptr[N];
+}
 EXPORT int ref_padding_d(double *val, int *cind, int *rptr, int size, int N) {
     int nnz = rptr[N];
     double *v = (double *)malloc(sizeof(double) * nnz); std::memcpy(v, val, sizeof(double) * nnz);

@@ -13,6 +13,7 @@ device is present.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import re
 from pathlib import Path
 
@@ -26,12 +27,13 @@ STATUS = {
     0: "SENK_OK", 1: "SENK_ERR_ARG", 2: "SENK_ERR_HIP", 3: "SENK_ERR_ZERO_PIVOT",
     4: "SENK_ERR_DIVISIBILITY", 5: "SENK_ERR_BREAKDOWN", 6: "SENK_ERR_NOT_CONVERGED",
     7: "SENK_ERR_NO_DEVICE", 8: "SENK_ERR_ALLOC", 9: "SENK_ERR_UNSUPPORTED", 10: "SENK_ERR_SCHEDULE",
-    11: "SENK_ERR_TIMEOUT",
+    11: "SENK_ERR_TIMEOUT", 12: "SENK_ERR_IO", 13: "SENK_ERR_FORMAT",
 }
 SENK_GMRES, SENK_FGMRES, SENK_GCR, SENK_FGCR, SENK_BICGSTAB = range(5)
 SENK_MGS, SENK_CGS2 = 0, 1
 SENK_ILU0, SENK_ILUP, SENK_ILUB = 0, 1, 2
 SENK_FACTOR_F64, SENK_FACTOR_F32, SENK_FACTOR_F16 = 0, 1, 2
+SENK_SHAPE_SYM, SENK_SHAPE_UNSYM = 0, 1
 
 
 class SenkError(RuntimeError):
@@ -133,7 +135,7 @@ def lib():
 
 def check(fn: str, status: int):
     if status != 0:
-        detail = lib().senk_last_error().decode() if status == 2 else ""
+        detail = lib().senk_last_error().decode() if status in (2, 12, 13) else ""
         raise SenkError(fn, status, detail)
 
 
@@ -428,6 +430,46 @@ def csr2csc(val, cind, rptr, N, M):
     call("senk_csr2csc", _ptr(_f64(val)), _ptr(_i32(cind)), _ptr(_i32(rptr)), C.byref(v), C.byref(r), C.byref(c), N, M)
     nnz = int(np.asarray(rptr)[N])
     return _take(v, nnz, np.float64), _take(r, nnz, np.int32), _take(c, M + 1, np.int32)
+
+
+def read_matrix_market(path, remove_zeros=True):
+    """io::ReadMatrixMarket (senk_io.hpp:41-103): (val, cind, rptr, N, M, shape), shape
+    SENK_SHAPE_SYM (the stored lower triangle; complete it with expand) or SENK_SHAPE_UNSYM."""
+    v, c, r = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    N, M, sh = C.c_int(), C.c_int(), C.c_int()
+    call("senk_read_matrix_market", os.fsencode(str(path)), C.byref(v), C.byref(c), C.byref(r), C.byref(N), C.byref(M),
+         C.byref(sh), int(bool(remove_zeros)))
+    rptr = _take(r, N.value + 1, np.int32)
+    nnz = int(rptr[-1])
+    return _take(v, nnz, np.float64), _take(c, nnz, np.int32), rptr, N.value, M.value, sh.value
+
+
+def remove_zeros(val, cind, rptr, N):
+    """RemoveZeros (senk_matrix.hpp:29-43); returns (val, cind, rptr)."""
+    pv, pc = _malloc_copy(_f64(val)), _malloc_copy(_i32(cind))
+    rp = _i32(rptr).copy()
+    call("senk_remove_zeros", C.byref(pv), C.byref(pc), _ptr(rp), N)
+    nnz = int(rp[N])
+    return _take(pv, nnz, np.float64), _take(pc, nnz, np.int32), rp
+
+
+def expand(val, cind, rptr, N, key="L"):
+    """Expand (senk_matrix.hpp:335-371): the full matrix from a stored triangle."""
+    v, c, r = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    call("senk_expand", _ptr(_f64(val)), _ptr(_i32(cind)), _ptr(_i32(rptr)), C.byref(v), C.byref(c), C.byref(r), N,
+         key.encode())
+    rp = _take(r, N + 1, np.int32)
+    nnz = int(rp[-1])
+    return _take(v, nnz, np.float64), _take(c, nnz, np.int32), rp
+
+
+def duplicate(val, cind, rptr, N):
+    """Duplicate (senk_matrix.hpp:373-389)."""
+    v, c, r = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    call("senk_duplicate", _ptr(_f64(val)), _ptr(_i32(cind)), _ptr(_i32(rptr)), C.byref(v), C.byref(c), C.byref(r), N)
+    rp = _take(r, N + 1, np.int32)
+    nnz = int(rp[-1])
+    return _take(v, nnz, np.float64), _take(c, nnz, np.int32), rp
 
 
 def solve_opts(**kw) -> senk_solve_opts:

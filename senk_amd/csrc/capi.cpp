@@ -185,6 +185,8 @@ const char *senk_status_string(int s) {
         case SENK_ERR_UNSUPPORTED: return "unsupported shape";
         case SENK_ERR_SCHEDULE: return "factor entry couples two blocks of one launch";
         case SENK_ERR_TIMEOUT: return "triangular solve timed out waiting on an earlier level";
+        case SENK_ERR_IO: return "file could not be read";
+        case SENK_ERR_FORMAT: return "file is not a supported MatrixMarket matrix";
         default: return "unknown status";
     }
 }

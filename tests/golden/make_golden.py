@@ -156,6 +156,61 @@ def digest_case(name, gen, bsize, isSym=0, ilup=None):
     print("wrote", name, meta)
 
 
+MM_CASES = {  # name: (kind, seed, n, m)
+    "general": ("general", 11, 36, None), "rect_wide": ("general", 12, 24, 31),
+    "rect_tall": ("general", 13, 33, 21), "symmetric": ("symmetric", 14, 40, None),
+    "integer": ("integer", 15, 30, None),
+}
+
+
+def _ref_read(path, rz, cap=4096):
+    ref = O.ref()
+    v, c, r, d = np.zeros(cap), np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(3, np.int32)
+    nnz = capture_value(lambda: ref.ref_read_mm(str(path).encode(), rz, P(v), P(c), P(r), P(d), cap, cap))
+    assert nnz >= 0, nnz
+    return v[:nnz].copy(), c[:nnz].copy(), r[:d[0] + 1].copy(), d.copy()
+
+
+def capture_value(fn):
+    out = []
+    capture(lambda: out.append(fn()))
+    return out[0]
+
+
+def matrix_market_case():
+    """MatrixMarket files made by tests/helpers.mm_text, read by the reference's
+    io::ReadMatrixMarket with and without zero removal; its RemoveZeros on the kept-zeros
+    read, and its Expand ("L" on the symmetric file's triangle, "U" on the transpose)."""
+    from tests.helpers import mm_text
+    ref = O.ref()
+    fx = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name, (kind, seed, n, m) in MM_CASES.items():
+            text = mm_text(kind, seed, n, m)
+            path = Path(d) / f"{name}.mtx"
+            path.write_text(text)
+            fx[f"{name}_text"] = np.frombuffer(text.encode(), dtype=np.uint8)
+            for rz in (1, 0):
+                v, c, r, dims = _ref_read(path, rz)
+                fx.update({f"{name}_rz{rz}_val": v, f"{name}_rz{rz}_cind": c, f"{name}_rz{rz}_rptr": r,
+                           f"{name}_rz{rz}_dims": dims})
+            v, c, r, dims = _ref_read(path, 0)
+            N = int(dims[0])
+            rr = r.copy()
+            nnz = ref.ref_remove_zeros_d(P(v), P(c), P(rr), N)
+            fx.update({f"{name}_rmz_val": v[:nnz].copy(), f"{name}_rmz_cind": c[:nnz].copy(), f"{name}_rmz_rptr": rr})
+            if kind == "symmetric":
+                v, c, r, dims = _ref_read(path, 1)
+                cap = 2 * len(v)
+                for key, (tv, tc, tr) in (("L", (v, c, r)), ("U", tuple(S.csr2csc(v, c, r, N, N)[i] for i in (0, 1, 2)))):
+                    ev, ec, er = np.zeros(cap), np.zeros(cap, np.int32), np.zeros(N + 1, np.int32)
+                    k = ref.ref_expand_d(P(tv), P(tc), P(tr), N, key.encode(), P(ev), P(ec), P(er), cap)
+                    assert k >= 0
+                    fx.update({f"{name}_expand{key}_val": ev[:k].copy(), f"{name}_expand{key}_cind": ec[:k].copy(),
+                               f"{name}_expand{key}_rptr": er})
+    np.savez_compressed(OUT / "matrix_market.npz", **fx)
+
+
 def main():
     if O.ref() is None:
         sys.exit("oracle/_ref/libsenkref.so is not built (make -C oracle with the reference tree present)")
@@ -165,6 +220,7 @@ def main():
     small_case("elasticity_3", *S.gen_elasticity3(3, 4), bsizes=(3, 9), ps=(1,))
     small_case("vardiff7_6", *S.gen_vardiff7(6, 3, 4.0, 1.5), bsizes=(4, 12), ps=(1,))
     poisson32_case()
+    matrix_market_case()
     if "--full" in sys.argv:
         digest_case("convdiff27_128_abmc16", lambda: S.gen_convdiff27(128, 10.0), 16)
         digest_case("vardiff7_128_abmc16", lambda: S.gen_vardiff7(128, 1, 8.0, 1.5), 16)

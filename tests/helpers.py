@@ -54,3 +54,62 @@ def relerr(a, b):
     a, b = np.asarray(a, float), np.asarray(b, float)
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def mm_text(kind, seed, n=40, m=None):
+    """A MatrixMarket coordinate file (text) for the ingestion tests.
+
+    kind "general": a nonsymmetric n x m matrix with a nonzero diagonal; "symmetric": the
+    lower triangle of a symmetric n x n matrix; "integer": a general matrix with integer
+    values.  Entries come in shuffled order with a few explicit zeros off the diagonal,
+    comment lines, a blank line between entries and values written in several notations.
+    Every row keeps a nonzero entry (the diagonal in the square part); no coordinate
+    repeats.
+    """
+    rng = np.random.default_rng(seed)
+    m = n if m is None else m
+    A = np.where(rng.random((n, m)) < 0.12, rng.normal(size=(n, m)), 0.0)
+    if kind == "integer":
+        A = np.round(A * 7)
+    k = min(n, m)
+    A[np.arange(k), np.arange(k)] = rng.uniform(2, 4, k)
+    if kind == "integer":
+        A[np.arange(k), np.arange(k)] = np.round(A[np.arange(k), np.arange(k)])
+    if n > m:
+        A[np.arange(m, n), rng.integers(0, m, n - m)] = 1.5  # rows beyond the square part
+    if kind == "symmetric":
+        A = np.tril(A) + np.tril(A, -1).T
+        r, c = np.nonzero(np.tril(A))
+    else:
+        r, c = np.nonzero(A)
+    v = A[r, c]
+    # explicit zeros off the diagonal
+    zr = rng.integers(1, n, 6)
+    zc = np.array([rng.integers(0, i) for i in zr]) if kind == "symmetric" else rng.integers(0, m, 6)
+    keep = (zr != zc) & (A[zr, np.minimum(zc, m - 1)] == 0)
+    z = np.unique(np.stack([zr[keep], zc[keep]], 1), axis=0).reshape(-1, 2)
+    r = np.concatenate([r, z[:, 0]])
+    c = np.concatenate([c, z[:, 1]])
+    v = np.concatenate([v, np.zeros(len(z))])
+    perm = rng.permutation(len(v))
+    r, c, v = r[perm], c[perm], v[perm]
+    field = "integer" if kind == "integer" else "real"
+    sym = "symmetric" if kind == "symmetric" else "general"
+    out = [f"%%MatrixMarket matrix coordinate {field} {sym}", "% generated for the ingestion tests", "%"]
+    out.append(f"{n} {m} {len(v)}")
+    for e, (i, j, x) in enumerate(zip(r, c, v)):
+        if kind == "integer":
+            s = str(int(x))
+        elif e % 5 == 1:
+            s = f"{x:.17e}"
+        elif e % 5 == 2:
+            s = ("+" if x >= 0 else "") + repr(float(x))
+        elif e % 5 == 3:
+            s = f"{x:.6g}"
+        else:
+            s = repr(float(x))
+        sep = "   " if e % 7 == 3 else " "
+        out.append(f"{i + 1}{sep}{j + 1} {s}")
+        if e == len(v) // 2:
+            out.append("")
+    return "\n".join(out) + "\n"
