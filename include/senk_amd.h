@@ -233,6 +233,9 @@ int senk_duplicate(const double *tval, const int *tcind, const int *trptr,
 enum senk_shape { SENK_SHAPE_SYM = 0, SENK_SHAPE_UNSYM = 1 };
 int senk_read_matrix_market(const char *filename, double **val, int **cind, int **rptr,
                             int *N, int *M, int *shape, int removeZeros);
+/* The banner and size line only (senk_io.hpp:52-68: what the reader prints before the
+ * entries): rows, columns, stored entries (before zero removal) and the shape. */
+int senk_matrix_market_size(const char *filename, int *N, int *M, long long *entries, int *shape);
 /* Split (:257-333): key "LD-U", "L-DU", "L-D-U" or "LU-D".  Unused outputs may be NULL. */
 int senk_split(const double *val, const int *cind, const int *rptr,
                double **lval, int **lcind, int **lrptr, double **uval, int **ucind, int **urptr,

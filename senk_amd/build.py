@@ -2,7 +2,9 @@
 
 Compiles every source under senk_amd/csrc with hipcc into senk_amd/_build/*.o (in
 parallel) and links senk_amd/libsenk_amd.so in-tree, so the library travels with the
-source tree.  Run as ``python -m senk_amd.build`` or through __graft_entry__.build().
+source tree; then builds the command-line workflow senk_amd/senk_main (host C++ over the
+library, senk_amd/cli/senk_main.cpp).  Run as ``python -m senk_amd.build`` or through
+__graft_entry__.build().
 """
 from __future__ import annotations
 
@@ -17,6 +19,9 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libsenk_amd.so"
+CLI_SRC = PKG / "cli" / "senk_main.cpp"
+CLI = PKG / "senk_main"
+CXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("SENK_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -58,9 +63,22 @@ def build(verbose: bool = False) -> Path:
     if proc.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
+
+
+def build_cli() -> Path:
+    """The command-line workflow, linked against the in-tree library (rpath $ORIGIN)."""
+    tmp = CLI.with_suffix(".tmp")
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(CLI_SRC), f"-L{PKG}", "-lsenk_amd",
+           "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"build of {CLI.name} failed: {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -56,9 +56,84 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+// Banner ("%%MatrixMarket matrix coordinate <real|integer|double> <general|symmetric>",
+// tokens case-insensitive), comment and blank lines, then "rows cols entries"; the
+// cursor is left after the size line.  *sh: 0 symmetric, 1 general.
+int read_head(Cursor &c, int *sh, long long *n, long long *m, long long *pe) {
+    const char *eol = std::find(c.p, c.end, '\n');
+    std::vector<std::string> tok;
+    {
+        Cursor b{c.p, eol};
+        while (true) {
+            b.skip_space();
+            if (b.p >= b.end) break;
+            const char *s = b.p;
+            while (b.p < b.end && !std::isspace((unsigned char)*b.p)) b.p++;
+            tok.push_back(lower(std::string(s, b.p)));
+        }
+    }
+    if (tok.size() < 5 || tok[0] != "%%matrixmarket" || tok[1] != "matrix" || tok[2] != "coordinate")
+        return fail(SENK_ERR_FORMAT, "MatrixMarket: banner is not '%%MatrixMarket matrix coordinate ...'");
+    if (tok[3] != "real" && tok[3] != "integer" && tok[3] != "double")
+        return fail(SENK_ERR_FORMAT, "MatrixMarket: field '" + tok[3] + "' is not real or integer");
+    if (tok[4] == "general") *sh = 1;
+    else if (tok[4] == "symmetric") *sh = 0;
+    else return fail(SENK_ERR_FORMAT, "MatrixMarket: symmetry '" + tok[4] + "' is not general or symmetric");
+    const char *p = eol < c.end ? eol + 1 : c.end;
+    while (p < c.end) {
+        const char *e = std::find(p, c.end, '\n');
+        const char *q = p;
+        while (q < e && std::isspace((unsigned char)*q)) q++;
+        if (q < e && *q != '%') break;
+        p = e < c.end ? e + 1 : c.end;
+    }
+    c.p = p;
+    if (!c.read_long(n) || !c.read_long(m) || !c.read_long(pe) || *n < 0 || *m < 0 || *pe < 0)
+        return fail(SENK_ERR_FORMAT, "MatrixMarket: size line is not 'rows cols entries'");
+    if (*n >= INT_MAX || *m >= INT_MAX || *pe > INT_MAX)
+        return fail(SENK_ERR_UNSUPPORTED, "MatrixMarket: sizes beyond 32-bit indices");
+    return SENK_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+// The banner and size line of a MatrixMarket file, as the reference's reader reports
+// them before it reads the entries (senk_io.hpp:52-68): rows, columns, stored entries
+// and the shape (0 symmetric, 1 general).  Reads only the head of the file.
+int senk_matrix_market_size(const char *filename, int *N, int *M, long long *entries, int *shape) {
+    if (!filename || !N || !M || !entries || !shape) return SENK_ERR_ARG;
+    FILE *f = std::fopen(filename, "rb");
+    if (!f) return fail(SENK_ERR_IO, std::string("could not open ") + filename);
+    // banner, comments and the size line: read lines until the first one that is
+    // neither a comment nor blank after the banner
+    std::string head;
+    char line[4096];
+    bool new_line = true, size_seen = false;
+    long long lines = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        const bool starts = new_line;  // a chunk that begins a line (long lines come in pieces)
+        head += line;
+        new_line = head.back() == '\n';
+        if (starts && ++lines > 1) {
+            const char *q = line;
+            while (*q && std::isspace((unsigned char)*q)) q++;
+            if (*q && *q != '%') size_seen = true;
+        }
+        if (size_seen && new_line) break;
+    }
+    std::fclose(f);
+    Cursor c{head.data(), head.data() + head.size()};
+    int sh;
+    long long n, m, pe;
+    if (int st = read_head(c, &sh, &n, &m, &pe)) return st;
+    *N = (int)n;
+    *M = (int)m;
+    *entries = pe;
+    *shape = sh;
+    return SENK_OK;
+}
 
 // io::ReadMatrixMarket (senk_io.hpp:41-103).  The banner must name a real or integer
 // coordinate matrix, "general" (*shape = 1, the reference's Unsym) or "symmetric"
@@ -81,43 +156,10 @@ int senk_read_matrix_market(const char *filename, double **val, int **cind, int 
         std::fclose(f);
         if (err) return fail(SENK_ERR_IO, std::string("could not read ") + filename);
     }
-    const char *p = buf.data(), *end = buf.data() + buf.size();
-    // banner: %%MatrixMarket matrix coordinate <real|integer|double> <general|symmetric>
-    const char *eol = std::find(p, end, '\n');
-    std::vector<std::string> tok;
-    {
-        Cursor c{p, eol};
-        while (true) {
-            c.skip_space();
-            if (c.p >= c.end) break;
-            const char *s = c.p;
-            while (c.p < c.end && !std::isspace((unsigned char)*c.p)) c.p++;
-            tok.push_back(lower(std::string(s, c.p)));
-        }
-    }
-    if (tok.size() < 5 || tok[0] != "%%matrixmarket" || tok[1] != "matrix" || tok[2] != "coordinate")
-        return fail(SENK_ERR_FORMAT, "MatrixMarket: banner is not '%%MatrixMarket matrix coordinate ...'");
-    if (tok[3] != "real" && tok[3] != "integer" && tok[3] != "double")
-        return fail(SENK_ERR_FORMAT, "MatrixMarket: field '" + tok[3] + "' is not real or integer");
+    Cursor c{buf.data(), buf.data() + buf.size()};
     int sh;
-    if (tok[4] == "general") sh = 1;
-    else if (tok[4] == "symmetric") sh = 0;
-    else return fail(SENK_ERR_FORMAT, "MatrixMarket: symmetry '" + tok[4] + "' is not general or symmetric");
-    // comment and blank lines, then "rows cols entries"
-    p = eol < end ? eol + 1 : end;
-    while (p < end) {
-        const char *e = std::find(p, end, '\n');
-        const char *q = p;
-        while (q < e && std::isspace((unsigned char)*q)) q++;
-        if (q < e && *q != '%') break;
-        p = e < end ? e + 1 : end;
-    }
-    Cursor c{p, end};
     long long n, m, pe;
-    if (!c.read_long(&n) || !c.read_long(&m) || !c.read_long(&pe) || n < 0 || m < 0 || pe < 0)
-        return fail(SENK_ERR_FORMAT, "MatrixMarket: size line is not 'rows cols entries'");
-    if (n >= INT_MAX || m >= INT_MAX || pe > INT_MAX)
-        return fail(SENK_ERR_UNSUPPORTED, "MatrixMarket: sizes beyond 32-bit indices");
+    if (int st = read_head(c, &sh, &n, &m, &pe)) return st;
     std::vector<int> row, col;
     std::vector<double> v;
     row.reserve((size_t)pe);

@@ -211,6 +211,74 @@ def matrix_market_case():
     np.savez_compressed(OUT / "matrix_market.npz", **fx)
 
 
+def workflow_case(name):
+    """The reference's end-to-end workflow (main.cpp:9-187) on a MatrixMarket file, run
+    step by step through the compiled reference; its stdout lines (the solve's elapsed
+    time replaced by "<elapsed>") and the file's digest go to workflow_<name>.json."""
+    from tests.helpers import workflow_text
+    ref = O.ref()
+    text = workflow_text(name)
+    out = []
+    with tempfile.TemporaryDirectory() as d:
+        path = Path(d) / f"{name}.mtx"
+        path.write_text(text)
+        cap = 1 << 22
+        v, c, r, dims = np.zeros(cap), np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(3, np.int32)
+        res = []
+        out.append(capture(lambda: res.append(ref.ref_read_mm(str(path).encode(), 1, P(v), P(c), P(r), P(dims), cap,
+                                                              cap))))
+    assert res[0] >= 0
+    N = int(dims[0])
+    v, c, r = v[:res[0]].copy(), c[:res[0]].copy(), r[:N + 1].copy()
+    out.append("# %d\n" % r[N])
+    assert ref.ref_check_structure_d(P(v), P(c), P(r), N) == 1
+    if int(dims[2]) == 0:
+        cap = 2 * len(v)
+        ev, ec, er = np.zeros(cap), np.zeros(cap, np.int32), np.zeros(N + 1, np.int32)
+        k = ref.ref_expand_d(P(v), P(c), P(r), N, b"L", P(ev), P(ec), P(er), cap)
+        v, c, r = ev[:k].copy(), ec[:k].copy(), er
+    ori_N = N
+    Np = (N + 127) // 128 * 128
+    pv, pc, pr = np.zeros(len(v) + Np - N), np.zeros(len(v) + Np - N, np.int32), np.zeros(Np + 1, np.int32)
+    pv[:len(v)], pc[:len(v)], pr[:N + 1] = v, c, r
+    N = ref.ref_padding_d(P(pv), P(pc), P(pr), 128, N)
+    v, c, r = pv, pc, pr
+    out.append("%d\n" % r[N])
+    out.append("%d %d\n" % (N, N))
+    sc, LP, RP = np.zeros(N + 1, np.int32), np.zeros(N, np.int32), np.zeros(N, np.int32)
+    nc = ref.ref_abmc_perm(P(c), P(r), P(sc), P(LP), P(RP), N, 128, 0, 1)
+    ref.ref_reordering_d(P(v), P(c), P(r), P(LP), P(RP), N)
+    out.append("# " + "".join("%d " % (sc[i + 1] - sc[i]) for i in range(nc)) + "\n")
+    out.append("# Reordered!\n")
+    dg = np.zeros(N)
+    ref.ref_get_diag_d(P(v), P(c), P(r), P(dg), N)
+    dg = 1 / np.sqrt(np.abs(dg))
+    ref.ref_scaling_d(P(v), P(c), P(r), P(dg), P(dg), N)
+    cap = 8 * len(v)
+    fv, fc, fr = np.zeros(cap), np.zeros(cap, np.int32), np.zeros(N + 1, np.int32)
+    fn = ref.ref_alloc_block_zero_d(P(v), P(c), P(r), P(fv), P(fc), P(fr), N, 8, 1, cap)
+    assert fn > 0
+    fv, fc = fv[:fn].copy(), fc[:fn].copy()
+    out.append("# Allocated!\n")
+    ref.ref_ilu0_d(P(fv), P(fc), P(fr), N)
+    out.append("# Factrized!\n")
+    lv, lc, lr = np.zeros(fn), np.zeros(fn, np.int32), np.zeros(N + 1, np.int32)
+    uv, uc, ur = np.zeros(fn), np.zeros(fn, np.int32), np.zeros(N + 1, np.int32)
+    ref.ref_split_d(P(fv), P(fc), P(fr), P(lv), P(lc), P(lr), P(uv), P(uc), P(ur), None, N, 1, 1)
+    out.append("# Splited!\n")
+    b = S.gen_uniform(1, N)  # std::mt19937(1), uniform_real_distribution<>(-1, 1)
+    b[ori_N:] = 0.0
+    x = np.zeros(N)
+    nrm_b = ref.ref_nrm2_d(P(b), N)
+    out.append(capture(lambda: ref.ref_ilu_gmresm_d(P(v), P(c), P(r), P(lv), P(lc), P(lr), P(uv), P(uc), P(ur),
+                                                    P(b), P(x), nrm_b, 6000 // 50, 50, N, 1e-8)))
+    out.append("<elapsed>\n")
+    out.append(capture(lambda: ref.ref_relative_residual_error(P(v), P(c), P(r), P(b), P(x), N)))
+    lines = "".join(out).splitlines()
+    doc = {"file_sha256": hashlib.sha256(text.encode()).hexdigest(), "stdout": lines}
+    (OUT / f"workflow_{name}.json").write_text(json.dumps(doc, indent=1) + "\n")
+
+
 def main():
     if O.ref() is None:
         sys.exit("oracle/_ref/libsenkref.so is not built (make -C oracle with the reference tree present)")
@@ -221,6 +289,9 @@ def main():
     small_case("vardiff7_6", *S.gen_vardiff7(6, 3, 4.0, 1.5), bsizes=(4, 12), ps=(1,))
     poisson32_case()
     matrix_market_case()
+    from tests.helpers import WORKFLOW_CASES
+    for name in WORKFLOW_CASES:
+        workflow_case(name)
     if "--full" in sys.argv:
         digest_case("convdiff27_128_abmc16", lambda: S.gen_convdiff27(128, 10.0), 16)
         digest_case("vardiff7_128_abmc16", lambda: S.gen_vardiff7(128, 1, 8.0, 1.5), 16)

@@ -113,3 +113,28 @@ def mm_text(kind, seed, n=40, m=None):
         if e == len(v) // 2:
             out.append("")
     return "\n".join(out) + "\n"
+
+
+def mm_from_csr(rptr, cind, val, N, symmetric=False):
+    """MatrixMarket text of a CSR matrix (values exact: repr round-trips a double); with
+    symmetric=True only the lower triangle is written, as the format stores it."""
+    out = [f"%%MatrixMarket matrix coordinate real {'symmetric' if symmetric else 'general'}"]
+    rows = np.repeat(np.arange(N), np.diff(rptr))
+    keep = cind <= rows if symmetric else np.ones(len(cind), bool)
+    out.append(f"{N} {N} {int(keep.sum())}")
+    out += [f"{i + 1} {j + 1} {float(v)!r}" for i, j, v in zip(rows[keep], cind[keep], val[keep])]
+    return "\n".join(out) + "\n"
+
+
+# inputs of the command-line workflow tests (senk_amd/senk_main): name -> (generator, symmetric file)
+WORKFLOW_CASES = {
+    "poisson7_14_sym": (lambda: S.gen_poisson7(14), True),
+    "convdiff27_10": (lambda: S.gen_convdiff27(10, 10.0), False),
+    "unstructured_1500": (lambda: S.gen_unstructured(1500, 3), False),
+}
+
+
+def workflow_text(name):
+    gen, sym = WORKFLOW_CASES[name]
+    rptr, cind, val = gen()
+    return mm_from_csr(rptr, cind, val, len(rptr) - 1, sym)

@@ -214,3 +214,22 @@ def test_reader_edge_cases(tmp_path):
         f"{i + 1} {i + 1} {s}\n" for i, s in enumerate(vals))
     val = S.read_matrix_market(_write(tmp_path, "v", text))[0]
     assert _same(val, np.array([float(s) for s in vals]))
+
+
+def test_size_query(tmp_path):
+    for name in MM_CASES:
+        text = GOLDEN[f"{name}_text"].tobytes()
+        path = _write(tmp_path, name, text)
+        N, M, E, sh = C.c_int(), C.c_int(), C.c_longlong(), C.c_int()
+        S.call("senk_matrix_market_size", str(path).encode(), C.byref(N), C.byref(M), C.byref(E), C.byref(sh))
+        size_line = [ln for ln in text.decode().splitlines()[1:] if ln.strip() and not ln.startswith("%")][0]
+        assert [N.value, M.value, E.value] == [int(t) for t in size_line.split()]
+        assert sh.value == int(GOLDEN[f"{name}_rz1_dims"][2])
+    # a comment line longer than the reader's line buffer, then the size line
+    text = "%%MatrixMarket matrix coordinate real general\n%" + "x" * 10000 + "\n\n7 5 1\n1 1 2.0\n"
+    path = _write(tmp_path, "long", text)
+    S.call("senk_matrix_market_size", str(path).encode(), C.byref(N), C.byref(M), C.byref(E), C.byref(sh))
+    assert (N.value, M.value, E.value, sh.value) == (7, 5, 1, S.SENK_SHAPE_UNSYM)
+    assert S.read_matrix_market(path)[3:5] == (7, 5)
+    assert S.lib().senk_matrix_market_size(str(tmp_path / "absent").encode(), C.byref(N), C.byref(M), C.byref(E),
+                                           C.byref(sh)) == 12
