@@ -87,9 +87,10 @@ def build_system_kind(kind, n, bsize, seed=1):
 
 
 class Device:
-    """A system on the device: matrix (FP64 + FP32 values, SELL-C-sigma) and ILU(0) (FP64 + FP32)."""
+    """A system on the device: matrix (FP64 + FP32 values, SELL-C-sigma) and ILU(0) (FP64 + FP32);
+    kind=S.SENK_IC0 for the incomplete Cholesky factor instead."""
 
-    def __init__(self, ctx, sy):
+    def __init__(self, ctx, sy, kind=S.SENK_ILU0):
         self.ctx, self.sy, self.N = ctx, sy, sy["N"]
         self.A = C.c_void_p()
         t0 = time.perf_counter()
@@ -97,7 +98,7 @@ class Device:
                C.byref(self.A))
         self.t_upload = time.perf_counter() - t0
         self.M = C.c_void_p()
-        opts = S.senk_ilu_opts(S.SENK_ILU0, 0, 0, 0, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32), 0)
+        opts = S.senk_ilu_opts(kind, 0, 0, 0, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32), 0)
         t0 = time.perf_counter()
         S.call("senk_prec_create_ilu", ctx.h, self.A, S._ptr(sy["rptr"]), S._ptr(sy["cind"]), S._ptr(sy["val"]),
                S._ptr(sy["cptr"]), sy["nc"], sy["bsize"], C.byref(opts), C.byref(self.M))

@@ -396,7 +396,13 @@ int senk_mat_spmv_f64(senk_mat A, const double *dx, double *dy);
 int senk_mat_spmv_f32(senk_mat A, const float *dx, float *dy);
 int senk_mat_nnz(senk_mat A, int *nnz);
 
-enum senk_ilu_kind { SENK_ILU0 = 0, SENK_ILUP = 1, SENK_ILUB = 2 };
+/* SENK_IC0 / SENK_ICP: incomplete Cholesky (L D L^T) for a symmetric A (checked: pattern
+ * and values; SENK_ERR_ARG otherwise), on A's pattern or its ILU(p) fill pattern (p).  The
+ * reference announces IC(0)/IC(p) "using the symmetry" (README_ja.md:47) but has none.  The
+ * numeric factorization is the device Ilu0 (for symmetric A the same L and pivots D); the
+ * upper factor is then rewritten from the lower one, u_ij = d_i l_ji, so the
+ * preconditioner L D L^T is exactly symmetric, as CG requires. */
+enum senk_ilu_kind { SENK_ILU0 = 0, SENK_ILUP = 1, SENK_ILUB = 2, SENK_IC0 = 3, SENK_ICP = 4 };
 enum senk_factor_precision { SENK_FACTOR_F64 = 0, SENK_FACTOR_F32 = 1, SENK_FACTOR_F16 = 2 };
 typedef struct {
     int kind;       /* senk_ilu_kind                                                   */
@@ -449,7 +455,12 @@ int senk_prec_factor_nnz(senk_prec M, int *lnnz, int *unnz, int *nlevels_l, int 
 /* Copy the device factors back (host arrays sized by senk_prec_factor_nnz). */
 int senk_prec_download(senk_prec M, int *lrptr, int *lcind, double *lval, int *urptr, int *ucind, double *uval);
 
-enum senk_solver_kind { SENK_GMRES = 0, SENK_FGMRES = 1, SENK_GCR = 2, SENK_FGCR = 3, SENK_BICGSTAB = 4 };
+/* SENK_CG: preconditioned conjugate gradients for symmetric positive definite systems
+ * with a symmetric preconditioner (IC(0)/IC(p)); the reference announces CG
+ * (README_ja.md:44) but has none.  Stop test and count as BiCGStab's; no learned
+ * correction (SENK_ERR_ARG). */
+enum senk_solver_kind { SENK_GMRES = 0, SENK_FGMRES = 1, SENK_GCR = 2, SENK_FGCR = 3, SENK_BICGSTAB = 4,
+                        SENK_CG = 5 };
 enum senk_ortho { SENK_MGS = 0, SENK_CGS2 = 1 };
 typedef struct {
     int solver;          /* senk_solver_kind                                             */

@@ -486,6 +486,55 @@ void or_scaling_d(double *val, const int *cind, const int *rptr, const double *l
 DEF_ILU0(double, d)
 DEF_ILU0(float, f)
 
+/* Incomplete LDL^T on the pattern of a symmetric matrix: IC(0), or IC(p) on an ILU(p)
+ * fill pattern.  The reference announces IC(0)/IC(p) "using the symmetry" (README_ja.md:47,
+ * docs/index.html:180-181) but has none; restated from the up-looking (row) form of the
+ * factorization.  Row i, for its lower entries k < i in increasing order:
+ *   l_ik = (a_ik - sum_{m < k, (i,m) and (k,m) in the pattern} l_im d_m l_km) / d_k,
+ * then d_i = a_ii - sum_{k < i} l_ik^2 d_k.  Only the lower triangle and the diagonal are
+ * computed; the upper entries are set from them afterwards, u_ij = d_i l_ji, so the result
+ * in place on the full pattern is the combined L-DU factor of the preconditioner L D L^T.
+ * Returns 0, -1 on a zero pivot, -2 if the pattern is not symmetric or a row lacks its
+ * diagonal. */
+int or_ic0_d(double *val, const int *cind, const int *rptr, int N) {
+    int *pos = (int *)malloc(sizeof(int) * (N > 0 ? N : 1)), *dg = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+    int st = 0;
+    for (int i = 0; i < N; i++) pos[i] = -1;
+    for (int i = 0; i < N && !st; i++) {
+        dg[i] = -1;
+        for (int j = rptr[i]; j < rptr[i + 1]; j++) {
+            pos[cind[j]] = j;
+            if (cind[j] == i) dg[i] = j;
+        }
+        if (dg[i] < 0) { st = -2; break; }
+        double di = val[dg[i]];
+        for (int j = rptr[i]; j < rptr[i + 1] && cind[j] < i; j++) {
+            const int k = cind[j];
+            double sum = val[j];
+            for (int l = rptr[k]; l < rptr[k + 1] && cind[l] < k; l++)
+                if (pos[cind[l]] >= 0) sum -= val[pos[cind[l]]] * val[dg[cind[l]]] * val[l];
+            if (val[dg[k]] == 0) { st = -1; break; }
+            val[j] = sum / val[dg[k]];
+            di -= val[j] * val[j] * val[dg[k]];
+        }
+        val[dg[i]] = di;
+        if (!st && di == 0) st = -1;
+        for (int j = rptr[i]; j < rptr[i + 1]; j++) pos[cind[j]] = -1;
+    }
+    /* upper entries u_ki = d_k l_ik */
+    for (int i = 0; i < N && !st; i++)
+        for (int j = rptr[i]; j < rptr[i + 1] && cind[j] < i; j++) {
+            const int k = cind[j];
+            int f = -1;
+            for (int l = rptr[k]; l < rptr[k + 1]; l++)
+                if (cind[l] == i) { f = l; break; }
+            if (f < 0) { st = -2; break; }
+            val[f] = val[dg[k]] * val[j];
+        }
+    free(pos); free(dg);
+    return st;
+}
+
 /* senk_matrix.hpp:551-650 (AllocLevelZero) using senk_class.hpp's work-row semantics.
  * Row i starts from its own pattern at level 0.  Scanning the work row left to right,
  * every strictly-lower entry k with level <= p is a pivot: the work row is rebuilt as
@@ -941,6 +990,39 @@ int or_bicgstab_d(const double *val, const int *cind, const int *rptr, const or_
     }
     if (converged) *converged = done;
     free(r); free(rs); free(p); free(Kp); free(AKp); free(s); free(Ks); free(AKs); free(tmp);
+    return i;
+}
+/* Preconditioned conjugate gradients.  The reference announces CG (README_ja.md:44,
+ * docs/index.html:174-175) but has none; this is the textbook method (Hestenes-Stiefel):
+ * r = b - A x, z = M r, p = z; per iteration q = A p, alpha = (r,z)/(p,q), x += alpha p,
+ * r -= alpha q, stop when ||r|| < eps nrm_b (the reference BiCGStab's test,
+ * senk_bicgstab.hpp:55-58, and its count), else z = M r, beta = (r,z)_new/(r,z)_old,
+ * p = z + beta p.  hist[i] = ||r_{i+1}|| / nrm_b. */
+int or_cg_d(const double *val, const int *cind, const int *rptr, const or_prec_t *M, const double *b, double *x,
+            double nrm_b, int max_iter, int N, double eps, double *hist, int hist_cap, int *converged) {
+    double *r = (double *)malloc(sizeof(double) * N), *z = (double *)malloc(sizeof(double) * N);
+    double *p = (double *)malloc(sizeof(double) * N), *q = (double *)malloc(sizeof(double) * N);
+    int i, done = 0;
+    or_spmv_csr_d(val, cind, rptr, x, r, N);
+    or_axpby_d(1, b, -1, r, N);
+    prec_apply(M, r, z, N);
+    or_copy_d(z, p, N);
+    double rz = or_dot_d(r, z, N);
+    for (i = 0; i < max_iter; i++) {
+        or_spmv_csr_d(val, cind, rptr, p, q, N);
+        double alpha = rz / or_dot_d(p, q, N);
+        or_axpy_d(alpha, p, x, N);
+        or_axpy_d(-alpha, q, r, N);
+        double nr = or_nrm2_d(r, N);
+        hist_put(hist, hist_cap, i, nr / nrm_b);
+        if (nr < eps * nrm_b) { done = 1; i++; break; }
+        prec_apply(M, r, z, N);
+        double prev = rz;
+        rz = or_dot_d(r, z, N);
+        or_axpby_d(1, z, rz / prev, p, N);
+    }
+    if (converged) *converged = done;
+    free(r); free(z); free(p); free(q);
     return i;
 }
 /* senk_test.hpp:11-22 */

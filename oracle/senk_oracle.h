@@ -89,6 +89,7 @@ void or_get_diag_d(const double *val, const int *cind, const int *rptr, double *
 void or_scaling_d(double *val, const int *cind, const int *rptr, const double *ld, const double *rd, int N);
 int or_ilu0_d(double *val, const int *cind, const int *rptr, int N);
 int or_ilu0_f(float *val, const int *cind, const int *rptr, int N);
+int or_ic0_d(double *val, const int *cind, const int *rptr, int N);
 int or_alloc_level_zero_d(const double *val, const int *cind, const int *rptr, double *oval, int *ocind, int *orptr,
                           int N, int p, int cap);
 int or_alloc_block_zero_d(const double *val, const int *cind, const int *rptr, double *oval, int *ocind, int *orptr,
@@ -126,6 +127,8 @@ int or_gcr_d(const double *val, const int *cind, const int *rptr, const or_prec_
 int or_bicgstab_d(const double *val, const int *cind, const int *rptr, const or_prec_t *M,
                   const double *b, double *x, double nrm_b, int max_iter, int N, double eps,
                   double *hist, int hist_cap, int *converged);
+int or_cg_d(const double *val, const int *cind, const int *rptr, const or_prec_t *M, const double *b, double *x,
+            double nrm_b, int max_iter, int N, double eps, double *hist, int hist_cap, int *converged);
 double or_relative_residual_d(const double *val, const int *cind, const int *rptr, const double *b,
                               const double *x, int N);
 

@@ -29,9 +29,9 @@ STATUS = {
     7: "SENK_ERR_NO_DEVICE", 8: "SENK_ERR_ALLOC", 9: "SENK_ERR_UNSUPPORTED", 10: "SENK_ERR_SCHEDULE",
     11: "SENK_ERR_TIMEOUT", 12: "SENK_ERR_IO", 13: "SENK_ERR_FORMAT",
 }
-SENK_GMRES, SENK_FGMRES, SENK_GCR, SENK_FGCR, SENK_BICGSTAB = range(5)
+SENK_GMRES, SENK_FGMRES, SENK_GCR, SENK_FGCR, SENK_BICGSTAB, SENK_CG = range(6)
 SENK_MGS, SENK_CGS2 = 0, 1
-SENK_ILU0, SENK_ILUP, SENK_ILUB = 0, 1, 2
+SENK_ILU0, SENK_ILUP, SENK_ILUB, SENK_IC0, SENK_ICP = 0, 1, 2, 3, 4
 SENK_FACTOR_F64, SENK_FACTOR_F32, SENK_FACTOR_F16 = 0, 1, 2
 SENK_SHAPE_SYM, SENK_SHAPE_UNSYM = 0, 1
 

@@ -99,6 +99,63 @@ double avg_row(int nnz, int N) { return N > 0 ? (double)nnz / N : 0.0; }
 // elimination of senk_matrix.hpp updates exactly those), so the factors' off-diagonal
 // entries are A's own and only the pivots change (a triangle-free graph, e.g. a 5- or
 // 7-point stencil in any ordering).
+// A symmetric in pattern and values (bit for bit).
+bool symmetric_csr(int N, const int *rp, const int *ci, const double *v) {
+    std::vector<int> tp((size_t)N + 1, 0);
+    const int nnz = rp[N];
+    for (int j = 0; j < nnz; j++) {
+        if (ci[j] < 0 || ci[j] >= N) return false;
+        tp[ci[j] + 1]++;
+    }
+    for (int i = 0; i < N; i++) {
+        tp[i + 1] += tp[i];
+        if (tp[i + 1] - tp[i] != rp[i + 1] - rp[i]) return false;
+    }
+    std::vector<std::pair<int, double>> t((size_t)nnz);
+    std::vector<int> fill(tp.begin(), tp.end() - 1);
+    for (int i = 0; i < N; i++)
+        for (int j = rp[i]; j < rp[i + 1]; j++) t[fill[ci[j]]++] = {i, v[j]};
+    std::vector<std::pair<int, double>> row;
+    for (int i = 0; i < N; i++) {
+        row.clear();
+        for (int j = rp[i]; j < rp[i + 1]; j++) row.push_back({ci[j], v[j]});
+        std::sort(row.begin(), row.end());
+        // column i of A, rows ascending by construction
+        for (int k = 0; k < (int)row.size(); k++)
+            if (row[k].first != t[tp[i] + k].first || !(row[k].second == t[tp[i] + k].second)) return false;
+    }
+    return true;
+}
+
+// For IC: the U position of every L entry's transpose.  L entry e = the t-th lower entry
+// of row j (column i) in pattern order; U row i holds its inverted diagonal first, then
+// its upper entries in pattern order.  Fails (false) if some (i, j) is missing.
+bool ic_upper_map(int N, const int *fr, const int *fc, const std::vector<int> &lrp, const std::vector<int> &urp,
+                  std::vector<int> &umap, std::vector<int> &lcol) {
+    std::vector<std::vector<std::pair<int, int>>> up((size_t)N);
+    for (int i = 0; i < N; i++) {
+        int t = 0;
+        for (int a = fr[i]; a < fr[i + 1]; a++)
+            if (fc[a] > i) up[i].push_back({fc[a], urp[i] + 1 + t++});
+        std::sort(up[i].begin(), up[i].end());
+    }
+    umap.assign((size_t)lrp[N], -1);
+    lcol.assign((size_t)lrp[N], -1);
+    for (int j = 0; j < N; j++) {
+        int e = lrp[j];
+        for (int a = fr[j]; a < fr[j + 1]; a++) {
+            const int i = fc[a];
+            if (i >= j) continue;
+            auto it = std::lower_bound(up[i].begin(), up[i].end(), std::make_pair(j, -1));
+            if (it == up[i].end() || it->first != j) return false;
+            umap[(size_t)e] = it->second;
+            lcol[(size_t)e] = i;
+            e++;
+        }
+    }
+    return true;
+}
+
 bool dilu_pattern(int N, const int *rp, const int *ci) {
     std::vector<int> stamp((size_t)N, -1);
     for (int i = 0; i < N; i++) {
@@ -1151,7 +1208,13 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
     std::memcpy(fc, cind, sizeof(int) * nnzA);
     std::memcpy(fr, rptr, sizeof(int) * (N + 1));
     int st = SENK_OK;
-    if (opts->kind == SENK_ILUP && opts->p > 0) st = senk_alloc_level_zero(&fv, &fc, &fr, N, opts->p);
+    const bool ic = opts->kind == SENK_IC0 || opts->kind == SENK_ICP;
+    if (opts->kind < SENK_ILU0 || opts->kind > SENK_ICP) st = SENK_ERR_ARG;
+    else if (ic && !symmetric_csr(N, rptr, cind, val)) {
+        record_error("IC(0)/IC(p): the matrix is not symmetric");
+        st = SENK_ERR_ARG;
+    } else if ((opts->kind == SENK_ILUP || opts->kind == SENK_ICP) && opts->p > 0)
+        st = senk_alloc_level_zero(&fv, &fc, &fr, N, opts->p);
     else if (opts->kind == SENK_ILUB) st = senk_alloc_block_zero(&fv, &fc, &fr, N, opts->bnl, opts->bnw);
     if (st) {
         std::free(fv); std::free(fc); std::free(fr);
@@ -1196,6 +1259,7 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
     M->lnnz = M->hlrptr[N];
     M->unnz = M->hurptr[N];
     int *dfr = nullptr, *dfc = nullptr;
+    int *dpos = nullptr;  // diagonal positions of the combined factor (kept for IC)
     double *dfv = nullptr;
     int *lr = nullptr, *lc = nullptr, *ur = nullptr, *uc = nullptr;
     double *l64 = nullptr, *u64 = nullptr;
@@ -1212,7 +1276,7 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
             M->owned.push_back(v.us.dblk);
         }
         // numeric factorization on the device (dependency-respecting schedule)
-        int *dpos = nullptr, *derr = nullptr;
+        int *derr = nullptr;
         if ((st = dalloc(&dpos, (size_t)N))) break;
         if ((st = dalloc(&derr, 1))) break;
         if (hipMemsetAsync(derr, 0, sizeof(int), c->stream) != hipSuccess) { st = SENK_ERR_HIP; break; }
@@ -1221,7 +1285,6 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
         int herr = 0;
         if (hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess) { st = SENK_ERR_HIP; break; }
-        (void)hipFree(dpos);
         (void)hipFree(derr);
         if (herr == 1) { st = SENK_ERR_ZERO_PIVOT; break; }
         if (herr == 2) { st = SENK_ERR_UNSUPPORTED; break; }
@@ -1234,6 +1297,22 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
         if ((st = dalloc(&u64, (size_t)M->unnz))) break;
         M->owned.insert(M->owned.end(), {lr, ur, lc, uc, l64, u64});
         if ((st = launch_split_ldu<double>(c, N, dfr, dfc, dfv, lr, l64, lc, ur, u64, uc))) break;
+        if (ic) {
+            // exploit the symmetry: U = D L^T from the lower factor (exactly symmetric M)
+            std::vector<int> umap, lcol;
+            int *dumap = nullptr, *dlcol = nullptr;
+            if (!ic_upper_map(N, fr, fc, M->hlrptr, M->hurptr, umap, lcol)) {
+                record_error("IC(p): the fill pattern is not symmetric");
+                st = SENK_ERR_ARG;
+            } else if (!(st = upload(ctx, &dumap, umap.data(), umap.size())) &&
+                       !(st = upload(ctx, &dlcol, lcol.data(), lcol.size()))) {
+                st = launch_ic_upper(c, M->lnnz, dumap, dlcol, dfv, dpos, l64, u64);
+                if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = SENK_ERR_HIP;
+            }
+            (void)hipFree(dumap);
+            (void)hipFree(dlcol);
+            if (st) break;
+        }
         v.lrptr = lr; v.lcind = lc; v.l64 = l64;
         v.urptr = ur; v.ucind = uc; v.u64 = u64;
         if (opts->precision & (1 << SENK_FACTOR_F32)) {
@@ -1263,7 +1342,7 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
             v.ut = &M->ut;
             // ILU(0) with no elimination update of an off-diagonal entry: the split form of
             // the mixed forms' inner BiCGStab
-            if (v.l32 && A->v64 && fnnz == nnzA && !(opts->precision & (1 << SENK_FACTOR_F16)) &&
+            if (!ic && v.l32 && A->v64 && fnnz == nnzA && !(opts->precision & (1 << SENK_FACTOR_F16)) &&
                 dilu_pattern(N, fr, fc)) {
                 if ((st = dalloc(&M->dilu_d, (size_t)N))) break;
                 if ((st = dalloc(&M->dilu_e, (size_t)N))) break;
@@ -1286,6 +1365,7 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
     (void)hipFree(dfr);
     (void)hipFree(dfc);
     (void)hipFree(dfv);
+    (void)hipFree(dpos);
     std::free(fv);
     std::free(fc);
     std::free(fr);

@@ -166,6 +166,8 @@ int launch_ilu0(Ctx *c, const Schedule &s, int N, const int *rptr, const int *ci
                 int *derr);
 int launch_find_diag(Ctx *c, int N, const int *rptr, const int *cind, int *dpos);
 // d = diag(U) (from U's inverted diagonal, first in each row), e = 2 d - diag(A), FP32
+int launch_ic_upper(Ctx *c, int lnnz, const int *umap, const int *lcol, const double *f, const int *dpos,
+                    const double *l, double *u);
 int launch_dilu_diag(Ctx *c, int N, const int *urptr, const double *uval, const int *arptr, const int *acind,
                      const double *aval, float *d, float *e);
 template <typename T>

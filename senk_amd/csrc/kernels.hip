@@ -791,6 +791,25 @@ int launch_dilu_diag(Ctx *c, int N, const int *urptr, const double *uval, const 
     return SENK_OK;
 }
 
+// IC(0)/IC(p): the upper factor from the lower one, u_ij = d_i l_ji, so that L D L^T is
+// exactly symmetric.  L entry e (row j, column i = lcol[e]) goes to U position umap[e];
+// d_i is the factorization's pivot, f[dpos[i]] (the full combined factor).
+__global__ __launch_bounds__(kBlock) void k_ic_upper(int lnnz, const int *__restrict__ umap,
+                                                     const int *__restrict__ lcol, const double *__restrict__ f,
+                                                     const int *__restrict__ dpos, const double *__restrict__ l,
+                                                     double *u) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= lnnz) return;
+    u[umap[e]] = f[dpos[lcol[e]]] * l[e];
+}
+int launch_ic_upper(Ctx *c, int lnnz, const int *umap, const int *lcol, const double *f, const int *dpos,
+                    const double *l, double *u) {
+    if (lnnz <= 0) return SENK_OK;
+    k_ic_upper<<<(lnnz + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(lnnz, umap, lcol, f, dpos, l, u);
+    SENK_LAUNCH_CHECK();
+    return SENK_OK;
+}
+
 // Split "L-DU" with inverted diagonal (senk_matrix.hpp:257-333) on the device; the
 // output row pointers come from the host pattern split.
 template <typename T>

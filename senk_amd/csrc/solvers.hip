@@ -1405,6 +1405,162 @@ static int bicgstab(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *
     return SENK_OK;
 }
 
+// ================================================================= CG
+// Preconditioned conjugate gradients.  The reference announces CG (README_ja.md:44,
+// docs/index.html:174-175) but has none; these are the textbook recurrences with the
+// reference BiCGStab's stop test ||r|| < eps ||b|| and count.  M must be symmetric
+// positive definite (IC(0)/IC(p): L D L^T).  Scalars in sc[]: [2]/[3] (r, z),
+// double-buffered by iteration parity.
+namespace {
+// alpha = (r, z) / (p, q);  x += alpha p;  r -= alpha q;  partials of ||r||^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int N, const T *__restrict__ p, const T *__restrict__ q,
+                                                      T *x, T *r, const double *rz, const double *Pin, int B,
+                                                      double *Pout, const int *stop) {
+    if (*stop) return;
+    const double alpha = *rz / sum_partials(Pin, B);
+    const T a = (T)alpha;
+    double rr = 0;
+    SENK_FOR4(i0, N) {
+        T pv[4], xv[4], qv[4], rv[4];
+        get4(p, i0, N, pv);
+        get4(x, i0, N, xv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) xv[u] = a * pv[u] + xv[u];
+        put4(x, i0, N, xv);
+        get4(q, i0, N, qv);
+        get4(r, i0, N, rv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            rv[u] = (-a) * qv[u] + rv[u];
+            rr += (double)rv[u] * (double)rv[u];
+        }
+        put4(r, i0, N, rv);
+    }
+    const double v[1] = {rr};
+    store_partials<1>(v, Pout);
+}
+// stop test on ||r||; beta = (r, z)_new / (r, z)_old;  p = z + beta p
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_p(int N, const T *__restrict__ z, T *p, double *sc, int parity,
+                                                 const double *Prr, const double *Prz, int B, double tol,
+                                                 double nrm_b, int iter, int *state, double *hist, int hcap,
+                                                 unsigned long long *mark, unsigned seq) {
+    if (state[0]) {
+        if (mark && blockIdx.x == 0 && threadIdx.x == 0) write_mark(mark, seq, 1);
+        return;
+    }
+    const double nr = sqrt(sum_partials(Prr, B));
+    const double rz_new = sum_partials(Prz, B);
+    const bool fin = !isfinite(nr) || !isfinite(rz_new), conv = nr < tol;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (hist && iter < hcap) hist[iter] = nr / nrm_b;
+        state[1] = iter + 1;
+        if (fin) {
+            state[2] = 1;
+            state[0] = 1;
+        } else if (conv) {
+            state[0] = 1;
+        } else {
+            sc[2 + (parity ^ 1)] = rz_new;
+        }
+        if (mark) write_mark(mark, seq, fin || conv);
+    }
+    if (fin || conv) return;
+    const T be = (T)(rz_new / sc[2 + parity]);
+    SENK_FOR4(i0, N) {
+        T zv[4], pv[4];
+        get4(z, i0, N, zv);
+        get4(p, i0, N, pv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) pv[u] = zv[u] + be * pv[u];
+        put4(p, i0, N, pv);
+    }
+}
+}  // namespace
+
+template <typename T>
+static int cg(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
+              senk_result *res, double *hist, int hist_cap) {
+    if (M.learned) return SENK_ERR_ARG;  // CG needs a fixed, symmetric preconditioner
+    const int N = A.N;
+    const int hcap = p.max_iter + 1;
+    size_t need = 5 * bytes_of<T>(N) + bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(16);
+    void *wsp;
+    SENK_TRY(ctx_work(c, need, &wsp));
+    Carver cv{(char *)wsp, 0, need};
+    T *r = cv.take<T>(N), *z = cv.take<T>(N), *pp = cv.take<T>(N), *q = cv.take<T>(N), *tmp = cv.take<T>(N);
+    double *dhist = cv.take<double>(hcap + 1);
+    int *state = cv.take<int>(8);
+    double *sc = cv.take<double>(16);
+    Poller poll;
+    SENK_TRY(poll.init(c, state));
+    SolveTimer tm;
+    SENK_TRY(tm.start(c));
+    const double tol = p.eps * p.nrm_b;
+    const int B = red_grid(N);
+    // r = b - A x; z = M r; p = z; (r, z)
+    int rg = 0;
+    SENK_TRY(residual<T>(c, A, x, b, r, &rg));
+    // the state words come from the shared arena: clear the stop flag first
+    k_state_reset<<<1, 64, 0, c->stream>>>(state);
+    SENK_LAUNCH_CHECK();
+    SENK_TRY(prec_apply<T>(c, M, r, nullptr, z, tmp, state));
+    SENK_TRY(launch_copy<T>(c, N, z, pp));
+    k_bicg_alpha<T><<<B, kBlock, 0, c->stream>>>(N, r, z, red_slot(c, kSlotC), state);
+    SENK_LAUNCH_CHECK();
+    k_bicg_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotC), B, sc, state);
+    SENK_LAUNCH_CHECK();
+    poll.reset();
+    bool stop = false;
+    for (int i = 0; i < p.max_iter && !stop; i++) {
+        const int par = i & 1;
+        SENK_TRY(spmv<T>(c, A, pp, q, state));
+        k_bicg_alpha<T><<<B, kBlock, 0, c->stream>>>(N, pp, q, red_slot(c, kSlotA), state);  // (p, q)
+        SENK_LAUNCH_CHECK();
+        k_cg_update<T><<<B, kBlock, 0, c->stream>>>(N, pp, q, x, r, sc + 2 + par, red_slot(c, kSlotA), B,
+                                                    red_slot(c, kSlotB), state);
+        SENK_LAUNCH_CHECK();
+        SENK_TRY(prec_apply<T>(c, M, r, nullptr, z, tmp, state));
+        k_bicg_alpha<T><<<B, kBlock, 0, c->stream>>>(N, r, z, red_slot(c, kSlotC), state);  // (r, z)
+        SENK_LAUNCH_CHECK();
+        unsigned seq = 0;
+        unsigned long long *mark = poll.dev_mark(&seq);
+        k_cg_p<T><<<B, kBlock, 0, c->stream>>>(N, z, pp, sc, par, red_slot(c, kSlotB), red_slot(c, kSlotC), B, tol,
+                                               p.nrm_b, i, state, dhist, hcap, mark, seq);
+        SENK_LAUNCH_CHECK();
+        SENK_TRY(poll.poll(kLag, &stop));
+    }
+    int hstate[4];
+    SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+    float ms = 0;
+    SENK_TRY(tm.stop(c, &ms));
+    const int converged = hstate[0] && !hstate[2];
+    const int iters = hstate[0] ? hstate[1] : p.max_iter;
+    std::vector<double> hh((size_t)std::max(iters, 1), 0.0);
+    if (iters > 0)
+        SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * std::min(iters, hcap), hipMemcpyDeviceToHost));
+    if (hist)
+        for (int k = 0; k < std::min(iters, hist_cap); k++) hist[k] = hh[k];
+    res->iters = iters;
+    res->converged = converged;
+    res->restarts = 0;
+    res->implicit_relres = iters > 0 ? hh[iters - 1] : 0;
+    res->t_solve_ms = ms;
+    res->status = hstate[2] ? SENK_ERR_BREAKDOWN : SENK_OK;
+    if (p.verbose) {
+        if (converged) {
+            std::printf("I iter %d\n", iters);
+            std::printf("# res %e\n", res->implicit_relres);
+        } else {
+            std::printf("# iter %d (max)\n", iters);
+            std::printf("# res %e\n", res->implicit_relres);
+        }
+        std::fflush(stdout);
+    }
+    return SENK_OK;
+}
+
 template <typename T>
 int krylov_solve(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
                  senk_result *res, double *hist, int hist_cap) {
@@ -1416,6 +1572,7 @@ int krylov_solve(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
         case SENK_GCR:
         case SENK_FGCR: return gcr<T>(c, A, M, b, x, p, res, hist, hist_cap);
         case SENK_BICGSTAB: return bicgstab<T>(c, A, M, b, x, p, res, hist, hist_cap);
+        case SENK_CG: return cg<T>(c, A, M, b, x, p, res, hist, hist_cap);
         default: return SENK_ERR_ARG;
     }
 }
