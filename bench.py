@@ -15,6 +15,9 @@ driver times only the solve).
                  reference's AbmcIluGmresm algorithm, modified Gram-Schmidt), at its
                  fastest ABMC block size; fp64_gmres30_cgs2 the same with two-pass
                  classical Gram-Schmidt (reported beside it, not the baseline)
+  fp64_ic0_cg    FP64 conjugate gradients with the incomplete Cholesky factor (the system
+                 is symmetric positive definite); mixed_cg_ic0 the nested mixed form with an
+                 FP32 inner CG
   learned_*      the same with the trained message-passing correction of the ILU
                  application (weights from senk_amd/data/gnn_vardiff.npz)
   plain_fp64_c2  the 27-point convection-diffusion 128^3 system (Pe 10, ABMC blocks of
@@ -286,6 +289,8 @@ MIXED = dict(m=30, eps=1e-10, max_iter=6000, mixed=2, inner_eps=1e-3, outer_m=20
 # tolerances 3e-3..1e-4 and block sizes 4 / 8 / 16 on this system, with the split inner
 # form and without: this setting is within noise of the best of either)
 INNER_BICGSTAB = 60
+# inner CG steps per outer step (40 / 80 / 150 steps at 1e-2 / 1e-3: 48-61 ms, this the best)
+INNER_CG = 40
 
 
 def main():
@@ -368,6 +373,12 @@ def main():
     runs["fp64_gmres30_cgs2_b8"] = timed(d8, K, W, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000,
                                          ortho=S.SENK_CGS2)
     runs["fp64_bicgstab_b8"] = timed(d8, K, W, solver=S.SENK_BICGSTAB, m=30, eps=1e-10, max_iter=6000)
+    # conjugate gradients with IC(0) (the reference announces both, has neither)
+    dic = Device(ctx, d8.sy, kind=S.SENK_IC0)
+    runs["fp64_ic0_cg_b8"] = timed(dic, K, W, solver=S.SENK_CG, eps=1e-10, max_iter=6000)
+    runs["mixed_cg_ic0_b8"] = timed(dic, K, W, solver=S.SENK_CG, inner_max=INNER_CG, **MIXED)
+    log(f"[bench] fp64_ic0_cg_b8: {json.dumps(runs['fp64_ic0_cg_b8'])}")
+    dic.close()
     for tag, dev in (("b8", d8), ("b4", d4)):
         runs[f"mixed_bicgstab_{tag}"] = timed(dev, K, W, solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)
         runs[f"mixed_fgcr_{tag}"] = timed(dev, K, W, solver=S.SENK_FGCR, inner_max=30, **MIXED)
