@@ -348,6 +348,11 @@ def main():
                 "learned_fgcr30_fp64_b8": (d8, dict(solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
                                                     check_every=1, use_learned=1)),
                 "learned_mixed_fgcr_b8": (d8, dict(solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED))}
+        if args.run in ("fp64_ic0_cg_b8", "mixed_cg_ic0_b8"):
+            d4.close()
+            d4 = Device(ctx, d8.sy, kind=S.SENK_IC0)  # the IC(0) factor of the same system
+            cfgs[args.run] = (d4, dict(solver=S.SENK_CG, eps=1e-10, max_iter=6000) if args.run.startswith("fp64")
+                              else dict(solver=S.SENK_CG, inner_max=INNER_CG, **MIXED))
         dev, kw = cfgs[args.run]
         net = None
         if kw.get("use_learned"):
