@@ -9,14 +9,16 @@ solve with every input resident in HBM (setup timed separately, as the reference
 driver times only the solve).
 
   value          time-to-1e-10 of the fastest mixed-precision solve: FP64 flexible GCR
-                 outer iteration around FP32 inner Krylov solves with FP32 SpMV and FP32
-                 ILU(0) factors (ms per solve, median of --steps)
+                 outer iteration around FP32 inner Krylov solves (BiCGStab, or CG: the
+                 system is symmetric positive definite) with FP32 ILU(0) factors, in the
+                 split form where the factorization allows it (ms per solve, median of
+                 --steps); config.mixed names the solve
   fp64_gmres30   the plain FP64 ILU(0)-GMRES(30) solve it is compared with (the
                  reference's AbmcIluGmresm algorithm, modified Gram-Schmidt), at its
                  fastest ABMC block size; fp64_gmres30_cgs2 the same with two-pass
                  classical Gram-Schmidt (reported beside it, not the baseline)
   fp64_ic0_cg    FP64 conjugate gradients with the incomplete Cholesky factor (the system
-                 is symmetric positive definite); mixed_cg_ic0 the nested mixed form with an
+                 is symmetric positive definite); mixed_cg_* the nested mixed form with an
                  FP32 inner CG
   learned_*      the same with the trained message-passing correction of the ILU
                  application (weights from senk_amd/data/gnn_vardiff.npz)
@@ -289,8 +291,10 @@ MIXED = dict(m=30, eps=1e-10, max_iter=6000, mixed=2, inner_eps=1e-3, outer_m=20
 # tolerances 3e-3..1e-4 and block sizes 4 / 8 / 16 on this system, with the split inner
 # form and without: this setting is within noise of the best of either)
 INNER_BICGSTAB = 60
-# inner CG steps per outer step (40 / 80 / 150 steps at 1e-2 / 1e-3: 48-61 ms, this the best)
-INNER_CG = 40
+# inner CG steps per outer step of the nested solve (split form; sweeps over 30..100 steps,
+# tolerances 1e-2 / 1e-3 and block sizes 4 / 8: 30.8-47 ms at 128^3, this within noise of
+# the best at blocks of 8)
+INNER_CG = 100
 
 
 def main():
@@ -345,14 +349,14 @@ def main():
         cfgs = {"fp64_gmres30_b8": (d8, dict(solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000)),
                 "mixed_bicgstab_b4": (d4, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)),
                 "mixed_bicgstab_b8": (d8, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)),
+                "mixed_cg_b8": (d8, dict(solver=S.SENK_CG, inner_max=INNER_CG, **MIXED)),
                 "learned_fgcr30_fp64_b8": (d8, dict(solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
                                                     check_every=1, use_learned=1)),
                 "learned_mixed_fgcr_b8": (d8, dict(solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED))}
-        if args.run in ("fp64_ic0_cg_b8", "mixed_cg_ic0_b8"):
+        if args.run == "fp64_ic0_cg_b8":
             d4.close()
             d4 = Device(ctx, d8.sy, kind=S.SENK_IC0)  # the IC(0) factor of the same system
-            cfgs[args.run] = (d4, dict(solver=S.SENK_CG, eps=1e-10, max_iter=6000) if args.run.startswith("fp64")
-                              else dict(solver=S.SENK_CG, inner_max=INNER_CG, **MIXED))
+            cfgs[args.run] = (d4, dict(solver=S.SENK_CG, eps=1e-10, max_iter=6000))
         dev, kw = cfgs[args.run]
         net = None
         if kw.get("use_learned"):
@@ -381,12 +385,12 @@ def main():
     # conjugate gradients with IC(0) (the reference announces both, has neither)
     dic = Device(ctx, d8.sy, kind=S.SENK_IC0)
     runs["fp64_ic0_cg_b8"] = timed(dic, K, W, solver=S.SENK_CG, eps=1e-10, max_iter=6000)
-    runs["mixed_cg_ic0_b8"] = timed(dic, K, W, solver=S.SENK_CG, inner_max=INNER_CG, **MIXED)
     log(f"[bench] fp64_ic0_cg_b8: {json.dumps(runs['fp64_ic0_cg_b8'])}")
     dic.close()
     for tag, dev in (("b8", d8), ("b4", d4)):
         runs[f"mixed_bicgstab_{tag}"] = timed(dev, K, W, solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)
         runs[f"mixed_fgcr_{tag}"] = timed(dev, K, W, solver=S.SENK_FGCR, inner_max=30, **MIXED)
+        runs[f"mixed_cg_{tag}"] = timed(dev, K, W, solver=S.SENK_CG, inner_max=INNER_CG, **MIXED)
         log(f"[bench] mixed_{tag}: {json.dumps(runs[f'mixed_bicgstab_{tag}'])}")
     net, meta = load_learned(ctx)
     if net is not None:

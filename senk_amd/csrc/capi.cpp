@@ -1342,7 +1342,7 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
             v.ut = &M->ut;
             // ILU(0) with no elimination update of an off-diagonal entry: the split form of
             // the mixed forms' inner BiCGStab
-            if (!ic && v.l32 && A->v64 && fnnz == nnzA && !(opts->precision & (1 << SENK_FACTOR_F16)) &&
+            if (v.l32 && A->v64 && fnnz == nnzA && !(opts->precision & (1 << SENK_FACTOR_F16)) &&
                 dilu_pattern(N, fr, fc)) {
                 if ((st = dalloc(&M->dilu_d, (size_t)N))) break;
                 if ((st = dalloc(&M->dilu_e, (size_t)N))) break;

@@ -1479,10 +1479,223 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int N, const T *__restrict__ z,
 }
 }  // namespace
 
+// Split form of CG for a D-ILU factorization of a symmetric A (IC(0), or ILU(0) on a
+// triangle-free graph: U = D L^T to rounding).  PCG with M = L D L^T is CG on
+// B = L^-1 A L^-T (y = L^T x, c = L^-1 b) preconditioned by D^-1.  Carrying u = D p and
+// yd = D y instead of p and y: B p = L^-1 A U^-1 u (the split product: one U and one L
+// solve, no product with A), D z = r, so u <- r + beta u, yd <- yd + alpha u, and at the
+// end x = U^-1 yd.  The residual it stops on is L^-1 (b - A x), relative to ||L^-1 b||.
+namespace {
+// partials of (r, r) and (r, r/d)
+__global__ __launch_bounds__(kBlock) void k_cgs_start(int N, const float *__restrict__ r,
+                                                      const float *__restrict__ d, double *Pout, const int *stop) {
+    if (*stop) return;
+    double a0 = 0, a1 = 0;
+    SENK_FOR4(i0, N) {
+        float rv[4], dv[4];
+        get4(r, i0, N, rv);
+        get4(d, i0, N, dv);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a0 += (double)rv[u] * (double)rv[u];
+            a1 += (double)rv[u] * (double)rv[u] / (double)dv[u];
+        }
+    }
+    const double v[2] = {a0, a1};
+    store_partials<2>(v, Pout);
+}
+// sc[2] = rho = (r, r/d); dtol = {eps ||r||, ||r||}; a zero r stops as converged, a
+// non-finite one as a breakdown
+__global__ __launch_bounds__(kBlock) void k_cgs_init(const double *P, int B, double *sc, int *state, double eps,
+                                                     double *dtol) {
+    const double rr = sum_partials(P, B), rho = sum_partials(P + kPartLd, B);
+    if (threadIdx.x != 0) return;
+    sc[2] = rho;
+    state[0] = 0;
+    state[1] = 0;
+    state[2] = 0;
+    dtol[0] = eps * sqrt(rr);
+    dtol[1] = sqrt(rr);
+    if (!isfinite(rr) || !isfinite(rho)) {
+        state[0] = 1;
+        state[2] = 1;
+    } else if (!(rr > 0)) {
+        state[0] = 1;
+    }
+}
+// q = D t + w and the partials of (q, u/d)
+__global__ __launch_bounds__(kBlock) void k_cgs_post(int N, const float *__restrict__ d, const float *__restrict__ t,
+                                                     const float *__restrict__ w, float *q,
+                                                     const float *__restrict__ u, double *Pout, const int *stop) {
+    if (*stop) return;
+    double a0 = 0;
+    SENK_FOR4(i0, N) {
+        float dv[4], tv[4], wv[4], uv[4];
+        get4(d, i0, N, dv);
+        get4(t, i0, N, tv);
+        get4(w, i0, N, wv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) wv[k] = dv[k] * tv[k] + wv[k];
+        put4(q, i0, N, wv);
+        get4(u, i0, N, uv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) a0 += (double)wv[k] * (double)uv[k] / (double)dv[k];
+    }
+    const double v[1] = {a0};
+    store_partials<1>(v, Pout);
+}
+// alpha = rho / (q, u/d); yd += alpha u; r -= alpha q; partials of (r, r) and (r, r/d)
+__global__ __launch_bounds__(kBlock) void k_cgs_update(int N, const float *__restrict__ u,
+                                                       const float *__restrict__ q, const float *__restrict__ d,
+                                                       float *yd, float *r, const double *rho, const double *Pin,
+                                                       int B, double *Pout, const int *stop) {
+    if (*stop) return;
+    const float a = (float)(*rho / sum_partials(Pin, B));
+    double rr = 0, rz = 0;
+    SENK_FOR4(i0, N) {
+        float uv[4], yv[4], qv[4], rv[4], dv[4];
+        get4(u, i0, N, uv);
+        get4(yd, i0, N, yv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) yv[k] = a * uv[k] + yv[k];
+        put4(yd, i0, N, yv);
+        get4(q, i0, N, qv);
+        get4(r, i0, N, rv);
+        get4(d, i0, N, dv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            rv[k] = (-a) * qv[k] + rv[k];
+            rr += (double)rv[k] * (double)rv[k];
+            rz += (double)rv[k] * (double)rv[k] / (double)dv[k];
+        }
+        put4(r, i0, N, rv);
+    }
+    const double v[2] = {rr, rz};
+    store_partials<2>(v, Pout);
+}
+// stop test on ||r|| (tolerance formed on the device); beta = rho_new / rho; u = r + beta u
+__global__ __launch_bounds__(kBlock) void k_cgs_u(int N, const float *__restrict__ r, float *u, double *sc,
+                                                  int parity, const double *Pin, int B, const double *dtol, int iter,
+                                                  int *state, double *hist, int hcap, unsigned long long *mark,
+                                                  unsigned seq) {
+    if (state[0]) {
+        if (mark && blockIdx.x == 0 && threadIdx.x == 0) write_mark(mark, seq, 1);
+        return;
+    }
+    const double nr = sqrt(sum_partials(Pin, B));
+    const double rho_new = sum_partials(Pin + kPartLd, B);
+    const bool fin = !isfinite(nr) || !isfinite(rho_new), conv = nr < dtol[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (hist && iter < hcap) hist[iter] = nr / dtol[1];
+        state[1] = iter + 1;
+        if (fin) {
+            state[2] = 1;
+            state[0] = 1;
+        } else if (conv) {
+            state[0] = 1;
+        } else {
+            sc[2 + (parity ^ 1)] = rho_new;
+        }
+        if (mark) write_mark(mark, seq, fin || conv);
+    }
+    if (fin || conv) return;
+    const float be = (float)(rho_new / sc[2 + parity]);
+    SENK_FOR4(i0, N) {
+        float rv[4], uv[4];
+        get4(r, i0, N, rv);
+        get4(u, i0, N, uv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) uv[k] = rv[k] + be * uv[k];
+        put4(u, i0, N, uv);
+    }
+}
+}  // namespace
+
+// x = U^-1 yd; x must be zero on entry.  Triangles alternate L, U as the layouts require:
+// L (start), (U, L) per product, U (end).
+static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b, float *x, const SolveParams &p,
+                    senk_result *res, double *hist, int hist_cap) {
+    const int N = A.N;
+    const int hcap = p.max_iter + 1;
+    size_t need = 6 * bytes_of<float>(N) + bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(16);
+    void *wsp;
+    SENK_TRY(ctx_work(c, need, &wsp));
+    Carver cv{(char *)wsp, 0, need};
+    float *r = cv.take<float>(N), *u = cv.take<float>(N), *q = cv.take<float>(N), *t = cv.take<float>(N);
+    float *w = cv.take<float>(N), *yd = cv.take<float>(N);
+    double *dhist = cv.take<double>(hcap + 1);
+    int *state = cv.take<int>(8);
+    double *sc = cv.take<double>(16);
+    double *dtol = sc + 8;  // {tolerance, ||L^-1 b||}, formed on the device
+    Poller poll;
+    SENK_TRY(poll.init(c, state));
+    SolveTimer tm;
+    SENK_TRY(tm.start(c));
+    const int B = red_grid(N);
+    k_state_reset<<<1, 64, 0, c->stream>>>(state);
+    SENK_LAUNCH_CHECK();
+    // r = L^-1 b; u = r; yd = 0; rho = (r, r/d)
+    SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, b, r, state));
+    SENK_TRY(launch_copy<float>(c, N, r, u));
+    SENK_HIP(hipMemsetAsync(yd, 0, sizeof(float) * N, c->stream));
+    k_cgs_start<<<B, kBlock, 0, c->stream>>>(N, r, M.dilu_d, red_slot(c, kSlotB), state);
+    SENK_LAUNCH_CHECK();
+    k_cgs_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotB), B, sc, state, p.eps, dtol);
+    SENK_LAUNCH_CHECK();
+    poll.reset();
+    bool stop = false;
+    for (int i = 0; i < p.max_iter && !stop; i++) {
+        const int par = i & 1;
+        // q = L^-1 A U^-1 u  (t = U^-1 u; w = u - E t; w = L^-1 w; q = D t + w)
+        bool fused = false;
+        SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, u, t, state, M.dilu_e, w, &fused));
+        if (!fused) {
+            k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, u, M.dilu_e, t, w, state);
+            SENK_LAUNCH_CHECK();
+        }
+        SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, w, w, state));
+        k_cgs_post<<<B, kBlock, 0, c->stream>>>(N, M.dilu_d, t, w, q, u, red_slot(c, kSlotA), state);
+        SENK_LAUNCH_CHECK();
+        k_cgs_update<<<B, kBlock, 0, c->stream>>>(N, u, q, M.dilu_d, yd, r, sc + 2 + par, red_slot(c, kSlotA), B,
+                                                 red_slot(c, kSlotB), state);
+        SENK_LAUNCH_CHECK();
+        unsigned seq = 0;
+        unsigned long long *mark = poll.dev_mark(&seq);
+        k_cgs_u<<<B, kBlock, 0, c->stream>>>(N, r, u, sc, par, red_slot(c, kSlotB), B, dtol, i, state, dhist, hcap,
+                                             mark, seq);
+        SENK_LAUNCH_CHECK();
+        SENK_TRY(poll.poll(kLag, &stop));
+    }
+    // x = U^-1 yd (after the stop flag is set: no flag for this sweep)
+    SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, yd, x, nullptr));
+    int hstate[4];
+    SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+    float ms = 0;
+    SENK_TRY(tm.stop(c, &ms));
+    const int converged = hstate[0] && !hstate[2];
+    const int iters = hstate[0] ? hstate[1] : p.max_iter;
+    std::vector<double> hh((size_t)std::max(iters, 1), 0.0);
+    if (iters > 0)
+        SENK_HIP(hipMemcpy(hh.data(), dhist, sizeof(double) * std::min(iters, hcap), hipMemcpyDeviceToHost));
+    if (hist)
+        for (int k = 0; k < std::min(iters, hist_cap); k++) hist[k] = hh[k];
+    res->iters = iters;
+    res->converged = converged;
+    res->restarts = 0;
+    res->implicit_relres = iters > 0 ? hh[iters - 1] : 0;
+    res->t_solve_ms = ms;
+    res->status = hstate[2] ? SENK_ERR_BREAKDOWN : SENK_OK;
+    return SENK_OK;
+}
+
 template <typename T>
 static int cg(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, const SolveParams &p,
               senk_result *res, double *hist, int hist_cap) {
     if (M.learned) return SENK_ERR_ARG;  // CG needs a fixed, symmetric preconditioner
+    if constexpr (std::is_same<T, float>::value) {
+        if (p.split && M.dilu_d && M.dilu_e && M.lt && M.ut && !M.f16)
+            return cg_split(c, A, M, b, x, p, res, hist, hist_cap);
+    }
     const int N = A.N;
     const int hcap = p.max_iter + 1;
     size_t need = 5 * bytes_of<T>(N) + bytes_of<double>(hcap + 1) + bytes_of<int>(8) + bytes_of<double>(16);

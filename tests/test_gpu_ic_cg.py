@@ -204,3 +204,43 @@ def test_errors(ctx):
     finally:
         S.lib().senk_prec_destroy(M)
         S.lib().senk_mat_destroy(A)
+
+
+@pytest.mark.parametrize("kind", [S.SENK_IC0, S.SENK_ILU0])
+@pytest.mark.parametrize("mode", [3, 2, 1, 0])
+def test_split_inner_cg(ctx, kind, mode):
+    """The split form of the FP32 inner CG (CG on L^-1 A L^-T preconditioned by D^-1,
+    one U and one L sweep per step, no SpMV) against the plain inner CG: both converge to
+    the FP64 tolerance, with iteration counts close to each other; a repeated solve
+    repeats its iterations and the plain application is unchanged after split solves."""
+    sy = _system(20, 8, seed=4)
+    N = sy["N"]
+    A, M, st = _handles(ctx, sy, kind)
+    assert st == 0
+    try:
+        S.call("senk_prec_set_trsv_mode", M, mode)
+        xr = np.random.default_rng(6).uniform(-1, 1, N).astype(np.float32)
+        dxr, dyr = ctx.array(xr), ctx.zeros(N, np.float32)
+        S.call("senk_prec_apply_f32", M, dxr.p, dyr.p)
+        z0 = dyr.download()
+        its = {}
+        for mixed in (2, 1):
+            for split in (1, 0):
+                S.call("senk_prec_set_split", M, split)
+                st, res, x = _solve(ctx, A, M, sy, solver=S.SENK_CG, eps=1e-10, mixed=mixed, inner_eps=1e-3,
+                                    inner_max=60, outer_m=20, max_iter=3000)
+                assert st == 0 and res.converged == 1 and res.status == 0, (mixed, split)
+                assert res.true_relres <= 1.5e-10, (mixed, split, res.true_relres)
+                assert relerr(x, sy["xs"]) <= 1e-7
+                its[(mixed, split)] = res.iters
+            assert its[(mixed, 1)] <= 1.3 * its[(mixed, 0)] + 10, its
+        S.call("senk_prec_set_split", M, 1)
+        st, r2, x2 = _solve(ctx, A, M, sy, solver=S.SENK_CG, eps=1e-10, mixed=2, inner_eps=1e-3, inner_max=60,
+                            outer_m=20, max_iter=3000)
+        assert r2.iters == its[(2, 1)]
+        S.call("senk_prec_apply_f32", M, dxr.p, dyr.p)
+        assert np.array_equal(dyr.download(), z0)
+        S.call("senk_prec_check", M)
+    finally:
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
