@@ -445,8 +445,10 @@ int senk_prec_set_trsv_mode(senk_prec M, int mode);
  * 7-point stencil in any ordering), A = L D + U - E with D = diag(U), E = 2D - diag(A),
  * and the inner FP32 BiCGStab runs on L^-1 A U^-1 y = L^-1 r, each product being one U
  * and one L solve plus two vector passes and no SpMV (Eisenstat's trick); d = U^-1 y.
- * On by default where it applies; enable = 1 where it does not is SENK_ERR_UNSUPPORTED.
- * The FP64 solvers and the other inner solvers are not affected. */
+ * The inner FP32 CG (symmetric A) likewise runs as CG on L^-1 A L^-T preconditioned by
+ * D^-1 with the same products.  On by default where it applies (also for IC(0) factors);
+ * enable = 1 where it does not is SENK_ERR_UNSUPPORTED.  The FP64 solvers and the other
+ * inner solvers are not affected. */
 int senk_prec_set_split(senk_prec M, int enable);
 /* Synchronise and report SENK_ERR_TIMEOUT if a single-launch solve since the last check
  * gave up a wait (never expected; senk_solve checks by itself). */
