@@ -297,6 +297,57 @@ INNER_BICGSTAB = 60
 INNER_CG = 100
 
 
+def ingest_rates(n):
+    """MatrixMarket ingestion on the host: the n^3 variable-coefficient matrix written as a
+    symmetric coordinate file (its lower triangle), read by the library
+    (senk_read_matrix_market + senk_expand) and by the reference's io::ReadMatrixMarket +
+    Expand (oracle/_ref), median of 3 reads each; both must give the same arrays."""
+    import tempfile
+    rptr, cind, val = S.gen_vardiff7(n, 1, 8.0, 1.5)
+    N = len(rptr) - 1
+    rows = np.repeat(np.arange(N), np.diff(rptr))
+    keep = cind <= rows
+    r1, c1, v1 = rows[keep] + 1, cind[keep] + 1, val[keep]
+    out = {"matrix": f"vardiff7 {n}^3 lower triangle", "N": N, "entries": int(keep.sum())}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "a.mtx")
+        with open(path, "w") as f:
+            f.write("%%MatrixMarket matrix coordinate real symmetric\n")
+            f.write(f"{N} {N} {int(keep.sum())}\n")
+            f.write("\n".join(f"{i} {j} {x!r}" for i, j, x in zip(r1.tolist(), c1.tolist(), v1.tolist())))
+            f.write("\n")
+        out["file_MB"] = round(os.path.getsize(path) / 1e6, 1)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            v, c, r, N_, _, shape = S.read_matrix_market(path)
+            ev, ec, er = S.expand(v, c, r, N_, "L")
+            ts.append(time.perf_counter() - t0)
+        out["library_s"] = round(statistics.median(ts), 3)
+        out["library_MBps"] = round(out["file_MB"] / out["library_s"], 1)
+        try:
+            from oracle import oracle as O
+            ref = O.ref()
+        except Exception:
+            ref = None
+        if ref is not None:
+            cap = len(v) + 16
+            rv, rc, rr, dims = np.zeros(cap), np.zeros(cap, np.int32), np.zeros(N + 1, np.int32), np.zeros(3, np.int32)
+            xv, xc, xr = np.zeros(2 * cap), np.zeros(2 * cap, np.int32), np.zeros(N + 1, np.int32)
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                k = ref.ref_read_mm(path.encode(), 1, O.P(rv), O.P(rc), O.P(rr), O.P(dims), cap, N + 1)
+                ke = ref.ref_expand_d(O.P(rv), O.P(rc), O.P(rr), N, b"L", O.P(xv), O.P(xc), O.P(xr), 2 * cap)
+                ts.append(time.perf_counter() - t0)
+            out["reference_s"] = round(statistics.median(ts), 3)
+            out["reference_MBps"] = round(out["file_MB"] / out["reference_s"], 1)
+            out["identical"] = bool(k == len(v) and np.array_equal(rv[:k].view(np.int64), v.view(np.int64))
+                                    and np.array_equal(xc[:ke], ec) and np.array_equal(xv[:ke], ev)
+                                    and np.array_equal(xr, er))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,10 +364,15 @@ def main():
     ap.add_argument("--run", default=None, help="time only the solve configuration of this name (profiling)")
     ap.add_argument("--sweep-modes", action="store_true",
                     help="also time the ILU application with one cooperative launch per triangle")
+    ap.add_argument("--ingest", action="store_true",
+                    help="host only: MatrixMarket reading, the library's reader against the reference's")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not supported (use --gpus 1)"}))
         sys.exit(2)
+    if args.ingest:
+        print(json.dumps(ingest_rates(args.n)), flush=True)
+        return
     n = args.n
     ctx = S.Context(0)
     log(f"[bench] building the {n}^3 variable-coefficient diffusion system (ABMC blocks of 8 and 4)")

@@ -10,8 +10,15 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "internal.hpp"
 
@@ -22,11 +29,14 @@ T *malloc_n(size_t n) {
     return (T *)std::malloc(sizeof(T) * (n ? n : 1));
 }
 
+// The C locale's white space (what stream extraction skips): ' ', \t \n \v \f \r.
+inline bool is_space(char ch) { return ch == ' ' || (ch >= '\t' && ch <= '\r'); }
+
 // Whitespace-separated token reader over a whole file held in memory.
 struct Cursor {
     const char *p, *end;
     void skip_space() {
-        while (p < end && std::isspace((unsigned char)*p)) p++;
+        while (p < end && is_space(*p)) p++;
     }
     bool read_long(long long *v) {
         skip_space();
@@ -34,7 +44,7 @@ struct Cursor {
         auto r = std::from_chars(p, end, *v);
         if (r.ec != std::errc()) return false;
         p = r.ptr;
-        return p == end || std::isspace((unsigned char)*p);
+        return p == end || is_space(*p);
     }
     bool read_double(double *v) {
         skip_space();
@@ -42,7 +52,48 @@ struct Cursor {
         auto r = std::from_chars(p, end, *v, std::chars_format::general);
         if (r.ec != std::errc()) return false;
         p = r.ptr;
-        return p == end || std::isspace((unsigned char)*p);
+        return p == end || is_space(*p);
+    }
+};
+
+// A whole file in memory: mapped read-only, or read into a buffer where mapping fails.
+struct FileText {
+    const char *data = nullptr;
+    size_t size = 0;
+    void *map = nullptr;
+    std::unique_ptr<char[]> buf;
+    ~FileText() {
+        if (map) munmap(map, size);
+    }
+    // 0, or errno-style failure
+    int open(const char *path) {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return 1;
+        struct stat st;
+        if (fstat(fd, &st) != 0) {
+            ::close(fd);
+            return 1;
+        }
+        size = (size_t)st.st_size;
+        if (size > 0) {
+            void *m = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (m != MAP_FAILED) {
+                map = m;
+                data = (const char *)m;
+            } else {
+                buf.reset(new char[size]);
+                size_t got = 0;
+                while (got < size) {
+                    const ssize_t k = ::read(fd, buf.get() + got, size - got);
+                    if (k <= 0) break;
+                    got += (size_t)k;
+                }
+                size = got;
+                data = buf.get();
+            }
+        }
+        ::close(fd);
+        return 0;
     }
 };
 
@@ -54,6 +105,83 @@ std::string lower(std::string s) {
 int fail(int code, const std::string &msg) {
     senk_amd::record_error(msg);
     return code;
+}
+
+// The entry list: `pe` triples "row col value" as whitespace-separated tokens (a triple
+// may span lines, as with stream extraction).  The text is cut into chunks at whitespace,
+// each chunk's tokens are counted in parallel, a prefix sum gives every chunk the global
+// index of its first token (and so the row / column / value phase of each token), and the
+// chunks are parsed in parallel.  Returns the index of the first bad entry in file order
+// (unparsable or outside n x m), or pe when all are good; *bad_range tells which.
+long long parse_entries(const char *p, const char *end, long long pe, long long n, long long m, int *row, int *col,
+                        double *val, bool *bad_range) {
+    const long long need = 3 * pe;
+    const size_t bytes = (size_t)(end - p);
+    int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (bytes < (size_t)(1 << 20)) nt = 1;
+    std::vector<const char *> cut((size_t)nt + 1);
+    cut[0] = p;
+    cut[nt] = end;
+    for (int t = 1; t < nt; t++) {
+        const char *b = std::max(cut[t - 1], p + bytes / nt * t);
+        while (b < end && !is_space(*b)) b++;  // never inside a token
+        cut[t] = b;
+    }
+    std::vector<long long> ntok((size_t)nt + 1, 0), first((size_t)nt, need), range((size_t)nt, 0);
+    auto run = [&](auto &&fn) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(fn, t);
+        fn(0);
+        for (auto &x : th) x.join();
+    };
+    run([&](int t) {
+        long long k = 0;
+        bool in = false;
+        for (const char *q = cut[t]; q < cut[t + 1]; q++) {
+            const bool sp = is_space(*q);
+            if (!sp && !in) k++;
+            in = !sp;
+        }
+        ntok[(size_t)t + 1] = k;
+    });
+    for (int t = 0; t < nt; t++) ntok[(size_t)t + 1] += ntok[t];
+    run([&](int t) {
+        long long g = ntok[t];
+        Cursor c{cut[t], cut[t + 1]};
+        while (g < need) {
+            c.skip_space();
+            if (c.p >= c.end) break;
+            const long long e = g / 3;
+            bool ok;
+            if (g % 3 == 2) {
+                ok = c.read_double(&val[e]);
+            } else {
+                long long v;
+                ok = c.read_long(&v);
+                if (ok && (v < 1 || v > (g % 3 ? m : n))) {
+                    ok = false;
+                    range[t] = 1;
+                }
+                if (ok) (g % 3 ? col : row)[e] = (int)(v - 1);
+            }
+            if (!ok) {
+                first[t] = g;
+                return;
+            }
+            g++;
+        }
+    });
+    long long bad = need;
+    for (int t = 0; t < nt; t++)
+        if (first[t] < bad) {
+            bad = first[t];
+            *bad_range = range[t] != 0;
+        }
+    if (bad == need && ntok[nt] < need) {  // the list ends early
+        bad = ntok[nt];
+        *bad_range = false;
+    }
+    return bad / 3;
 }
 
 // Banner ("%%MatrixMarket matrix coordinate <real|integer|double> <general|symmetric>",
@@ -68,7 +196,7 @@ int read_head(Cursor &c, int *sh, long long *n, long long *m, long long *pe) {
             b.skip_space();
             if (b.p >= b.end) break;
             const char *s = b.p;
-            while (b.p < b.end && !std::isspace((unsigned char)*b.p)) b.p++;
+            while (b.p < b.end && !is_space(*b.p)) b.p++;
             tok.push_back(lower(std::string(s, b.p)));
         }
     }
@@ -83,7 +211,7 @@ int read_head(Cursor &c, int *sh, long long *n, long long *m, long long *pe) {
     while (p < c.end) {
         const char *e = std::find(p, c.end, '\n');
         const char *q = p;
-        while (q < e && std::isspace((unsigned char)*q)) q++;
+        while (q < e && is_space(*q)) q++;
         if (q < e && *q != '%') break;
         p = e < c.end ? e + 1 : c.end;
     }
@@ -118,7 +246,7 @@ int senk_matrix_market_size(const char *filename, int *N, int *M, long long *ent
         new_line = head.back() == '\n';
         if (starts && ++lines > 1) {
             const char *q = line;
-            while (*q && std::isspace((unsigned char)*q)) q++;
+            while (*q && is_space(*q)) q++;
             if (*q && *q != '%') size_seen = true;
         }
         if (size_seen && new_line) break;
@@ -145,37 +273,31 @@ int senk_matrix_market_size(const char *filename, int *N, int *M, long long *ent
 int senk_read_matrix_market(const char *filename, double **val, int **cind, int **rptr, int *N, int *M, int *shape,
                             int removeZeros) {
     if (!filename || !val || !cind || !rptr || !N || !M || !shape) return SENK_ERR_ARG;
-    FILE *f = std::fopen(filename, "rb");
-    if (!f) return fail(SENK_ERR_IO, std::string("could not open ") + filename);
-    std::vector<char> buf;
-    {
-        char chunk[1 << 16];
-        size_t got;
-        while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
-        const bool err = std::ferror(f);
-        std::fclose(f);
-        if (err) return fail(SENK_ERR_IO, std::string("could not read ") + filename);
-    }
-    Cursor c{buf.data(), buf.data() + buf.size()};
+    FileText text;
+    if (text.open(filename)) return fail(SENK_ERR_IO, std::string("could not open ") + filename);
+    Cursor c{text.data, text.data + text.size};
     int sh;
     long long n, m, pe;
     if (int st = read_head(c, &sh, &n, &m, &pe)) return st;
-    std::vector<int> row, col;
-    std::vector<double> v;
-    row.reserve((size_t)pe);
-    col.reserve((size_t)pe);
-    v.reserve((size_t)pe);
-    for (long long k = 0; k < pe; k++) {
-        long long r, q;
-        double x;
-        if (!c.read_long(&r) || !c.read_long(&q) || !c.read_double(&x))
-            return fail(SENK_ERR_FORMAT, "MatrixMarket: entry " + std::to_string(k + 1) + " is not 'row col value'");
-        if (r < 1 || r > n || q < 1 || q > m)
-            return fail(SENK_ERR_FORMAT, "MatrixMarket: entry " + std::to_string(k + 1) + " lies outside the matrix");
-        if (removeZeros && x == 0) continue;
-        row.push_back((int)(r - 1));
-        col.push_back((int)(q - 1));
-        v.push_back(x);
+    std::vector<int> row((size_t)pe), col((size_t)pe);
+    std::vector<double> v((size_t)pe);
+    bool bad_range = false;
+    const long long bad = parse_entries(c.p, c.end, pe, n, m, row.data(), col.data(), v.data(), &bad_range);
+    if (bad < pe)
+        return fail(SENK_ERR_FORMAT, "MatrixMarket: entry " + std::to_string(bad + 1) +
+                                         (bad_range ? " lies outside the matrix" : " is not 'row col value'"));
+    if (removeZeros) {  // drop stored zeros, keeping file order
+        size_t k = 0;
+        for (size_t e = 0; e < (size_t)pe; e++)
+            if (v[e] != 0) {
+                row[k] = row[e];
+                col[k] = col[e];
+                v[k] = v[e];
+                k++;
+            }
+        row.resize(k);
+        col.resize(k);
+        v.resize(k);
     }
     const size_t nnz = v.size();
     // two stable counting passes (by column, then by row): rows in order, columns

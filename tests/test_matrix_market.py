@@ -233,3 +233,39 @@ def test_size_query(tmp_path):
     assert S.read_matrix_market(path)[3:5] == (7, 5)
     assert S.lib().senk_matrix_market_size(str(tmp_path / "absent").encode(), C.byref(N), C.byref(M), C.byref(E),
                                            C.byref(sh)) == 12
+
+
+def test_large_file_parallel_parse(tmp_path):
+    """A file above the reader's parallel threshold (1 MB) whose triples are split across
+    lines and runs of blank space, so that chunk cuts fall inside triples: the same arrays
+    as the oracle (and the reference where built); a bad entry near the end is reported
+    with its index in file order."""
+    rng = np.random.default_rng(9)
+    n, k = 5000, 120000
+    r = rng.integers(1, n + 1, k)
+    c = rng.integers(1, n + 1, k)
+    key = np.unique(r.astype(np.int64) * (n + 1) + c, return_index=True)[1]
+    r, c = r[np.sort(key)], c[np.sort(key)]
+    v = np.round(rng.normal(size=len(r)), 6)
+    v[::97] = 0.0
+    seps = ["\n", " ", "\t", "\n\n", "  \n "]
+    body = "".join(f"{i}{seps[e % 5]}{j}{seps[(e + 2) % 5]}{x!r}\n"
+                   for e, (i, j, x) in enumerate(zip(r.tolist(), c.tolist(), v.tolist())))
+    text = f"%%MatrixMarket matrix coordinate real general\n{n} {n} {len(r)}\n" + body
+    assert len(text) > (1 << 20)
+    path = _write(tmp_path, "big", text)
+    for rz in (1, 0):
+        got = S.read_matrix_market(path, rz)
+        _check_csr(got, *MO.read_matrix_market(path, rz)[:3])
+    ref = O.ref()
+    if ref is not None:
+        cap = len(r) + 8
+        rv, rc, rr, d = np.zeros(cap), np.zeros(cap, np.int32), np.zeros(n + 1, np.int32), np.zeros(3, np.int32)
+        nnz = ref.ref_read_mm(str(path).encode(), 1, O.P(rv), O.P(rc), O.P(rr), O.P(d), cap, n + 1)
+        _check_csr(S.read_matrix_market(path, 1), rv[:nnz], rc[:nnz], rr)
+    lines = text.split("\n")
+    bad = "\n".join(lines[:-40]) + "\n" + f"{n + 1} 1 1.0\n" + "\n".join(lines[-40:])
+    bad = bad.replace(f"{n} {n} {len(r)}", f"{n} {n} {len(r) + 1}", 1)
+    out = [C.c_void_p() for _ in range(3)] + [C.c_int() for _ in range(3)]
+    st = S.lib().senk_read_matrix_market(str(_write(tmp_path, "bad", bad)).encode(), *(C.byref(o) for o in out), 1)
+    assert st == 13 and b"lies outside the matrix" in S.lib().senk_last_error()
