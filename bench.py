@@ -30,9 +30,12 @@ driver times only the solve).
                  same for the FP32 SELL-C-sigma SpMV
   cpu_baseline   the reference's own OpenMP AbmcIluGmresm (oracle/_ref) on the host cores,
                  the full FP64 solve of the same system and factors
+  ingest         MatrixMarket reading + symmetric expansion on the host: the library's
+                 reader against the reference's io::ReadMatrixMarket + Expand (64^3 file)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--kernels-only]
-       [--sweep-modes] [--learned-only] [--run NAME]
+       [--sweep-modes] [--learned-only] [--run NAME] [--ingest] [--no-cpu] [--no-c2]
+       [--no-ingest]
 """
 from __future__ import annotations
 
@@ -358,6 +361,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the MatrixMarket ingestion entry")
     ap.add_argument("--kernels-only", action="store_true", help="kernel rates and the copy calibration only")
     ap.add_argument("--learned-only", action="store_true",
                     help="time the learned correction's application alone (FP32, 10 applications)")
@@ -514,6 +518,8 @@ def main():
                           "frac": round(kern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4),
                           "traffic": traffic_from_profile("spmv_sell_f32_vardiff128.md")},
         "cpu_baseline": cpu,
+        # host-side MatrixMarket ingestion (64^3 file; --ingest --n 128 for the full size)
+        "ingest": None if args.no_ingest else ingest_rates(64),
     }
     d8.close()
     d4.close()
