@@ -19,7 +19,7 @@ from pathlib import Path
 import pytest
 
 import senk_amd as S
-from senk_amd.build import CLI
+from senk_amd.build import CLI, LIB, build_cli
 from tests.helpers import WORKFLOW_CASES, workflow_text
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -34,10 +34,17 @@ def _case(tmp_path, name):
     return path, g["stdout"]
 
 
-def _run(path):
+def _cli():
+    # the workflow program is built with the library; build it alone if only the library is
+    if not CLI.exists() and LIB.exists():
+        build_cli()
     if not CLI.exists():
         pytest.fail(f"{CLI} is not built (python -m senk_amd.build)")
-    return subprocess.run([str(CLI), str(path)], capture_output=True, text=True, timeout=300)
+    return str(CLI)
+
+
+def _run(path):
+    return subprocess.run([_cli(), str(path)], capture_output=True, text=True, timeout=300)
 
 
 def _has_device():
@@ -60,18 +67,17 @@ def test_host_steps_print_the_reference_lines(tmp_path, name):
 
 
 def test_usage_and_bad_file(tmp_path):
-    if not CLI.exists():
-        pytest.fail(f"{CLI} is not built")
-    assert subprocess.run([str(CLI)], capture_output=True, text=True).returncode == 2
-    run = subprocess.run([str(CLI), str(tmp_path / "absent.mtx")], capture_output=True, text=True)
+    cli = _cli()
+    assert subprocess.run([cli], capture_output=True, text=True).returncode == 2
+    run = subprocess.run([cli, str(tmp_path / "absent.mtx")], capture_output=True, text=True)
     assert run.returncode == 1 and "file could not be read" in run.stdout
     bad = tmp_path / "bad.mtx"
     bad.write_text("%%MatrixMarket matrix coordinate pattern general\n2 2 1\n1 1\n")
-    run = subprocess.run([str(CLI), str(bad)], capture_output=True, text=True)
+    run = subprocess.run([cli, str(bad)], capture_output=True, text=True)
     assert run.returncode == 1 and "not a supported MatrixMarket matrix" in run.stdout
     nodiag = tmp_path / "nodiag.mtx"
     nodiag.write_text("%%MatrixMarket matrix coordinate real general\n2 2 2\n1 1 1.0\n2 1 1.0\n")
-    run = subprocess.run([str(CLI), str(nodiag)], capture_output=True, text=True)
+    run = subprocess.run([cli, str(nodiag)], capture_output=True, text=True)
     assert run.returncode == 1 and run.stdout.splitlines()[-1] == "False: CheckStructure"
 
 
