@@ -1440,6 +1440,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int N, const T *__restrict
     const double v[1] = {rr};
     store_partials<1>(v, Pout);
 }
+// sc[2] = (r, z); a zero (r, z) (r = 0: x already solves the system) stops as converged
+// with x unchanged, a negative or non-finite one (M not positive definite) as a breakdown
+__global__ __launch_bounds__(kBlock) void k_cg_init(const double *P, int B, double *sc, int *state) {
+    const double rz = sum_partials(P, B);
+    if (threadIdx.x != 0) return;
+    sc[2] = rz;
+    state[0] = 0;
+    state[1] = 0;
+    state[2] = 0;
+    if (!isfinite(rz) || rz < 0) {
+        state[0] = 1;
+        state[2] = 1;
+    } else if (rz == 0) {
+        state[0] = 1;
+    }
+}
 // stop test on ||r||; beta = (r, z)_new / (r, z)_old;  p = z + beta p
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_p(int N, const T *__restrict__ z, T *p, double *sc, int parity,
@@ -1722,7 +1738,7 @@ static int cg(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, con
     SENK_TRY(launch_copy<T>(c, N, z, pp));
     k_bicg_alpha<T><<<B, kBlock, 0, c->stream>>>(N, r, z, red_slot(c, kSlotC), state);
     SENK_LAUNCH_CHECK();
-    k_bicg_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotC), B, sc, state);
+    k_cg_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotC), B, sc, state);
     SENK_LAUNCH_CHECK();
     poll.reset();
     bool stop = false;

@@ -244,3 +244,21 @@ def test_split_inner_cg(ctx, kind, mode):
     finally:
         S.lib().senk_prec_destroy(M)
         S.lib().senk_mat_destroy(A)
+
+
+@pytest.mark.parametrize("mode", [3, "csr"])
+def test_cg_zero_right_hand_side(ctx, mode):
+    """b = 0 from x0 = 0: (r, z) = 0 at the start, so CG stops at once as converged and x
+    stays exactly zero (the first step would otherwise divide 0 by 0)."""
+    sy = _system(8, 8)
+    sy["b"] = np.zeros(sy["N"])
+    A, M, st = _handles(ctx, sy, csr_only=1 if mode == "csr" else 0)
+    assert st == 0
+    try:
+        st, res, x = _solve(ctx, A, M, sy, solver=S.SENK_CG, eps=1e-10, max_iter=100)
+        assert st == 0 and res.status == 0 and res.converged == 1 and res.iters == 0
+        assert np.array_equal(x, np.zeros(sy["N"]))
+        S.call("senk_prec_check", M)
+    finally:
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
