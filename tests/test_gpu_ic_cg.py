@@ -262,3 +262,26 @@ def test_cg_zero_right_hand_side(ctx, mode):
     finally:
         S.lib().senk_prec_destroy(M)
         S.lib().senk_mat_destroy(A)
+
+
+@pytest.mark.parametrize("mixed", [2, 1])
+def test_split_inner_bicgstab_with_ic_factors(ctx, mixed):
+    """IC(0) factors also carry the split-form vectors (U = D L^T equals A's upper part to
+    rounding on a triangle-free graph): the split inner BiCGStab converges with them to the
+    FP64 tolerance, as with ILU(0) factors, in a count close to the plain inner solve's."""
+    sy = _system(20, 8, seed=7)
+    A, M, st = _handles(ctx, sy)
+    assert st == 0
+    try:
+        its = {}
+        for split in (1, 0):
+            S.call("senk_prec_set_split", M, split)
+            st, res, x = _solve(ctx, A, M, sy, solver=S.SENK_BICGSTAB, eps=1e-10, mixed=mixed, inner_eps=1e-3,
+                                inner_max=40, outer_m=20, max_iter=3000)
+            assert st == 0 and res.converged == 1 and res.status == 0, split
+            assert res.true_relres <= 1.5e-10 and relerr(x, sy["xs"]) <= 1e-7
+            its[split] = res.iters
+        assert its[1] <= 1.3 * its[0] + 10, its
+    finally:
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
