@@ -132,6 +132,10 @@ int senk_sptrsv_csr_l_amc_f64(senk_ctx ctx, const double *dval, const int *dcind
                               const double *dx, double *dy, int N, const int *cptr, int cnum);
 int senk_sptrsv_csr_u_amc_f64(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
                               const double *dx, double *dy, int N, const int *cptr, int cnum);
+int senk_sptrsv_csr_l_amc_f32(senk_ctx ctx, const float *dval, const int *dcind, const int *drptr,
+                              const float *dx, float *dy, int N, const int *cptr, int cnum);
+int senk_sptrsv_csr_u_amc_f32(senk_ctx ctx, const float *dval, const int *dcind, const int *drptr,
+                              const float *dx, float *dy, int N, const int *cptr, int cnum);
 
 /* Sequential and block-Jacobi forms (senk_sparse.hpp:132-167, 328-379) need the
  * dependency levels of the factor, computed once on the host from its pattern:
@@ -149,12 +153,21 @@ int senk_sptrsv_csr_l_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval,
 /* SptrsvCsr_u<T>(val,cind,rptr,x,y,N)       senk_sparse.hpp:154-167 */
 int senk_sptrsv_csr_u_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind,
                           const int *drptr, const double *dx, double *dy, int N);
+/* The same for T = float (senk_sparse.hpp:132-167 instantiated on float). */
+int senk_sptrsv_csr_l_f32(senk_ctx ctx, senk_trsv_plan plan, const float *dval, const int *dcind,
+                          const int *drptr, const float *dx, float *dy, int N);
+int senk_sptrsv_csr_u_f32(senk_ctx ctx, senk_trsv_plan plan, const float *dval, const int *dcind,
+                          const int *drptr, const float *dx, float *dy, int N);
 /* SptrsvCsr_l / _u block-Jacobi form (bnum ranges of N/bnum rows; the plan is built from
  * the RemoveOffDiagonal factors, so ranges are independent).   senk_sparse.hpp:328-379 */
 int senk_sptrsv_csr_l_bj_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind,
                              const int *drptr, const double *dx, double *dy, int N, int bnum);
 int senk_sptrsv_csr_u_bj_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind,
                              const int *drptr, const double *dx, double *dy, int N, int bnum);
+int senk_sptrsv_csr_l_bj_f32(senk_ctx ctx, senk_trsv_plan plan, const float *dval, const int *dcind,
+                             const int *drptr, const float *dx, float *dy, int N, int bnum);
+int senk_sptrsv_csr_u_bj_f32(senk_ctx ctx, senk_trsv_plan plan, const float *dval, const int *dcind,
+                             const int *drptr, const float *dx, float *dy, int N, int bnum);
 /* SptrsvBcsr_l / _u, ABMC form (bnl x bnw blocks)      senk_sparse.hpp:481-581 */
 int senk_sptrsv_bcsr_l_abmc_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
                                 const double *dx, double *dy, int N, const int *cptr, int cnum, int bsize,
@@ -162,11 +175,21 @@ int senk_sptrsv_bcsr_l_abmc_f64(senk_ctx ctx, const double *dbval, const int *db
 int senk_sptrsv_bcsr_u_abmc_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
                                 const double *dx, double *dy, int N, const int *cptr, int cnum, int bsize,
                                 int bnl, int bnw);
+int senk_sptrsv_bcsr_l_abmc_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr,
+                                const float *dx, float *dy, int N, const int *cptr, int cnum, int bsize,
+                                int bnl, int bnw);
+int senk_sptrsv_bcsr_u_abmc_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr,
+                                const float *dx, float *dy, int N, const int *cptr, int cnum, int bsize,
+                                int bnl, int bnw);
 /* SptrsvBcsr_l / _u sequential form               senk_sparse.hpp:392-465 */
 int senk_sptrsv_bcsr_l_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
                            const double *dx, double *dy, int N, int bnl, int bnw);
 int senk_sptrsv_bcsr_u_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
                            const double *dx, double *dy, int N, int bnl, int bnw);
+int senk_sptrsv_bcsr_l_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr,
+                           const float *dx, float *dy, int N, int bnl, int bnw);
+int senk_sptrsv_bcsr_u_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr,
+                           const float *dx, float *dy, int N, int bnl, int bnw);
 
 /* ------------------------------------------------------------------ BLAS-1 / BLAS-2
  * senk_blas1.hpp / senk_blas2.hpp.  Vectors are device pointers; scalars are host
@@ -187,6 +210,10 @@ int senk_haddiv_f64(senk_ctx ctx, const double *dx, double *dy, int N);         
 int senk_copy_f32(senk_ctx ctx, const float *dx, float *dy, int N);
 int senk_scal_f32(senk_ctx ctx, float a, float *dx, int N);
 int senk_axpy_f32(senk_ctx ctx, float a, const float *dx, float *dy, int N);
+int senk_axpby_f32(senk_ctx ctx, float a, const float *dx, float b, float *dy, int N);
+int senk_axpyz_f32(senk_ctx ctx, float a, const float *dx, const float *dy, float *dz, int N);
+int senk_hadprod_f32(senk_ctx ctx, const float *dx, float *dy, int N);
+int senk_haddiv_f32(senk_ctx ctx, const float *dx, float *dy, int N);
 int senk_dot_f32(senk_ctx ctx, const float *dx, const float *dy, int N, double *result);
 int senk_nrm2_f32(senk_ctx ctx, const float *dx, int N, double *result);
 /* Ggen / Grot are scalar host functions             senk_blas1.hpp:145-168 */
@@ -198,6 +225,12 @@ void senk_trsv(const double *U, const double *b, double *x, int n, int m);
 /* utils::Convert<double,float> on the device          senk_utils.hpp:103-110 */
 int senk_convert_f64_f32(senk_ctx ctx, const double *din, float *dout, int n);
 int senk_convert_f32_f64(senk_ctx ctx, const float *din, double *dout, int n);
+
+/* test::RelativeResidualError (senk_test.hpp:11-22) on the device: *relres = ||b - A x|| /
+ * ||b|| in FP64 (CSR A, device vectors); print != 0 also prints the reference's line
+ * "# test %e". */
+int senk_relative_residual_error(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                                 const double *db, const double *dx, int N, int print, double *relres);
 
 /* ------------------------------------------------------------------ matrix setup
  * senk_matrix.hpp.  Host arrays; integer outputs are bit-identical to the reference. */

@@ -63,6 +63,7 @@ def orc():
 _REF_PROTOS = {
     "ref_spmv_csr_d": (None, "pppppi"), "ref_spmv_csr_f": (None, "pppppi"), "ref_spmv_csr_diag_d": (None, "ppppppi"),
     "ref_spmv_sell_d": (None, "pppippi"), "ref_spmv_bcsr_d": ("i", "pppppiii"),
+    "ref_spmv_sell_f": (None, "pppippi"), "ref_spmv_bcsr_f": ("i", "pppppiii"),
     "ref_sptrsv_l_d": (None, "pppppi"), "ref_sptrsv_u_d": (None, "pppppi"),
     "ref_sptrsv_l_f": (None, "pppppi"), "ref_sptrsv_u_f": (None, "pppppi"),
     "ref_sptrsv_l_amc_d": (None, "pppppipi"), "ref_sptrsv_u_amc_d": (None, "pppppipi"),

@@ -42,6 +42,9 @@ EXPORT void ref_spmv_csr_diag_d(double *val, int *cind, int *rptr, double *diag,
 EXPORT void ref_spmv_sell_d(double *val, int *cind, int *wid, int len, double *x, double *y, int N) {
     senk::sparse::SpmvSell<double>(val, cind, wid, len, x, y, N);
 }
+EXPORT void ref_spmv_sell_f(float *val, int *cind, int *wid, int len, float *x, float *y, int N) {
+    senk::sparse::SpmvSell<float>(val, cind, wid, len, x, y, N);
+}
 
 #define BCSR_DISPATCH(BNL, BNW, CALL)                                   \
     if (BNL == 2 && BNW == 1) { constexpr int bl = 2, bw = 1; CALL; return 0; } \
@@ -55,6 +58,9 @@ EXPORT void ref_spmv_sell_d(double *val, int *cind, int *wid, int len, double *x
 
 EXPORT int ref_spmv_bcsr_d(double *bval, int *bcind, int *brptr, double *x, double *y, int N, int bnl, int bnw) {
     BCSR_DISPATCH(bnl, bnw, (senk::sparse::SpmvBcsr<double, bl, bw>(bval, bcind, brptr, x, y, N)))
+}
+EXPORT int ref_spmv_bcsr_f(float *bval, int *bcind, int *brptr, float *x, float *y, int N, int bnl, int bnw) {
+    BCSR_DISPATCH(bnl, bnw, (senk::sparse::SpmvBcsr<float, bl, bw>(bval, bcind, brptr, x, y, N)))
 }
 EXPORT void ref_sptrsv_l_d(double *val, int *cind, int *rptr, double *x, double *y, int N) {
     senk::sparse::SptrsvCsr_l<double>(val, cind, rptr, x, y, N);

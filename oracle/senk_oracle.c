@@ -14,7 +14,10 @@
  *    validity and equality with first-fit greedy; Reordering through
  *    B[i][RP[c]] = A[LP[i]][c]; Ggen/Grot/Trsv through closed forms; the Krylov
  *    drivers through the true residual reaching the tolerance.
- * No function is left unpinned.
+ * The FP32 variants (SpMV CSR / SELL / BCSR, sequential SpTRSV) are compared with the
+ * reference's own float instantiations and, without the reference, with the FP32 CSR
+ * product bit for bit and the triangular residuals to FP32 rounding; Copy and the
+ * relative residual likewise (test_float_kernels_*, test_relative_residual_pinned).
  *
  * Deliberate deviations from the reference (documented reference defects):
  *  - or_spmv_sell_*: the reference's last-slice test (senk_sparse.hpp:110) compares

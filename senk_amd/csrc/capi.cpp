@@ -99,8 +99,12 @@ double avg_row(int nnz, int N) { return N > 0 ? (double)nnz / N : 0.0; }
 // elimination of senk_matrix.hpp updates exactly those), so the factors' off-diagonal
 // entries are A's own and only the pivots change (a triangle-free graph, e.g. a 5- or
 // 7-point stencil in any ordering).
-// A symmetric in pattern and values (bit for bit).
-bool symmetric_csr(int N, const int *rp, const int *ci, const double *v) {
+// A symmetric in pattern, and in values to a relative tolerance rtol (0: bit for bit):
+// |a_ij - a_ji| <= rtol * max(|a_ij|, |a_ji|).  A symmetric matrix that went through
+// Jacobi scaling (val = ld_i * a_ij * rd_j, senk_matrix.hpp:406-416) keeps its symmetry only
+// to the last bit or two of each product.
+constexpr double kSymTol = 16 * 2.220446049250313e-16;  // a few units in the last place
+bool symmetric_csr(int N, const int *rp, const int *ci, const double *v, double rtol) {
     std::vector<int> tp((size_t)N + 1, 0);
     const int nnz = rp[N];
     for (int j = 0; j < nnz; j++) {
@@ -121,10 +125,26 @@ bool symmetric_csr(int N, const int *rp, const int *ci, const double *v) {
         for (int j = rp[i]; j < rp[i + 1]; j++) row.push_back({ci[j], v[j]});
         std::sort(row.begin(), row.end());
         // column i of A, rows ascending by construction
-        for (int k = 0; k < (int)row.size(); k++)
-            if (row[k].first != t[tp[i] + k].first || !(row[k].second == t[tp[i] + k].second)) return false;
+        for (int k = 0; k < (int)row.size(); k++) {
+            const double a = row[k].second, b = t[tp[i] + k].second;
+            if (row[k].first != t[tp[i] + k].first) return false;
+            if (!(a == b) && !(std::fabs(a - b) <= rtol * std::max(std::fabs(a), std::fabs(b)))) return false;
+        }
     }
     return true;
+}
+
+// Overwrite every strictly upper entry with its transpose's value (the pattern is
+// symmetric): the factorization then sees the symmetric matrix of A's lower triangle.
+void mirror_lower(int N, const int *rp, const int *ci, double *v) {
+    for (int i = 0; i < N; i++)
+        for (int e = rp[i]; e < rp[i + 1]; e++) {
+            const int j = ci[e];
+            if (j <= i) continue;
+            const int *b = ci + rp[j], *end = ci + rp[j + 1];
+            const int *pos = std::is_sorted(b, end) ? std::lower_bound(b, end, i) : std::find(b, end, i);
+            if (pos != end && *pos == i) v[e] = v[rp[j] + (pos - b)];
+        }
 }
 
 // For IC: the U position of every L entry's transpose.  L entry e = the t-th lower entry
@@ -313,6 +333,9 @@ int senk_device_free(senk_ctx ctx, void *dptr) {
     if (dptr) {
         SENK_HIP(hipStreamSynchronize(ctx->c.stream));
         SENK_HIP(hipFree(dptr));
+        // a later allocation at this address may hold another row-pointer array
+        for (auto it = ctx->nnz_cache.begin(); it != ctx->nnz_cache.end();)
+            it = it->first.first == dptr ? ctx->nnz_cache.erase(it) : std::next(it);
     }
     return SENK_OK;
 }
@@ -487,6 +510,14 @@ int senk_sptrsv_csr_u_amc_f64(senk_ctx ctx, const double *dval, const int *dcind
                               double *dy, int N, const int *cptr, int cnum) {
     return senk_sptrsv_csr_u_abmc_f64(ctx, dval, dcind, drptr, dx, dy, N, cptr, cnum, 1);
 }
+int senk_sptrsv_csr_l_amc_f32(senk_ctx ctx, const float *dval, const int *dcind, const int *drptr, const float *dx,
+                              float *dy, int N, const int *cptr, int cnum) {
+    return senk_sptrsv_csr_l_abmc_f32(ctx, dval, dcind, drptr, dx, dy, N, cptr, cnum, 1);
+}
+int senk_sptrsv_csr_u_amc_f32(senk_ctx ctx, const float *dval, const int *dcind, const int *drptr, const float *dx,
+                              float *dy, int N, const int *cptr, int cnum) {
+    return senk_sptrsv_csr_u_abmc_f32(ctx, dval, dcind, drptr, dx, dy, N, cptr, cnum, 1);
+}
 
 int senk_trsv_plan_create(senk_ctx ctx, const int *rptr, const int *cind, int N, int upper, senk_trsv_plan *out) {
     CTX_CHECK(ctx);
@@ -512,45 +543,60 @@ int senk_trsv_plan_destroy(senk_trsv_plan plan) {
     delete plan;
     return SENK_OK;
 }
-int senk_sptrsv_csr_l_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind, const int *drptr,
-                          const double *dx, double *dy, int N) {
+}  // extern "C"
+template <typename T>
+static int plan_trsv(senk_ctx ctx, senk_trsv_plan plan, bool upper, const T *dval, const int *dcind,
+                     const int *drptr, const T *dx, T *dy, int N) {
     CTX_CHECK(ctx);
     if (!plan || N < 0) return SENK_ERR_ARG;
-    return launch_sptrsv<double>(&ctx->c, plan->s, false, drptr, dcind, dval, dx, dy, nullptr, nullptr);
+    return launch_sptrsv<T>(&ctx->c, plan->s, upper, drptr, dcind, dval, dx, dy, nullptr, nullptr);
+}
+extern "C" {
+int senk_sptrsv_csr_l_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind, const int *drptr,
+                          const double *dx, double *dy, int N) {
+    return plan_trsv<double>(ctx, plan, false, dval, dcind, drptr, dx, dy, N);
 }
 int senk_sptrsv_csr_u_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind, const int *drptr,
                           const double *dx, double *dy, int N) {
-    CTX_CHECK(ctx);
-    if (!plan || N < 0) return SENK_ERR_ARG;
-    return launch_sptrsv<double>(&ctx->c, plan->s, true, drptr, dcind, dval, dx, dy, nullptr, nullptr);
+    return plan_trsv<double>(ctx, plan, true, dval, dcind, drptr, dx, dy, N);
+}
+int senk_sptrsv_csr_l_f32(senk_ctx ctx, senk_trsv_plan plan, const float *dval, const int *dcind, const int *drptr,
+                          const float *dx, float *dy, int N) {
+    return plan_trsv<float>(ctx, plan, false, dval, dcind, drptr, dx, dy, N);
+}
+int senk_sptrsv_csr_u_f32(senk_ctx ctx, senk_trsv_plan plan, const float *dval, const int *dcind, const int *drptr,
+                          const float *dx, float *dy, int N) {
+    return plan_trsv<float>(ctx, plan, true, dval, dcind, drptr, dx, dy, N);
 }
 int senk_sptrsv_csr_l_bj_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind,
                              const int *drptr, const double *dx, double *dy, int N, int bnum) {
     if (bnum <= 0) return SENK_ERR_ARG;
-    return senk_sptrsv_csr_l_f64(ctx, plan, dval, dcind, drptr, dx, dy, N);
+    return plan_trsv<double>(ctx, plan, false, dval, dcind, drptr, dx, dy, N);
 }
 int senk_sptrsv_csr_u_bj_f64(senk_ctx ctx, senk_trsv_plan plan, const double *dval, const int *dcind,
                              const int *drptr, const double *dx, double *dy, int N, int bnum) {
     if (bnum <= 0) return SENK_ERR_ARG;
-    return senk_sptrsv_csr_u_f64(ctx, plan, dval, dcind, drptr, dx, dy, N);
+    return plan_trsv<double>(ctx, plan, true, dval, dcind, drptr, dx, dy, N);
 }
-int senk_sptrsv_bcsr_l_abmc_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
-                                const double *dx, double *dy, int N, const int *cptr, int cnum, int bsize, int bnl,
-                                int bnw) {
+int senk_sptrsv_csr_l_bj_f32(senk_ctx ctx, senk_trsv_plan plan, const float *dval, const int *dcind,
+                             const int *drptr, const float *dx, float *dy, int N, int bnum) {
+    if (bnum <= 0) return SENK_ERR_ARG;
+    return plan_trsv<float>(ctx, plan, false, dval, dcind, drptr, dx, dy, N);
+}
+int senk_sptrsv_csr_u_bj_f32(senk_ctx ctx, senk_trsv_plan plan, const float *dval, const int *dcind,
+                             const int *drptr, const float *dx, float *dy, int N, int bnum) {
+    if (bnum <= 0) return SENK_ERR_ARG;
+    return plan_trsv<float>(ctx, plan, true, dval, dcind, drptr, dx, dy, N);
+}
+}  // extern "C"
+template <typename T>
+static int bcsr_abmc(senk_ctx ctx, const T *dbval, const int *dbcind, const int *dbrptr, const T *dx, T *dy, int N,
+                     const int *cptr, int cnum, int bsize, int bnl, int bnw, bool upper) {
     CTX_CHECK(ctx);
     if (!cptr || cnum < 0 || bsize <= 0 || bnl <= 0 || bnw <= 0) return SENK_ERR_ARG;
     if (N % bsize || bsize % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
     Schedule s = colors(cptr, cnum, bsize);
-    return launch_sptrsv_bcsr<double>(&ctx->c, s, false, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
-}
-int senk_sptrsv_bcsr_u_abmc_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
-                                const double *dx, double *dy, int N, const int *cptr, int cnum, int bsize, int bnl,
-                                int bnw) {
-    CTX_CHECK(ctx);
-    if (!cptr || cnum < 0 || bsize <= 0 || bnl <= 0 || bnw <= 0) return SENK_ERR_ARG;
-    if (N % bsize || bsize % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
-    Schedule s = colors(cptr, cnum, bsize);
-    return launch_sptrsv_bcsr<double>(&ctx->c, s, true, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
+    return launch_sptrsv_bcsr<T>(&ctx->c, s, upper, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
 }
 // Sequential BCSR forms: the group-level dependency levels are computed from a host
 // copy of the block pattern on every call (the reference runs these serially too).
@@ -570,25 +616,55 @@ static int bcsr_seq_schedule(senk_ctx ctx, const int *dbcind, const int *dbrptr,
     s->nlev = (int)s->lev_ptr.size() - 1;
     return upload(ctx, &s->dblk, blk.data(), blk.size());
 }
-static int bcsr_seq(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
-                    double *dy, int N, int bnl, int bnw, bool upper) {
+template <typename T>
+static int bcsr_seq(senk_ctx ctx, const T *dbval, const int *dbcind, const int *dbrptr, const T *dx, T *dy, int N,
+                    int bnl, int bnw, bool upper) {
     CTX_CHECK(ctx);
     if (bnl <= 0 || bnw <= 0 || N < 0) return SENK_ERR_ARG;
     if (N % bnl || bnl % bnw) return SENK_ERR_DIVISIBILITY;
     Schedule s;
     SENK_TRY(bcsr_seq_schedule(ctx, dbcind, dbrptr, N, bnl, bnw, upper, &s));
-    const int st = launch_sptrsv_bcsr<double>(&ctx->c, s, upper, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
+    const int st = launch_sptrsv_bcsr<T>(&ctx->c, s, upper, bnl, bnw, dbrptr, dbcind, dbval, dx, dy);
     SENK_HIP(hipStreamSynchronize(ctx->c.stream));
     (void)hipFree(s.dblk);
     return st;
 }
+extern "C" {
+int senk_sptrsv_bcsr_l_abmc_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
+                                const double *dx, double *dy, int N, const int *cptr, int cnum, int bsize, int bnl,
+                                int bnw) {
+    return bcsr_abmc<double>(ctx, dbval, dbcind, dbrptr, dx, dy, N, cptr, cnum, bsize, bnl, bnw, false);
+}
+int senk_sptrsv_bcsr_u_abmc_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr,
+                                const double *dx, double *dy, int N, const int *cptr, int cnum, int bsize, int bnl,
+                                int bnw) {
+    return bcsr_abmc<double>(ctx, dbval, dbcind, dbrptr, dx, dy, N, cptr, cnum, bsize, bnl, bnw, true);
+}
+int senk_sptrsv_bcsr_l_abmc_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr,
+                                const float *dx, float *dy, int N, const int *cptr, int cnum, int bsize, int bnl,
+                                int bnw) {
+    return bcsr_abmc<float>(ctx, dbval, dbcind, dbrptr, dx, dy, N, cptr, cnum, bsize, bnl, bnw, false);
+}
+int senk_sptrsv_bcsr_u_abmc_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr,
+                                const float *dx, float *dy, int N, const int *cptr, int cnum, int bsize, int bnl,
+                                int bnw) {
+    return bcsr_abmc<float>(ctx, dbval, dbcind, dbrptr, dx, dy, N, cptr, cnum, bsize, bnl, bnw, true);
+}
 int senk_sptrsv_bcsr_l_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
                            double *dy, int N, int bnl, int bnw) {
-    return bcsr_seq(ctx, dbval, dbcind, dbrptr, dx, dy, N, bnl, bnw, false);
+    return bcsr_seq<double>(ctx, dbval, dbcind, dbrptr, dx, dy, N, bnl, bnw, false);
 }
 int senk_sptrsv_bcsr_u_f64(senk_ctx ctx, const double *dbval, const int *dbcind, const int *dbrptr, const double *dx,
                            double *dy, int N, int bnl, int bnw) {
-    return bcsr_seq(ctx, dbval, dbcind, dbrptr, dx, dy, N, bnl, bnw, true);
+    return bcsr_seq<double>(ctx, dbval, dbcind, dbrptr, dx, dy, N, bnl, bnw, true);
+}
+int senk_sptrsv_bcsr_l_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr, const float *dx,
+                           float *dy, int N, int bnl, int bnw) {
+    return bcsr_seq<float>(ctx, dbval, dbcind, dbrptr, dx, dy, N, bnl, bnw, false);
+}
+int senk_sptrsv_bcsr_u_f32(senk_ctx ctx, const float *dbval, const int *dbcind, const int *dbrptr, const float *dx,
+                           float *dy, int N, int bnl, int bnw) {
+    return bcsr_seq<float>(ctx, dbval, dbcind, dbrptr, dx, dy, N, bnl, bnw, true);
 }
 
 // ---------------------------------------------------------------- BLAS-1
@@ -638,11 +714,11 @@ int senk_nrm2_f64(senk_ctx ctx, const double *dx, int N, double *result) {
 }
 int senk_hadprod_f64(senk_ctx ctx, const double *dx, double *dy, int N) {
     CTX_CHECK(ctx);
-    return launch_hadprod_f64(&ctx->c, N, dx, dy);
+    return launch_hadprod<double>(&ctx->c, N, dx, dy);
 }
 int senk_haddiv_f64(senk_ctx ctx, const double *dx, double *dy, int N) {
     CTX_CHECK(ctx);
-    return launch_haddiv_f64(&ctx->c, N, dx, dy);
+    return launch_haddiv<double>(&ctx->c, N, dx, dy);
 }
 int senk_copy_f32(senk_ctx ctx, const float *dx, float *dy, int N) {
     CTX_CHECK(ctx);
@@ -655,6 +731,22 @@ int senk_scal_f32(senk_ctx ctx, float a, float *dx, int N) {
 int senk_axpy_f32(senk_ctx ctx, float a, const float *dx, float *dy, int N) {
     CTX_CHECK(ctx);
     return launch_axpby<float>(&ctx->c, N, a, dx, 1.0, dy);
+}
+int senk_axpby_f32(senk_ctx ctx, float a, const float *dx, float b, float *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_axpby<float>(&ctx->c, N, a, dx, b, dy);
+}
+int senk_axpyz_f32(senk_ctx ctx, float a, const float *dx, const float *dy, float *dz, int N) {
+    CTX_CHECK(ctx);
+    return launch_axpyz<float>(&ctx->c, N, a, dx, dy, dz);
+}
+int senk_hadprod_f32(senk_ctx ctx, const float *dx, float *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_hadprod<float>(&ctx->c, N, dx, dy);
+}
+int senk_haddiv_f32(senk_ctx ctx, const float *dx, float *dy, int N) {
+    CTX_CHECK(ctx);
+    return launch_haddiv<float>(&ctx->c, N, dx, dy);
 }
 int senk_dot_f32(senk_ctx ctx, const float *dx, const float *dy, int N, double *result) {
     CTX_CHECK(ctx);
@@ -736,6 +828,22 @@ int senk_ilu0_f32(senk_ctx ctx, float *dval, const int *dcind, const int *drptr,
 }
 
 // ---------------------------------------------------------------- solver twins
+static int raw_mat(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr, int N, MatView *A);
+int senk_relative_residual_error(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                                 const double *db, const double *dx, int N, int print, double *relres) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || !dval || !dcind || !drptr || !db || !dx || !relres) return SENK_ERR_ARG;
+    double nb = 0;
+    SENK_TRY(senk_nrm2_f64(ctx, db, N, &nb));
+    MatView A;
+    SENK_TRY(raw_mat(ctx, dval, dcind, drptr, N, &A));
+    SENK_TRY(true_relres(&ctx->c, A, db, dx, nb, relres));
+    if (print) {
+        std::printf("# test %e\n", *relres);
+        std::fflush(stdout);
+    }
+    return SENK_OK;
+}
 static int finish(senk_ctx ctx, const MatView &A, const double *db, const double *dx, double nrm_b, senk_result *res) {
     double tr = 0;
     SENK_TRY(true_relres(&ctx->c, A, db, dx, nrm_b, &tr));
@@ -1210,7 +1318,7 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
     int st = SENK_OK;
     const bool ic = opts->kind == SENK_IC0 || opts->kind == SENK_ICP;
     if (opts->kind < SENK_ILU0 || opts->kind > SENK_ICP) st = SENK_ERR_ARG;
-    else if (ic && !symmetric_csr(N, rptr, cind, val)) {
+    else if (ic && !symmetric_csr(N, rptr, cind, val, kSymTol)) {
         record_error("IC(0)/IC(p): the matrix is not symmetric");
         st = SENK_ERR_ARG;
     } else if ((opts->kind == SENK_ILUP || opts->kind == SENK_ICP) && opts->p > 0)
@@ -1220,6 +1328,9 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
         std::free(fv); std::free(fc); std::free(fr);
         return st;
     }
+    // IC: factor the symmetric matrix of A's lower triangle (A may differ from its
+    // transpose in the last bits, e.g. after Jacobi scaling)
+    if (ic) mirror_lower(N, fr, fc, fv);
     if ((st = set_sell_order(A, cptr, cnum, bsize))) {
         std::free(fv); std::free(fc); std::free(fr);
         return st;
@@ -1350,6 +1461,9 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
                 if ((st = launch_dilu_diag(c, N, ur, u64, A->rptr, A->cind, A->v64, M->dilu_d, M->dilu_e))) break;
                 v.dilu_d = M->dilu_d;
                 v.dilu_e = M->dilu_e;
+                // the split inner CG needs a symmetric preconditioner: IC, or ILU(0) of a
+                // symmetric A (U = D L^T to rounding)
+                v.symmetric = ic || symmetric_csr(N, rptr, cind, val, kSymTol);
             }
         } else {
             tri_layout_free(&M->lt);

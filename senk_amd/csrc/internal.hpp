@@ -183,8 +183,10 @@ template <typename T>
 int launch_scal(Ctx *c, int N, double a, T *x);
 template <typename T>
 int launch_copy(Ctx *c, int N, const T *x, T *y);
-int launch_hadprod_f64(Ctx *c, int N, const double *x, double *y);
-int launch_haddiv_f64(Ctx *c, int N, const double *x, double *y);
+template <typename T>
+int launch_hadprod(Ctx *c, int N, const T *x, T *y);                        // y *= x
+template <typename T>
+int launch_haddiv(Ctx *c, int N, const T *x, T *y);                         // y /= x
 template <typename TI, typename TO>
 int launch_convert(Ctx *c, int N, const TI *in, TO *out, const double *dscale);
 // result[k] = sum_i x_k[i]*y[i] for k < K (x_k = xs + k*ldx), FP64 accumulation,

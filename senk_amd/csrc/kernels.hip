@@ -904,9 +904,10 @@ __global__ __launch_bounds__(kBlock) void k_copy(int N, const T *__restrict__ x,
         put4(y, i0, N, xv);
     }
 }
-__global__ __launch_bounds__(kBlock) void k_hadprod(int N, const double *__restrict__ x, double *y) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hadprod(int N, const T *__restrict__ x, T *y) {
     SENK_FOR4(i0, N) {
-        double xv[4], yv[4];
+        T xv[4], yv[4];
         get4(x, i0, N, xv);
         get4(y, i0, N, yv);
 #pragma unroll
@@ -914,9 +915,10 @@ __global__ __launch_bounds__(kBlock) void k_hadprod(int N, const double *__restr
         put4(y, i0, N, yv);
     }
 }
-__global__ __launch_bounds__(kBlock) void k_haddiv(int N, const double *__restrict__ x, double *y) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_haddiv(int N, const T *__restrict__ x, T *y) {
     SENK_FOR4(i0, N) {
-        double xv[4], yv[4];
+        T xv[4], yv[4];
         get4(x, i0, N, xv);
         get4(y, i0, N, yv);
 #pragma unroll
@@ -974,15 +976,17 @@ int launch_copy(Ctx *c, int N, const T *x, T *y) {
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
-int launch_hadprod_f64(Ctx *c, int N, const double *x, double *y) {
+template <typename T>
+int launch_hadprod(Ctx *c, int N, const T *x, T *y) {
     if (N <= 0) return SENK_OK;
-    k_hadprod<<<ew_grid(N), kBlock, 0, c->stream>>>(N, x, y);
+    k_hadprod<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, x, y);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
-int launch_haddiv_f64(Ctx *c, int N, const double *x, double *y) {
+template <typename T>
+int launch_haddiv(Ctx *c, int N, const T *x, T *y) {
     if (N <= 0) return SENK_OK;
-    k_haddiv<<<ew_grid(N), kBlock, 0, c->stream>>>(N, x, y);
+    k_haddiv<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, x, y);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
@@ -1001,6 +1005,10 @@ template int launch_scal<double>(Ctx *, int, double, double *);
 template int launch_scal<float>(Ctx *, int, double, float *);
 template int launch_copy<double>(Ctx *, int, const double *, double *);
 template int launch_copy<float>(Ctx *, int, const float *, float *);
+template int launch_hadprod<double>(Ctx *, int, const double *, double *);
+template int launch_hadprod<float>(Ctx *, int, const float *, float *);
+template int launch_haddiv<double>(Ctx *, int, const double *, double *);
+template int launch_haddiv<float>(Ctx *, int, const float *, float *);
 template int launch_convert<double, float>(Ctx *, int, const double *, float *, const double *);
 template int launch_convert<float, double>(Ctx *, int, const float *, double *, const double *);
 template int launch_convert<double, double>(Ctx *, int, const double *, double *, const double *);

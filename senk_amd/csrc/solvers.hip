@@ -1521,7 +1521,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_start(int N, const float *__rest
     store_partials<2>(v, Pout);
 }
 // sc[2] = rho = (r, r/d); dtol = {eps ||r||, ||r||}; a zero r stops as converged, a
-// non-finite one as a breakdown
+// non-finite one or a negative rho (a pivot d < 0: M not positive definite) as a breakdown
 __global__ __launch_bounds__(kBlock) void k_cgs_init(const double *P, int B, double *sc, int *state, double eps,
                                                      double *dtol) {
     const double rr = sum_partials(P, B), rho = sum_partials(P + kPartLd, B);
@@ -1532,7 +1532,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_init(const double *P, int B, dou
     state[2] = 0;
     dtol[0] = eps * sqrt(rr);
     dtol[1] = sqrt(rr);
-    if (!isfinite(rr) || !isfinite(rho)) {
+    if (!isfinite(rr) || !isfinite(rho) || rho < 0) {
         state[0] = 1;
         state[2] = 1;
     } else if (!(rr > 0)) {
@@ -1600,7 +1600,8 @@ __global__ __launch_bounds__(kBlock) void k_cgs_u(int N, const float *__restrict
     }
     const double nr = sqrt(sum_partials(Pin, B));
     const double rho_new = sum_partials(Pin + kPartLd, B);
-    const bool fin = !isfinite(nr) || !isfinite(rho_new), conv = nr < dtol[0];
+    // rho_new < 0: (r, D^-1 r) of an indefinite D, a breakdown as in the plain CG
+    const bool fin = !isfinite(nr) || !isfinite(rho_new) || rho_new < 0, conv = nr < dtol[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (hist && iter < hcap) hist[iter] = nr / dtol[1];
         state[1] = iter + 1;
@@ -1712,7 +1713,7 @@ static int cg(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, con
               senk_result *res, double *hist, int hist_cap) {
     if (M.learned) return SENK_ERR_ARG;  // CG needs a fixed, symmetric preconditioner
     if constexpr (std::is_same<T, float>::value) {
-        if (p.split && M.dilu_d && M.dilu_e && M.lt && M.ut && !M.f16)
+        if (p.split && M.dilu_d && M.dilu_e && M.lt && M.ut && !M.f16 && M.symmetric)
             return cg_split(c, A, M, b, x, p, res, hist, hist_cap);
     }
     const int N = A.N;

@@ -53,6 +53,8 @@ struct PrecView {
     // of the mixed forms' inner BiCGStab; null when the factorization is not D-ILU or the
     // split form is switched off
     const float *dilu_d = nullptr, *dilu_e = nullptr;
+    // A (to a few ulps) and the factorization symmetric: the split form of the inner CG applies
+    bool symmetric = false;
     // learned correction added to the ILU application (flexible solvers only)
     LearnedPrec *learned = nullptr;
     int *learned_off = nullptr;  // device flag: 1 = correction disabled by the monitor

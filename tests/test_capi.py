@@ -53,13 +53,39 @@ def test_ggen_grot_trsv_host_functions():
     assert abs(a.value - 5.0) < 1e-15 and abs(b.value) < 1e-15
 
 
-@pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")
-def test_dropin_header_compiles(tmp_path):
-    hdr = ROOT / "include" / "senk_gpu.hpp"
-    if not hdr.exists():
-        pytest.skip("drop-in header not present")
-    src = tmp_path / "t.cpp"
-    src.write_text('#include "senk_gpu.hpp"\nint main(){return 0;}\n')
-    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT / 'include'}", str(src)],
-                       capture_output=True, text=True)
+REF_INCLUDE = Path("/root/reference/include")
+needs_gxx = pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")
+
+
+def _syntax(src, *extra, std="c++17"):
+    return subprocess.run(["g++", f"-std={std}", "-fsyntax-only", "-Wall", f"-I{ROOT / 'include'}", *extra, str(src)],
+                          capture_output=True, text=True)
+
+
+@needs_gxx
+def test_dropin_header_compiles_every_template():
+    """Every drop-in template for double and float, as tests/dropin/instantiate.cpp calls
+    them with the reference's argument types."""
+    r = _syntax(ROOT / "tests" / "dropin" / "instantiate.cpp")
     assert r.returncode == 0, r.stderr
+
+
+@needs_gxx
+@pytest.mark.skipif(not (REF_INCLUDE / "senk.hpp").exists(), reason="reference headers not present")
+def test_dropin_header_beside_reference_headers(tmp_path):
+    """Included first, the drop-in takes the solver / kernel / BLAS headers' place and the
+    reference's senk.hpp (its host setup code) compiles after it, as the integration notes
+    show; included after the reference's headers it is refused with its own #error."""
+    r = _syntax(ROOT / "tests" / "dropin" / "instantiate.cpp", "-DSENK_WITH_REFERENCE", f"-I{REF_INCLUDE}", "-fopenmp",
+                std="c++20")
+    assert r.returncode == 0, r.stderr
+    snippet = re.search(r"```cpp\n(#include \"senk_gpu.hpp\".*?)```", (ROOT / "INTEGRATION.md").read_text(), re.S)
+    assert snippet, "INTEGRATION.md lost its drop-in snippet"
+    src = tmp_path / "snippet.cpp"
+    src.write_text(snippet.group(1) + "int main() { return 0; }\n")
+    r = _syntax(src, f"-I{REF_INCLUDE}", "-fopenmp", std="c++20")
+    assert r.returncode == 0, r.stderr
+    bad = tmp_path / "bad.cpp"
+    bad.write_text('#include "senk.hpp"\n#include "senk_gpu.hpp"\nint main() { return 0; }\n')
+    r = _syntax(bad, f"-I{REF_INCLUDE}", "-fopenmp", std="c++20")
+    assert r.returncode != 0 and "include it before senk.hpp" in r.stderr

@@ -285,3 +285,47 @@ def test_split_inner_bicgstab_with_ic_factors(ctx, mixed):
     finally:
         S.lib().senk_prec_destroy(M)
         S.lib().senk_mat_destroy(A)
+
+
+def test_ic0_after_jacobi_scaling(ctx):
+    """The reference's workflow scales A by D^-1/2 on both sides before factorizing
+    (main.cpp:66-73, Scaling senk_matrix.hpp:406-416: ld_i * a_ij * rd_j); the products'
+    rounding leaves a_ij and a_ji unequal in the last bit for many entries.  IC(0) accepts
+    that matrix (symmetric to a few ulps), factors the symmetric matrix of its lower
+    triangle (L bit for bit the oracle's ILU(0) of that matrix, U = D L^T), and FP64 CG and
+    the split inner CG converge on the scaled system."""
+    sy = _system(16, 8, seed=6)
+    N = sy["N"]
+    d = np.zeros(N)
+    o.or_get_diag_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(d), N)
+    s = 1 / np.sqrt(np.abs(d))
+    v = sy["val"].copy()
+    o.or_scaling_d(O.P(v), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(s), O.P(s), N)
+    rows = np.repeat(np.arange(N), np.diff(sy["rptr"]))
+    T = {(int(c), int(r)): x for r, c, x in zip(rows, sy["cind"], v)}
+    asym = sum(T[(int(r), int(c))] != x for r, c, x in zip(rows, sy["cind"], v))
+    assert asym > 0  # the scaled matrix is not bit-symmetric
+    sc = dict(sy, val=v)
+    xs = sy["xs"] / s  # the scaled system's solution for the same b' = D^-1/2 b
+    sc["b"] = sy["b"] * s
+    sc["xs"] = xs
+    A, M, st = _handles(ctx, sc)
+    assert st == 0, S.lib().senk_last_error()
+    try:
+        (lv, lc, lr), (uv, uc, ur) = _download(M, N)
+        mirrored = v.copy()
+        for e, (r, c) in enumerate(zip(rows, sy["cind"])):
+            if c > r:
+                mirrored[e] = T[(int(r), int(c))]
+        ilu = S.split(oracle_ilu0(mirrored, sy["cind"], sy["rptr"]), sy["cind"], sy["rptr"], N, "L-DU", True)
+        assert np.array_equal(lv, ilu["L"][0]) and np.array_equal(uv[ur[:-1]], ilu["U"][0][ur[:-1]])
+        for kw in (dict(solver=S.SENK_CG, eps=1e-10, max_iter=5000),
+                   dict(solver=S.SENK_CG, eps=1e-10, mixed=2, inner_eps=1e-3, inner_max=60, outer_m=20,
+                        max_iter=3000)):
+            st, res, x = _solve(ctx, A, M, sc, **kw)
+            assert st == 0 and res.converged == 1 and res.status == 0, kw
+            assert res.true_relres <= 1.5e-10
+            assert relerr(x, xs) <= 1e-7
+    finally:
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)

@@ -335,6 +335,136 @@ def test_blas_match_reference():
     assert np.array_equal(x1[:m], x2[:m])
 
 
+@needs_ref
+@pytest.mark.parametrize("name", list(MATS))
+def test_float_kernels_and_copy_match_reference_bitwise(name):
+    """The FP32 sequential SpTRSV (senk_sparse.hpp:132-167), the FP32 SELL and BCSR SpMV
+    (:103-121, :70-91) and blas1::Copy (senk_blas1.hpp:26-30) against the compiled
+    reference's own float instantiations, bit for bit."""
+    rptr, cind, val = MATS[name]
+    N = len(rptr) - 1
+    x = RNG.uniform(-1, 1, N)
+    c1, c2 = np.zeros(N), np.full(N, np.nan)
+    o.or_copy_d(P(x), P(c1), N)
+    ref.ref_copy_d(P(x), P(c2), N)
+    assert np.array_equal(c1, c2) and np.array_equal(c1, x)
+    x32 = x.astype(np.float32)
+    L, U = oracle_split_ldu(oracle_ilu0(val, cind, rptr), cind, rptr)
+    L32 = (L[0].astype(np.float32), L[1], L[2])
+    U32 = (U[0].astype(np.float32), U[1], U[2])
+    yl, yu, zl, zu = (np.zeros(N, np.float32) for _ in range(4))
+    o.or_sptrsv_l_f(P(L32[0]), P(L32[1]), P(L32[2]), P(x32), P(yl), N)
+    o.or_sptrsv_u_f(P(U32[0]), P(U32[1]), P(U32[2]), P(yl), P(yu), N)
+    ref.ref_sptrsv_l_f(P(L32[0]), P(L32[1]), P(L32[2]), P(x32), P(zl), N)
+    ref.ref_sptrsv_u_f(P(U32[0]), P(U32[1]), P(U32[2]), P(zl), P(zu), N)
+    assert np.array_equal(yl, zl) and np.array_equal(yu, zu)
+    # in place, as the ILU drivers call the backward solve (x == y)
+    w = yl.copy()
+    o.or_sptrsv_u_f(P(U32[0]), P(U32[1]), P(U32[2]), P(w), P(w), N)
+    assert np.array_equal(w, yu)
+    v32 = val.astype(np.float32)
+    for size in (64, 32, 8, 1):
+        if N % size:  # the reference's SpmvSell is only right for full slices
+            continue
+        ns = N // size
+        sl = np.zeros(ns + 1, np.int32)
+        tot = o.or_csr2sell_count(P(rptr), P(sl), size, N)
+        sv, sc = np.zeros(tot), np.zeros(tot, np.int32)
+        o.or_csr2sell_fill_d(P(val), P(cind), P(rptr), P(sl), P(sv), P(sc), size, N)
+        sv32 = sv.astype(np.float32)
+        a, b = np.zeros(N, np.float32), np.full(N, np.nan, np.float32)
+        o.or_spmv_sell_f(P(sv32), P(sc), P(sl), size, P(x32), P(a), N)
+        ref.ref_spmv_sell_f(P(sv32), P(sc), P(sl), size, P(x32), P(b), N)
+        assert np.array_equal(a, b), size
+    for bnl, bnw in ((3, 3), (2, 1), (2, 2), (4, 2), (8, 1)):
+        if N % bnl or N % bnw:
+            continue
+        br = np.zeros(N // bnl + 1, np.int32)
+        nb = o.or_csr2bcsr_count(P(cind), P(rptr), P(br), N, bnl, bnw)
+        bc, bv = np.zeros(nb, np.int32), np.zeros(nb * bnl * bnw)
+        o.or_csr2bcsr_fill_d(P(val), P(cind), P(rptr), P(br), P(bv), P(bc), N, bnl, bnw)
+        bv32 = bv.astype(np.float32)
+        a, b = np.zeros(N, np.float32), np.full(N, np.nan, np.float32)
+        o.or_spmv_bcsr_f(P(bv32), P(bc), P(br), P(x32), P(a), N, bnl, bnw)
+        assert ref.ref_spmv_bcsr_f(P(bv32), P(bc), P(br), P(x32), P(b), N, bnl, bnw) == 0
+        assert np.array_equal(a, b), (bnl, bnw)
+
+
+@pytest.mark.parametrize("name", list(MATS))
+def test_float_kernels_closed_forms(name):
+    """The same FP32 functions without the reference: the SELL and BCSR products equal
+    the FP32 CSR product bit for bit (their padding and block fill add exact zeros to the
+    same ordered row sums) and the FP64 dense product to FP32 rounding; the FP32 solves
+    satisfy L y = x and U z = y to FP32 rounding and agree with the FP64 solves."""
+    rptr, cind, val = MATS[name]
+    N = len(rptr) - 1
+    A = dense(rptr, cind, val, N)
+    x = RNG.uniform(-1, 1, N)
+    x32, v32 = x.astype(np.float32), val.astype(np.float32)
+    csr = np.zeros(N, np.float32)
+    o.or_spmv_csr_f(P(v32), P(cind), P(rptr), P(x32), P(csr), N)
+    exact = dense(rptr, cind, v32.astype(np.float64), N) @ x32.astype(np.float64)
+    scale = np.abs(A) @ np.abs(x)
+    assert np.all(np.abs(csr - exact) <= 64 * np.finfo(np.float32).eps * scale)
+    for size in (64, 32, 7, 1):
+        ns = (N + size - 1) // size
+        sl = np.zeros(ns + 1, np.int32)
+        tot = o.or_csr2sell_count(P(rptr), P(sl), size, N)
+        sv, sc = np.zeros(tot), np.zeros(tot, np.int32)
+        o.or_csr2sell_fill_d(P(val), P(cind), P(rptr), P(sl), P(sv), P(sc), size, N)
+        a = np.full(N, np.nan, np.float32)
+        o.or_spmv_sell_f(P(sv.astype(np.float32)), P(sc), P(sl), size, P(x32), P(a), N)
+        assert np.array_equal(a, csr), size
+    for bnl, bnw in ((3, 3), (2, 1), (2, 2), (4, 2), (8, 1)):
+        if N % bnl or N % bnw:
+            continue
+        br = np.zeros(N // bnl + 1, np.int32)
+        nb = o.or_csr2bcsr_count(P(cind), P(rptr), P(br), N, bnl, bnw)
+        bc, bv = np.zeros(nb, np.int32), np.zeros(nb * bnl * bnw)
+        o.or_csr2bcsr_fill_d(P(val), P(cind), P(rptr), P(br), P(bv), P(bc), N, bnl, bnw)
+        a = np.full(N, np.nan, np.float32)
+        o.or_spmv_bcsr_f(P(bv.astype(np.float32)), P(bc), P(br), P(x32), P(a), N, bnl, bnw)
+        assert np.array_equal(a, csr), (bnl, bnw)
+    f = oracle_ilu0(val, cind, rptr)
+    F = dense(rptr, cind, f, N)
+    Lm, Um = np.eye(N) + np.tril(F, -1), np.triu(F)
+    L, U = oracle_split_ldu(f, cind, rptr)
+    L32 = (L[0].astype(np.float32), L[1], L[2])
+    U32 = (U[0].astype(np.float32), U[1], U[2])
+    yl, yu = np.zeros(N, np.float32), np.zeros(N, np.float32)
+    o.or_sptrsv_l_f(P(L32[0]), P(L32[1]), P(L32[2]), P(x32), P(yl), N)
+    o.or_sptrsv_u_f(P(U32[0]), P(U32[1]), P(U32[2]), P(yl), P(yu), N)
+    eps32 = np.finfo(np.float32).eps
+    assert np.linalg.norm(Lm @ yl - x32) <= 1e3 * eps32 * np.linalg.norm(x32)
+    assert np.linalg.norm(Um @ yu - yl) <= 1e3 * eps32 * np.linalg.norm(yl) * np.linalg.cond(Um)
+    dl, du = np.zeros(N), np.zeros(N)
+    o.or_sptrsv_l_d(P(L[0]), P(L[1]), P(L[2]), P(x32.astype(np.float64)), P(dl), N)
+    o.or_sptrsv_u_d(P(U[0]), P(U[1]), P(U[2]), P(dl), P(du), N)
+    assert relerr(yu, du) <= 1e3 * eps32 * np.linalg.cond(Lm @ Um)
+    c = np.full(N, np.nan)
+    o.or_copy_d(P(x), P(c), N)
+    assert np.array_equal(c, x)
+
+
+@pytest.mark.parametrize("name", ["poisson7_6", "unstructured_500", "convdiff27_5"])
+def test_relative_residual_pinned(capfd, name):
+    """or_relative_residual_d (senk_test.hpp:11-22: ||b - A x|| / ||b|| in FP64) against
+    the line the compiled reference prints ("# test %e") and against numpy's dense form."""
+    rptr, cind, val = MATS[name]
+    N = len(rptr) - 1
+    A = dense(rptr, cind, val, N)
+    b, x = RNG.uniform(-1, 1, N), RNG.uniform(-1, 1, N) * 1e-3
+    got = o.or_relative_residual_d(P(val), P(cind), P(rptr), P(b), P(x), N)
+    want = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert abs(got - want) <= 1e-13 * want
+    if ref is not None:
+        capfd.readouterr()
+        ref.ref_relative_residual_error(P(val), P(cind), P(rptr), P(b), P(x), N)
+        ref.ref_flush()
+        line = [ln for ln in capfd.readouterr().out.splitlines() if ln.startswith("# test")][-1]
+        assert line == "# test %e" % got, (line, got)
+
+
 def _ref_iters(capfd, fn):
     capfd.readouterr()
     fn()

@@ -89,6 +89,37 @@ def test_reordering_split_scaling_bit_exact(name):
 
 
 @pytest.mark.parametrize("name", list(CASES))
+def test_get_diag_scaling_remove_off_diagonal_bit_exact(name):
+    """senk_get_diag / senk_scaling / senk_remove_off_diagonal (senk_matrix.hpp:391-416,
+    :668-689) against or_get_diag_d / or_scaling_d / or_remove_off_diagonal_d, bit for
+    bit, with main.cpp:66-73's scale D^-1/2 and with distinct left and right scales."""
+    rptr, cind, val = CASES[name]()
+    N = len(rptr) - 1
+    pd = C.c_void_p()
+    S.call("senk_get_diag", S._ptr(val), S._ptr(cind), S._ptr(rptr), C.byref(pd), N)
+    d = S._take(pd, N, np.float64)
+    d2 = np.full(N, np.nan)
+    o.or_get_diag_d(P(val), P(cind), P(rptr), P(d2), N)
+    assert np.array_equal(d, d2)
+    assert np.array_equal(d, np.diag(dense(rptr, cind, val, N)))
+    rng = np.random.default_rng(11)
+    for ld, rd in ((1 / np.sqrt(np.abs(d)), 1 / np.sqrt(np.abs(d))), (rng.uniform(0.5, 2, N), rng.uniform(0.5, 2, N))):
+        v1, v2 = val.copy(), val.copy()
+        S.call("senk_scaling", S._ptr(v1), S._ptr(cind), S._ptr(rptr), S._ptr(ld), S._ptr(rd), N)
+        o.or_scaling_d(P(v2), P(cind), P(rptr), P(ld), P(rd), N)
+        assert np.array_equal(v1, v2)
+    for bnum in (1, 2, 4, 7):  # 7 leaves a short last block on most of these sizes
+        v, c, r = S.remove_off_diagonal(val, cind, rptr, N, bnum)
+        v2, c2, r2 = val.copy(), cind.copy(), rptr.copy()
+        n = o.or_remove_off_diagonal_d(P(v2), P(c2), P(r2), N, bnum)
+        assert len(v) == n and np.array_equal(r, r2) and np.array_equal(c, c2[:n]) and np.array_equal(v, v2[:n])
+        # closed form: the entries inside the bnum diagonal blocks of N/bnum rows
+        rows = np.repeat(np.arange(N), np.diff(rptr))
+        keep = rows // (N // bnum) == cind // (N // bnum)
+        assert np.array_equal(c, cind[keep]) and np.array_equal(v, val[keep])
+
+
+@pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("p", [1, 2])
 def test_alloc_level_zero_bit_exact(name, p):
     rptr, cind, val = CASES[name]()
