@@ -72,13 +72,20 @@ def ilu_bytes(n, lnnz, unnz, sv, si=4):
 
 
 def build_system_kind(kind, n, bsize, seed=1):
-    """kind "convdiff": 27-point convection-diffusion, Pe 10; "vardiff": 7-point
-    -div(k grad u) with k = exp(Gaussian random field, correlation length 8, sigma 1.5)."""
+    """kind "convdiff": 27-point convection-diffusion on n^3, Pe 10; "vardiff": 7-point
+    -div(k grad u) on n^3 with k = exp(Gaussian random field, correlation length 8, sigma
+    1.5); "unstructured": n rows, lognormal row lengths, banded-random columns, diagonal 1.1x
+    the row's absolute sum; "elasticity": 3 dof per node on n^3, 3x3-block 7-point stencil,
+    nu 0.3, lognormal moduli."""
     t0 = time.perf_counter()
     if kind == "convdiff":
         rptr, cind, val = S.gen_convdiff27(n, 10.0)
     elif kind == "vardiff":
         rptr, cind, val = S.gen_vardiff7(n, seed, 8.0, 1.5)
+    elif kind == "unstructured":
+        rptr, cind, val = S.gen_unstructured(n, seed)
+    elif kind == "elasticity":
+        rptr, cind, val = S.gen_elasticity3(n, seed)
     else:
         raise ValueError(kind)
     N = len(rptr) - 1
@@ -98,7 +105,7 @@ class Device:
     """A system on the device: matrix (FP64 + FP32 values, SELL-C-sigma) and ILU(0) (FP64 + FP32);
     kind=S.SENK_IC0 for the incomplete Cholesky factor instead."""
 
-    def __init__(self, ctx, sy, kind=S.SENK_ILU0):
+    def __init__(self, ctx, sy, kind=S.SENK_ILU0, p=0, bnl=0, bnw=0):
         self.ctx, self.sy, self.N = ctx, sy, sy["N"]
         self.A = C.c_void_p()
         t0 = time.perf_counter()
@@ -106,7 +113,7 @@ class Device:
                C.byref(self.A))
         self.t_upload = time.perf_counter() - t0
         self.M = C.c_void_p()
-        opts = S.senk_ilu_opts(kind, 0, 0, 0, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32), 0)
+        opts = S.senk_ilu_opts(kind, p, bnl, bnw, (1 << S.SENK_FACTOR_F64) | (1 << S.SENK_FACTOR_F32), 0)
         t0 = time.perf_counter()
         S.call("senk_prec_create_ilu", ctx.h, self.A, S._ptr(sy["rptr"]), S._ptr(sy["cind"]), S._ptr(sy["val"]),
                S._ptr(sy["cptr"]), sy["nc"], sy["bsize"], C.byref(opts), C.byref(self.M))
@@ -116,14 +123,25 @@ class Device:
         S.call("senk_prec_factor_nnz", self.M, C.byref(ln), C.byref(un), None, None)
         self.lnnz, self.unnz = ln.value, un.value
 
-    def solve(self, **kw):
+    def solve(self, keep_x=False, **kw):
         dx = self.ctx.zeros(self.N)
         o = S.solve_opts(**kw)
         res = S.senk_result()
         S.call("senk_solve", self.ctx.h, self.A, self.M, self.db.p, dx.p, C.byref(o), C.byref(res), None, 0)
         d = res.as_dict()
-        d["err_vs_xstar"] = float(np.linalg.norm(dx.download() - self.sy["xs"]) / np.linalg.norm(self.sy["xs"]))
+        x = dx.download()
+        d["err_vs_xstar"] = float(np.linalg.norm(x - self.sy["xs"]) / np.linalg.norm(self.sy["xs"]))
+        if keep_x:
+            d["x"] = x
         return d
+
+    def factors(self):
+        """The device factors (CSR: L strict lower, U with the inverted diagonal first)."""
+        N = self.N
+        lr, lc, lv = np.zeros(N + 1, np.int32), np.zeros(self.lnnz, np.int32), np.zeros(self.lnnz)
+        ur, uc, uv = np.zeros(N + 1, np.int32), np.zeros(self.unnz, np.int32), np.zeros(self.unnz)
+        S.call("senk_prec_download", self.M, S._ptr(lr), S._ptr(lc), S._ptr(lv), S._ptr(ur), S._ptr(uc), S._ptr(uv))
+        return (lv, lc, lr), (uv, uc, ur)
 
     def close(self):
         S.lib().senk_prec_destroy(self.M)
@@ -205,19 +223,117 @@ def traffic_from_profile(name):
         return None
 
 
-def timed(dev, steps, warmup, **kw):
+def timed(dev, steps, warmup, keep_x=False, **kw):
     for _ in range(warmup):
         dev.solve(**kw)
-    runs = [dev.solve(**kw) for _ in range(steps)]
+    runs = [dev.solve(keep_x=keep_x, **kw) for _ in range(steps)]
     d = runs[-1]
     d["t_solve_ms_median"] = statistics.median(r["t_solve_ms"] for r in runs)
     return d
 
 
-def cpu_baseline(dev, gpu_iters):
+def host_info():
+    """The host the CPU baseline runs on: CPU model, sockets x cores (lscpu), the CPUs this
+    process may use, and the OpenMP settings the reference runs with."""
+    info = {"cpu_model": None, "physical_cores": None, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {ln.split(":", 1)[0].strip(): ln.split(":", 1)[1].strip() for ln in out.splitlines() if ":" in ln}
+        info["cpu_model"] = kv.get("Model name")
+        if kv.get("Socket(s)", "").isdigit() and kv.get("Core(s) per socket", "").isdigit():
+            info["physical_cores"] = int(kv["Socket(s)"]) * int(kv["Core(s) per socket"])
+    except Exception as e:  # pragma: no cover
+        info["lscpu_error"] = str(e)
+    return info
+
+
+def ref_call(ref, fn):
+    """Run a reference call with the process's stdout captured (the reference prints its
+    iteration count and residual); returns (seconds, iterations, '# res' line)."""
+    import tempfile
+    sys.stdout.flush()
+    with tempfile.TemporaryFile(mode="w+b") as tmp:
+        saved = os.dup(1)
+        os.dup2(tmp.fileno(), 1)
+        try:
+            t0 = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t0
+            ref.ref_flush()
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        tmp.seek(0)
+        text = tmp.read().decode(errors="replace")
+    iters, res = None, None
+    for ln in text.splitlines():
+        if ln.startswith("I iter ") or ln.startswith("# iter "):
+            iters = int(ln.split()[2])
+        elif ln.startswith("# res"):
+            res = ln.strip()
+    return dt, iters, res
+
+
+def relres_of(sy, x):
+    rows = np.repeat(np.arange(sy["N"]), np.diff(sy["rptr"]))
+    r = sy["b"] - np.bincount(rows, weights=sy["val"] * x[sy["cind"]], minlength=sy["N"])
+    return float(np.linalg.norm(r) / np.linalg.norm(sy["b"]))
+
+
+def ref_abmc_gmres(dev, m, eps, budget_s, expect_iters, bcsr=None):
+    """The reference's AbmcIluGmresm<double> (or, with bcsr=(bnl, bnw), AbmcIlubGmresm
+    <double,bnl,bnw> on the BCSR copies of the factors) on the host cores, with the device's
+    factors (equal to the reference's Ilu0 of the same pattern bit for bit).  One restart
+    cycle runs first; the full solve runs when that rate, for 1.5x the expected iterations,
+    fits budget_s seconds."""
+    from oracle import oracle as O
+    ref = O.ref()
+    if ref is None:
+        return {"error": "oracle/_ref not built"}
+    sy, N = dev.sy, dev.N
+    (lv, lc, lr), (uv, uc, ur) = dev.factors()
+    if bcsr:
+        bnl, bnw = bcsr
+        L = S.csr2bcsr(lv, lc, lr, N, bnl, bnw)
+        U = S.csr2bcsr(uv, uc, ur, N, bnl, bnw)
+    else:
+        L, U = (lv, lc, lr), (uv, uc, ur)
+    b = sy["b"]
+    nrm_b = float(np.linalg.norm(b))
+
+    def run(outer, x):
+        if bcsr:
+            return lambda: ref.ref_abmc_ilub_gmresm_d(
+                O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), *(O.P(t) for t in L), *(O.P(t) for t in U),
+                O.P(sy["cptr"]), sy["nc"], sy["bsize"], O.P(b), O.P(x), nrm_b, outer, m, N, eps, bnl, bnw)
+        return lambda: ref.ref_abmc_ilu_gmresm_d(
+            O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), *(O.P(t) for t in L), *(O.P(t) for t in U),
+            O.P(sy["cptr"]), sy["nc"], sy["bsize"], O.P(b), O.P(x), nrm_b, outer, m, N, eps)
+
+    x = np.zeros(N)
+    t_cycle, _, _ = ref_call(ref, run(1, x))
+    out = {"ms_per_iteration_first_cycle": t_cycle * 1e3 / m}
+    x = np.zeros(N)
+    outer = 20000 // m
+    fits = t_cycle / m * 1.5 * expect_iters <= budget_s
+    t, iters, res = ref_call(ref, run(outer, x)) if fits else (None, None, None)
+    if t is None:
+        # the full solve does not fit the budget: one more cycle is the sample
+        out.update({"full": False})
+        return out
+    out.update({"full": True, "t_solve_ms": t * 1e3, "iters": iters, "res_line": res,
+                "ms_per_iteration": t * 1e3 / max(iters or 1, 1), "true_relres": relres_of(sy, x), "x": x})
+    return out
+
+
+def cpu_baseline(dev, gpu_iters, info):
     """The reference's AbmcIluGmresm (oracle/_ref, OpenMP) on the host: the full FP64
     ILU(0)-GMRES(30) solve of the same reordered system with the same factors (the device
-    factors equal the reference's Ilu0 bit for bit), plus 20 reference SpMVs."""
+    factors equal the reference's Ilu0 bit for bit), with the count and '# res' line it
+    prints, plus 20 reference SpMVs."""
     try:
         from oracle import oracle as O
     except Exception as e:  # pragma: no cover
@@ -226,55 +342,30 @@ def cpu_baseline(dev, gpu_iters):
     if ref is None:
         return {"error": "oracle/_ref not built"}
     sy, N = dev.sy, dev.N
-    lr, lc, lv = np.zeros(N + 1, np.int32), np.zeros(dev.lnnz, np.int32), np.zeros(dev.lnnz)
-    ur, uc, uv = np.zeros(N + 1, np.int32), np.zeros(dev.unnz, np.int32), np.zeros(dev.unnz)
-    S.call("senk_prec_download", dev.M, O.P(lr), O.P(lc), O.P(lv), O.P(ur), O.P(uc), O.P(uv))
-    b = sy["b"]
-    nrm_b = float(np.linalg.norm(b))
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    sys.stdout.flush()
-    saved = os.dup(1)
-    devnull = os.open(os.devnull, os.O_WRONLY)
-    os.dup2(devnull, 1)
-    try:
-        # one cycle first: the full solve runs only if it fits the time budget
-        x = np.zeros(N)
-        t0 = time.perf_counter()
-        ref.ref_abmc_ilu_gmresm_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(lv), O.P(lc), O.P(lr),
-                                  O.P(uv), O.P(uc), O.P(ur), O.P(sy["cptr"]), sy["nc"], sy["bsize"], O.P(b), O.P(x),
-                                  nrm_b, 1, 30, N, 1e-10)
-        t_cycle = time.perf_counter() - t0
-        full = t_cycle / 30 * gpu_iters <= 40.0
-        if full:
-            x = np.zeros(N)
-            t0 = time.perf_counter()
-            ref.ref_abmc_ilu_gmresm_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(lv), O.P(lc), O.P(lr),
-                                      O.P(uv), O.P(uc), O.P(ur), O.P(sy["cptr"]), sy["nc"], sy["bsize"], O.P(b),
-                                      O.P(x), nrm_b, 200, 30, N, 1e-10)
-            t_full = time.perf_counter() - t0
-        y = np.zeros(N)
-        xx = np.random.default_rng(0).uniform(-1, 1, N)
+    r = ref_abmc_gmres(dev, 30, 1e-10, 60.0, gpu_iters)
+    y = np.zeros(N)
+    xx = np.random.default_rng(0).uniform(-1, 1, N)
+    ref.ref_spmv_csr_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(xx), O.P(y), N)
+    t0 = time.perf_counter()
+    for _ in range(20):
         ref.ref_spmv_csr_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(xx), O.P(y), N)
-        t0 = time.perf_counter()
-        for _ in range(20):
-            ref.ref_spmv_csr_d(O.P(sy["val"]), O.P(sy["cind"]), O.P(sy["rptr"]), O.P(xx), O.P(y), N)
-        t_spmv = (time.perf_counter() - t0) / 20
-        ref.ref_flush()
-    finally:
-        os.dup2(saved, 1)
-        os.close(saved)
-        os.close(devnull)
-    if full:
-        rows = np.repeat(np.arange(N), np.diff(sy["rptr"]))
-        relres = float(np.linalg.norm(b - np.bincount(rows, weights=sy["val"] * x[sy["cind"]], minlength=N)) / nrm_b)
-        value, sample = t_full * 1e3, (f"the full reference AbmcIluGmresm<double>(m=30) solve to 1e-10 of the same "
-                                       f"128^3 system and ILU(0) factors (true relres {relres:.2e})")
+    t_spmv = (time.perf_counter() - t0) / 20
+    threads = int(os.environ.get("OMP_NUM_THREADS") or info["affinity_cpus"])
+    out = {"unit": "ms", "cores": threads, "kind": "reference", "cpu_model": info["cpu_model"],
+           "physical_cores": info["physical_cores"], "omp": info["omp"],
+           "spmv_GBps": spmv_bytes(N, sy["nnz"], 8) / t_spmv / 1e9}
+    if r.get("full"):
+        out.update({"value": r["t_solve_ms"], "ref_iters": r["iters"], "gpu_iters": gpu_iters,
+                    "ref_res_line": r["res_line"], "ms_per_iteration": r["ms_per_iteration"],
+                    "sample": (f"the full reference AbmcIluGmresm<double>(m=30) solve to 1e-10 of the same "
+                               f"{round(N ** (1 / 3))}^3 system and ILU(0) factors on {threads} OpenMP threads "
+                               f"(true relres {r['true_relres']:.2e})")})
     else:
-        value, sample = t_cycle * 1e3 / 30 * gpu_iters, (f"one restart cycle (30 iterations) of the reference "
-                                                         f"AbmcIluGmresm<double>(m=30) took {t_cycle:.2f} s; value = "
-                                                         f"per-iteration time x the {gpu_iters} GPU iterations")
-    return {"value": value, "unit": "ms", "cores": cores, "kind": "reference", "sample": sample,
-            "ms_per_iteration": t_cycle * 1e3 / 30, "spmv_GBps": spmv_bytes(N, sy["nnz"], 8) / t_spmv / 1e9}
+        msi = r["ms_per_iteration_first_cycle"]
+        out.update({"value": msi * gpu_iters, "ms_per_iteration": msi, "gpu_iters": gpu_iters,
+                    "sample": (f"one restart cycle (30 iterations) of the reference AbmcIluGmresm<double>(m=30); "
+                               f"value = per-iteration time x the {gpu_iters} GPU iterations")})
+    return out
 
 
 def load_learned(ctx):
@@ -361,6 +452,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the unstructured 4,194,304-row workload")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 96^3 block (3-dof elasticity) workload")
     ap.add_argument("--no-ingest", action="store_true", help="skip the MatrixMarket ingestion entry")
     ap.add_argument("--kernels-only", action="store_true", help="kernel rates and the copy calibration only")
     ap.add_argument("--learned-only", action="store_true",
@@ -377,6 +470,10 @@ def main():
     if args.ingest:
         print(json.dumps(ingest_rates(args.n)), flush=True)
         return
+    # the reference's OpenMP threads stay on their cores (read when libgomp starts, i.e.
+    # when the compiled reference is first loaded)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     n = args.n
     ctx = S.Context(0)
     log(f"[bench] building the {n}^3 variable-coefficient diffusion system (ABMC blocks of 8 and 4)")
@@ -439,7 +536,7 @@ def main():
     # plain FP64 (the reference's method) at both block sizes; the baseline is the faster
     for tag, dev in (("b8", d8), ("b4", d4)):
         runs[f"fp64_gmres30_{tag}"] = timed(dev, K, W, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000)
-        log(f"[bench] fp64_gmres30_{tag}: {json.dumps(runs[f'fp64_gmres30_{tag}'])}")
+        log(f"[bench] fp64_gmres30_{tag}: {json.dumps(slim(runs[f'fp64_gmres30_{tag}']))}")
     # the same GMRES(30) with two-pass classical Gram-Schmidt (not the reference's MGS)
     runs["fp64_gmres30_cgs2_b8"] = timed(d8, K, W, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000,
                                          ortho=S.SENK_CGS2)
@@ -447,63 +544,71 @@ def main():
     # conjugate gradients with IC(0) (the reference announces both, has neither)
     dic = Device(ctx, d8.sy, kind=S.SENK_IC0)
     runs["fp64_ic0_cg_b8"] = timed(dic, K, W, solver=S.SENK_CG, eps=1e-10, max_iter=6000)
-    log(f"[bench] fp64_ic0_cg_b8: {json.dumps(runs['fp64_ic0_cg_b8'])}")
+    log(f"[bench] fp64_ic0_cg_b8: {json.dumps(slim(runs['fp64_ic0_cg_b8']))}")
     dic.close()
     for tag, dev in (("b8", d8), ("b4", d4)):
         runs[f"mixed_bicgstab_{tag}"] = timed(dev, K, W, solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)
         runs[f"mixed_fgcr_{tag}"] = timed(dev, K, W, solver=S.SENK_FGCR, inner_max=30, **MIXED)
         runs[f"mixed_cg_{tag}"] = timed(dev, K, W, solver=S.SENK_CG, inner_max=INNER_CG, **MIXED)
-        log(f"[bench] mixed_{tag}: {json.dumps(runs[f'mixed_bicgstab_{tag}'])}")
+        log(f"[bench] mixed_{tag}: {json.dumps(slim(runs[f'mixed_cg_{tag}']))}")
     net, meta = load_learned(ctx)
     if net is not None:
         net.bind(d8.A)
         net.attach(d8.M)
-        runs["learned_fgcr30_fp64_b8"] = timed(d8, 1, 1, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
+        runs["learned_fgcr30_fp64_b8"] = timed(d8, K, W, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
                                                check_every=1, use_learned=1)
-        runs["plain_fgcr30_fp64_b8"] = timed(d8, 1, 1, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
+        runs["plain_fgcr30_fp64_b8"] = timed(d8, K, W, solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
                                              check_every=1)
-        runs["learned_mixed_fgcr_b8"] = timed(d8, 1, 1, solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED)
+        runs["learned_mixed_fgcr_b8"] = timed(d8, K, W, solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED)
+        log(f"[bench] learned_mixed_fgcr_b8: {json.dumps(slim(runs['learned_mixed_fgcr_b8']))}")
         dx32, dz32 = ctx.zeros(d8.N, np.float32), ctx.zeros(d8.N, np.float32)
         kern["learned_apply_f32_us"] = 1e3 * time_kernel(
             ctx, lambda: S.call("senk_gnn_apply_f32", net.h, dx32.p, dz32.p), 10)
         S.call("senk_prec_attach_gnn", d8.M, None)
         net.close()
-    base_key = min(("fp64_gmres30_b8", "fp64_gmres30_b4"), key=lambda k: runs[k]["t_solve_ms_median"])
-    mixed_key = min((k for k in runs if k.startswith("mixed_")), key=lambda k: runs[k]["t_solve_ms_median"])
+
+    def ok(r):  # a solve that counts: converged, FP64 true residual at the target
+        return r["converged"] == 1 and r["true_relres"] <= 1e-10
+
+    base_key = min((k for k in ("fp64_gmres30_b8", "fp64_gmres30_b4") if ok(runs[k])),
+                   key=lambda k: runs[k]["t_solve_ms_median"])
+    mixed_key = min((k for k in runs if k.startswith(("mixed_", "learned_mixed_")) and ok(runs[k])),
+                    key=lambda k: runs[k]["t_solve_ms_median"])
     base, best = runs[base_key], runs[mixed_key]
+    # the FP64 solve of the same method family as the headline solve (CG with IC(0) for the
+    # inner-CG forms, BiCGStab for the inner-BiCGStab forms, FGCR for the FGCR forms)
+    family = {"cg": "fp64_ic0_cg_b8", "bicgstab": "fp64_bicgstab_b8", "fgcr": "plain_fgcr30_fp64_b8"}
+    fam_key = next((v for k, v in family.items() if f"_{k}_" in mixed_key + "_"), None)
+    info = host_info()
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (reference OpenMP AbmcIluGmresm)")
-        cpu = cpu_baseline(d8 if base_key.endswith("b8") else d4, base["iters"])
+        cpu = cpu_baseline(d8 if base_key.endswith("b8") else d4, base["iters"], info)
         log(f"[bench] cpu: {json.dumps(cpu)}")
-    c2 = None
-    if not args.no_c2:
-        log(f"[bench] building the {n}^3 27-point convection-diffusion system (ABMC blocks of 16)")
-        syc = build_system_kind("convdiff", n, 16)
-        dc = Device(ctx, syc)
-        kc = kernel_rates(ctx, dc)
-        c2 = {"kernels": kc,
-              "fp64_gmres30": timed(dc, 2, 1, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000),
-              "fp64_bicgstab": timed(dc, 2, 1, solver=S.SENK_BICGSTAB, eps=1e-10, max_iter=6000),
-              "mixed_bicgstab": timed(dc, 2, 1, solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)}
-        log(f"[bench] c2: {json.dumps(c2)}")
-        dc.close()
-    keep = ("iters", "restarts", "converged", "fell_back", "t_solve_ms_median", "true_relres", "err_vs_xstar")
+    d8.close()
+    d4.close()
+    Ks = min(K, 5)  # the other workloads: median of at most 5 solves each
+    c2 = None if args.no_c2 else config_c2(ctx, n, Ks, args.no_cpu)
+    c3 = None if args.no_c3 else config_c3(ctx, Ks, args.no_cpu)
+    c5 = None if args.no_c5 else config_c5(ctx, Ks, args.no_cpu)
     value = best["t_solve_ms_median"]
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 inner solve + f64 outer iteration", "data": "synthetic",
         "config": {"workload": f"vardiff7 {n}^3 (k = exp(GRF, corr 8, sigma 1.5), seed 1), ABMC connect, ILU(0), "
-                               f"tol 1e-10", "n": d8.N, "nnz": sy8["nnz"], "mixed": mixed_key,
+                               f"tol 1e-10", "n": sy8["N"], "nnz": sy8["nnz"], "mixed": mixed_key,
                    "fp64_baseline": base_key},
         "speedup_vs_fp64_gmres30": round(base["t_solve_ms_median"] / value, 3),
-        "runs": {k: {q: v[q] for q in keep if q in v} for k, v in runs.items()},
+        "speedup_vs_fp64_same_method": None if fam_key not in runs else {
+            "fp64": fam_key, "speedup": round(runs[fam_key]["t_solve_ms_median"] / value, 3)},
+        "learned_vs_fp64_gmres30": None if "learned_mixed_fgcr_b8" not in runs else round(
+            base["t_solve_ms_median"] / runs["learned_mixed_fgcr_b8"]["t_solve_ms_median"], 3),
+        "runs": {k: slim(v) for k, v in runs.items()},
         "kernels": kern, "copy_calibration": cal,
         "learned_training": None if meta is None else {k: meta[k] for k in ("count", "sizes", "epochs", "loss_hist",
                                                                               "t_train_s", "t_data_s") if k in meta},
-        "plain_fp64_c2": None if c2 is None else {
-            "kernels": c2["kernels"], **{k: {q: v[q] for q in keep if q in v} for k, v in c2.items() if k != "kernels"}},
+        "plain_fp64_c2": c2, "mixed_c3": c3, "block_c5": c5,
         "setup_s": {"abmc": sy8["t_abmc"], "reorder": sy8["t_reorder"], "upload": d8.t_upload,
                     "device_ilu0_split_layout": d8.t_ilu},
         # the dominant kernels of the headline solve: the FP32 ILU(0) application (one L sweep
@@ -518,12 +623,128 @@ def main():
                           "frac": round(kern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4),
                           "traffic": traffic_from_profile("spmv_sell_f32_vardiff128.md")},
         "cpu_baseline": cpu,
+        "host": info,
         # host-side MatrixMarket ingestion (64^3 file; --ingest --n 128 for the full size)
         "ingest": None if args.no_ingest else ingest_rates(64),
     }
-    d8.close()
-    d4.close()
     print(json.dumps(line), flush=True)
+
+
+KEEP = ("iters", "restarts", "inner_iters", "converged", "fell_back", "t_solve_ms_median", "true_relres",
+        "err_vs_xstar", "err_vs_fp64_solve")
+
+
+def slim(d):
+    return {q: d[q] for q in KEEP if q in d}
+
+
+def relerr(x, y):
+    return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+
+def within5(gpu, ref):
+    """The plain path's count criterion: within 5% of the reference (at least +-2)."""
+    return None if ref is None else bool(abs(gpu - ref) <= max(2, int(np.ceil(0.05 * ref))))
+
+
+def reference_entry(r, gpu_iters):
+    if "error" in r:
+        return r
+    out = {k: r[k] for k in ("full", "t_solve_ms", "iters", "res_line", "ms_per_iteration",
+                             "ms_per_iteration_first_cycle", "true_relres") if k in r}
+    out["gpu_iters"] = gpu_iters
+    out["iters_within_5pct"] = within5(gpu_iters, r.get("iters"))
+    return out
+
+
+def config_c2(ctx, n, K, no_cpu):
+    """27-point convection-diffusion 128^3 (Pe 10), ABMC blocks of 16: plain FP64
+    ILU(0)-GMRES(30) and ILU(0)-BiCGStab, the mixed form, SpMV and ILU rates, and the
+    reference's AbmcIluGmresm count on the same system and factors."""
+    log(f"[bench] building the {n}^3 27-point convection-diffusion system (ABMC blocks of 16)")
+    sy = build_system_kind("convdiff", n, 16)
+    dc = Device(ctx, sy)
+    out = {"workload": f"convdiff27 {n}^3 (Pe 10), ABMC connect blocks of 16, ILU(0), tol 1e-10",
+           "n": sy["N"], "nnz": sy["nnz"], "kernels": kernel_rates(ctx, dc)}
+    out["fp64_gmres30"] = slim(timed(dc, K, 1, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000))
+    out["fp64_bicgstab"] = slim(timed(dc, K, 1, solver=S.SENK_BICGSTAB, eps=1e-10, max_iter=6000))
+    out["mixed_bicgstab"] = slim(timed(dc, K, 1, solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED))
+    if not no_cpu:
+        gi = out["fp64_gmres30"]["iters"]
+        out["reference_gmres30"] = reference_entry(ref_abmc_gmres(dc, 30, 1e-10, 40.0, gi), gi)
+    log(f"[bench] c2: {json.dumps(out)}")
+    dc.close()
+    return out
+
+
+def config_c3(ctx, K, no_cpu, n=4194304):
+    """Unstructured nonsymmetric matrix, n = 4,194,304: FP32 values and ILU(0) factors,
+    FP32 SpMV and inner GMRES(50), FP64 refinement to 1e-10; error against an FP64 solve
+    of the same system (the device's FP64 ILU(0)-GMRES(50) to 1e-14); the plain FP64
+    GMRES(50) and the reference's AbmcIluGmresm(m = 50) count beside it."""
+    log(f"[bench] building the unstructured system (n = {n}, ABMC blocks of 16)")
+    sy = build_system_kind("unstructured", n, 16)
+    dev = Device(ctx, sy)
+    out = {"workload": f"unstructured n={n} (lognormal rows, banded-random columns, diagonal 1.1x row sum), ABMC "
+                       f"connect blocks of 16 ({sy['nc']} colors), ILU(0), tol 1e-10",
+           "n": sy["N"], "nnz": sy["nnz"], "setup_s": {"generate": sy["t_gen"], "abmc": sy["t_abmc"],
+                                                      "reorder": sy["t_reorder"], "device": dev.t_ilu},
+           "kernels": kernel_rates(ctx, dev)}
+    r64 = dev.solve(keep_x=True, solver=S.SENK_GMRES, m=50, eps=1e-14, max_iter=20000)
+    x64 = r64.pop("x")
+    out["fp64_solve_1e-14"] = slim(r64)
+    out["fp64_gmres50"] = slim(timed(dev, K, 1, solver=S.SENK_GMRES, m=50, eps=1e-10, max_iter=20000))
+    m = timed(dev, K, 1, keep_x=True, solver=S.SENK_GMRES, m=50, eps=1e-10, max_iter=20000, mixed=1,
+              inner_eps=1e-5, inner_max=500)
+    m["err_vs_fp64_solve"] = relerr(m.pop("x"), x64)
+    out["mixed_ir_gmres50"] = slim(m)
+    if not no_cpu:
+        gi = out["fp64_gmres50"]["iters"]
+        out["reference_gmres50"] = reference_entry(ref_abmc_gmres(dev, 50, 1e-10, 40.0, gi), gi)
+    log(f"[bench] c3: {json.dumps(out)}")
+    dev.close()
+    return out
+
+
+def config_c5(ctx, K, no_cpu, n=96):
+    """Block path: 3 dof per node on n^3 (3x3-block 7-point stencil, nu 0.3, lognormal
+    moduli), ABMC blocks of 48: ILUB (3x3 block fill) and ILU(1) over the colors, FP32
+    factors + FP64 refinement + FGMRES(50), with and without the learned correction; error
+    against the FP64 ILUB-GMRES(50) solve to 1e-14; the plain FP64 ILUB-GMRES(50) and the
+    reference's AbmcIlubGmresm<double,3,3> count beside it."""
+    log(f"[bench] building the {n}^3 3-dof block system (ABMC blocks of 48)")
+    sy = build_system_kind("elasticity", n, 48)
+    out = {"workload": f"elasticity3 {n}^3 (3 dof, nu 0.3, lognormal E), ABMC connect blocks of 48 "
+                       f"({sy['nc']} colors), tol 1e-10", "n": sy["N"], "nnz": sy["nnz"]}
+    mixed = dict(solver=S.SENK_FGMRES, m=50, eps=1e-10, max_iter=20000, mixed=1, inner_eps=1e-5, inner_max=500)
+    x64 = None
+    for tag, kw in (("ilub", dict(kind=S.SENK_ILUB, bnl=3, bnw=3)), ("ilu1", dict(kind=S.SENK_ILUP, p=1))):
+        dev = Device(ctx, sy, **kw)
+        out[f"setup_{tag}_s"] = dev.t_ilu
+        if x64 is None:
+            r64 = dev.solve(keep_x=True, solver=S.SENK_GMRES, m=50, eps=1e-14, max_iter=20000)
+            x64 = r64.pop("x")
+            out["fp64_ilub_solve_1e-14"] = slim(r64)
+            out["fp64_ilub_gmres50"] = slim(timed(dev, K, 1, solver=S.SENK_GMRES, m=50, eps=1e-10, max_iter=20000))
+            if not no_cpu:
+                gi = out["fp64_ilub_gmres50"]["iters"]
+                out["reference_ilub_gmres50"] = reference_entry(
+                    ref_abmc_gmres(dev, 50, 1e-10, 40.0, gi, bcsr=(3, 3)), gi)
+        r = timed(dev, K, 1, keep_x=True, **mixed)
+        r["err_vs_fp64_solve"] = relerr(r.pop("x"), x64)
+        out[f"mixed_{tag}_fgmres50"] = slim(r)
+        net, _ = load_learned(ctx)
+        if net is not None:
+            net.bind(dev.A)
+            net.attach(dev.M)
+            r = timed(dev, K, 1, keep_x=True, use_learned=1, **mixed)
+            r["err_vs_fp64_solve"] = relerr(r.pop("x"), x64)
+            out[f"learned_mixed_{tag}_fgmres50"] = slim(r)
+            S.call("senk_prec_attach_gnn", dev.M, None)
+            net.close()
+        dev.close()
+    log(f"[bench] c5: {json.dumps(out)}")
+    return out
 
 
 if __name__ == "__main__":

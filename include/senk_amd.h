@@ -511,7 +511,11 @@ typedef struct {
     double inner_eps;    /* mixed: inner relative tolerance per refinement step          */
     int inner_max;       /* mixed: inner iteration budget per refinement step            */
     int check_every;     /* GCR: 1 = test ||r|| every step (reference: restarts only)    */
-    int use_learned;     /* 1: add the learned correction (flexible solvers only)         */
+    int use_learned;     /* 1: add the learned correction: in every ILU application of a
+                            flexible solver (FGCR, FGMRES, FP64 or as the mixed forms'
+                            inner solver); with mixed != 0 and another inner solver, once
+                            to each inner solve's result (the refinement's correction or
+                            the outer flexible GCR's preconditioner application)         */
     int verbose;         /* print the reference's lines                                  */
     int outer_m;         /* mixed == 2: restart length of the FP64 outer GCR               */
 } senk_solve_opts;
@@ -544,16 +548,31 @@ int senk_gnn_apply_f32(senk_gnn g, const float *dr, float *dz);
  * with M's FP32 factors, and its gradient (host, SENK_GNN_PARAMS doubles, may be NULL). */
 int senk_gnn_loss_grad(senk_gnn g, senk_mat A, senk_prec M, const float *dR, int nvec, double *loss,
                        double *grad);
+/* The same with z0 for row k of dR taken from row k of dZ (device FP32) instead of M's
+ * ILU application: the correction of another preconditioner's result. */
+int senk_gnn_loss_grad_pairs(senk_gnn g, senk_mat A, senk_prec M, const float *dR, const float *dZ, int nvec,
+                             double *loss, double *grad);
 /* Adam on the device over the residual sets of nmat matrices (dR[k]: nvec[k] FP32 rows of
  * A[k]'s N); loss_hist (host, epochs) receives the mean loss of each epoch. */
 int senk_gnn_train(senk_gnn g, int nmat, const senk_mat *A, const senk_prec *M, const float *const *dR,
                    const int *nvec, int epochs, int batch, double lr, unsigned seed, double *loss_hist,
                    int verbose);
+/* The same with z0 rows from dZ[k] (as senk_gnn_loss_grad_pairs; dZ == NULL: the ILU). */
+int senk_gnn_train_pairs(senk_gnn g, int nmat, const senk_mat *A, const senk_prec *M, const float *const *dR,
+                         const float *const *dZ, const int *nvec, int epochs, int batch, double lr,
+                         unsigned seed, double *loss_hist, int verbose);
 /* Residual/error pairs from the plain FP64 ILU-GMRES(m) of the handle API: after each of
  * up to ncycles restart cycles from x0 = 0, row k of dR receives r = b - A x and, when
  * dxstar is given, row k of dE receives x* - x (device FP64).  *count = rows written. */
 int senk_gnn_collect(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, const double *dxstar, int m,
                      int ncycles, double *dR, double *dE, int *count);
+/* Training pairs for the correction of the nested form's inner solve (senk_solve with
+ * opts->mixed = 2 and an inner solver other than FGCR / FGMRES): runs that solve from
+ * x0 = 0 without a correction and, at every outer step, stores the normalised FP32 residual
+ * the inner solve received as row k of dR and its result as row k of dZ (device FP32, cap
+ * rows of N each); *count = rows written. */
+int senk_gnn_collect_nested(senk_ctx ctx, senk_mat A, senk_prec M, const double *db,
+                            const senk_solve_opts *opts, float *dR, float *dZ, int cap, int *count);
 /* Use g (bound to M's matrix) as M's learned correction in solves with use_learned = 1;
  * g == NULL detaches.  A solve whose FP64 residual stagnates switches the correction off
  * for the rest of that solve (senk_result.fell_back). */

@@ -1582,6 +1582,11 @@ int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *
     LearnedPrec *lp = nullptr;
     if (o->use_learned) {
         if (!M || !M->gnn) return SENK_ERR_ARG;
+        // a Krylov method that is not flexible takes the correction only through a mixed form
+        if (!o->mixed && o->solver != SENK_FGCR && o->solver != SENK_FGMRES) {
+            record_error("use_learned: FGCR / FGMRES, or a mixed form");
+            return SENK_ERR_ARG;
+        }
         lp = gnn_learned(M->gnn);
         lp->off = false;
         pv.learned = lp;
@@ -1651,20 +1656,52 @@ static PrecView plain_view(senk_prec M) {
     return v;
 }
 int senk_gnn_loss_grad(senk_gnn g, senk_mat A, senk_prec M, const float *dR, int nvec, double *loss, double *grad) {
-    if (!g || !A || !M || !dR || nvec <= 0 || !M->view.l32 || !A->v32) return SENK_ERR_ARG;
-    return gnn_loss_grad(g, A->view, plain_view(M), dR, nvec, loss, grad);
+    return senk_gnn_loss_grad_pairs(g, A, M, dR, nullptr, nvec, loss, grad);
+}
+int senk_gnn_loss_grad_pairs(senk_gnn g, senk_mat A, senk_prec M, const float *dR, const float *dZ, int nvec,
+                             double *loss, double *grad) {
+    if (!g || !A || !M || !dR || nvec <= 0 || (!dZ && !M->view.l32) || !A->v32) return SENK_ERR_ARG;
+    return gnn_loss_grad(g, A->view, plain_view(M), dR, dZ, nvec, loss, grad);
 }
 int senk_gnn_train(senk_gnn g, int nmat, const senk_mat *A, const senk_prec *M, const float *const *dR,
                    const int *nvec, int epochs, int batch, double lr, unsigned seed, double *loss_hist, int verbose) {
+    return senk_gnn_train_pairs(g, nmat, A, M, dR, nullptr, nvec, epochs, batch, lr, seed, loss_hist, verbose);
+}
+int senk_gnn_train_pairs(senk_gnn g, int nmat, const senk_mat *A, const senk_prec *M, const float *const *dR,
+                         const float *const *dZ, const int *nvec, int epochs, int batch, double lr, unsigned seed,
+                         double *loss_hist, int verbose) {
     if (!g || nmat <= 0 || !A || !M || !dR || !nvec) return SENK_ERR_ARG;
     std::vector<MatView> av((size_t)nmat);
     std::vector<PrecView> pv((size_t)nmat);
     for (int k = 0; k < nmat; k++) {
-        if (!A[k] || !M[k] || !dR[k] || nvec[k] <= 0 || A[k]->N != M[k]->view.N) return SENK_ERR_ARG;
+        if (!A[k] || !M[k] || !dR[k] || nvec[k] <= 0 || A[k]->N != M[k]->view.N || (dZ && !dZ[k]))
+            return SENK_ERR_ARG;
         av[k] = A[k]->view;
         pv[k] = plain_view(M[k]);
     }
-    return gnn_train(g, nmat, av.data(), pv.data(), dR, nvec, epochs, batch, lr, seed, loss_hist, verbose);
+    return gnn_train(g, nmat, av.data(), pv.data(), dR, dZ, nvec, epochs, batch, lr, seed, loss_hist, verbose);
+}
+int senk_gnn_collect_nested(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, const senk_solve_opts *opts,
+                            float *dR, float *dZ, int cap, int *count) {
+    CTX_CHECK(ctx);
+    if (!A || !M || !db || !opts || !dR || !dZ || cap <= 0 || !count || opts->mixed != 2 || !A->v32 ||
+        opts->solver == SENK_FGCR || opts->solver == SENK_FGMRES)
+        return SENK_ERR_ARG;
+    double nrm_b = 0;
+    SENK_TRY(senk_nrm2_f64(ctx, db, A->N, &nrm_b));
+    double *dx = nullptr;
+    SENK_TRY(dalloc(&dx, (size_t)A->N));
+    SENK_HIP(hipMemsetAsync(dx, 0, sizeof(double) * A->N, ctx->c.stream));
+    NestedRecord rec;
+    rec.R = dR;
+    rec.Z = dZ;
+    rec.cap = cap;
+    senk_result res;
+    const int st = nested_solve(&ctx->c, A->view, plain_view(M), db, dx, *opts, nrm_b, &res, nullptr, 0, &rec);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    (void)hipFree(dx);
+    *count = rec.count;
+    return st;
 }
 int senk_gnn_collect(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, const double *dxstar, int m,
                      int ncycles, double *dR, double *dE, int *count) {

@@ -845,12 +845,14 @@ int gnn_backward(Ctx *c, senk_gnn_s *net, const GnnGraph &g, GnnWork &w, const f
     return SENK_OK;
 }
 
-// One residual of a batch of nb: z0 = M^-1 r (FP32 factors), z = z0 + correction,
-// loss term ||A z - r||^2/||r||^2 added to sc[4], gradient of loss/nb added to grad.
+// One residual of a batch of nb: z0 = M^-1 r (FP32 factors) or the given z0 (the result
+// of another preconditioner of r), z = z0 + correction, loss term ||A z - r||^2/||r||^2
+// added to sc[4], gradient of loss/nb added to grad.
 int gnn_step(Ctx *c, senk_gnn_s *net, const GnnGraph &g, GnnWork &w, const MatView &A, const PrecView &M,
-             const float *r, int nb) {
+             const float *r, const float *z0, int nb) {
     const int N = g.N;
-    SENK_TRY(prec_apply<float>(c, M, r, nullptr, w.z, w.tmp, nullptr));
+    if (z0) SENK_TRY(launch_copy<float>(c, N, z0, w.z));
+    else SENK_TRY(prec_apply<float>(c, M, r, nullptr, w.z, w.tmp, nullptr));
     SENK_TRY(gnn_forward<float>(c, net, g, w, r, nullptr, w.z, true));
     SENK_TRY(launch_spmv_csr<float>(c, N, g.rptr, g.cind, g.val, w.z, w.t, A.avg_row));
     const int B = red_grid(N);
@@ -937,8 +939,8 @@ int gnn_apply_f32(senk_gnn_s *net, const float *r, float *z) {
 }
 
 // Mean loss over nvec residuals (device, FP32, rows of N) and its gradient (host, FP64).
-int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const float *R, int nvec, double *loss,
-                  double *grad_host) {
+int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const float *R, const float *Z, int nvec,
+                  double *loss, double *grad_host) {
     Ctx *c = net->c;
     if (!A.v32 || nvec <= 0) return SENK_ERR_ARG;
     GnnGraph g;
@@ -948,7 +950,7 @@ int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const fl
         k_zero_d<<<(gnn::kParams + 255) / 256, 256, 0, c->stream>>>(gnn::kParams, net->grad);
         k_zero_d<<<1, 64, 0, c->stream>>>(8, net->w.sc);
         for (int b = 0; b < nvec && st == SENK_OK; b++)
-            st = gnn_step(c, net, g, net->w, A, M, R + (size_t)b * A.N, nvec);
+            st = gnn_step(c, net, g, net->w, A, M, R + (size_t)b * A.N, Z ? Z + (size_t)b * A.N : nullptr, nvec);
     }
     double sc[8] = {0};
     if (st == SENK_OK && (hipMemcpyAsync(sc, net->w.sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
@@ -964,13 +966,14 @@ int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const fl
 // Adam over the residual sets of several matrices: every step takes `batch` residuals of
 // one matrix; matrices and residual order are shuffled per epoch from `seed`.
 int gnn_train(senk_gnn_s *net, int nmat, const MatView *As, const PrecView *Ms, const float *const *Rs,
-              const int *nvecs, int epochs, int batch, double lr, unsigned seed, double *loss_hist, int verbose) {
+              const float *const *Zs, const int *nvecs, int epochs, int batch, double lr, unsigned seed,
+              double *loss_hist, int verbose) {
     Ctx *c = net->c;
     if (nmat <= 0 || epochs <= 0 || batch <= 0) return SENK_ERR_ARG;
     std::vector<GnnGraph> gs((size_t)nmat);
     int st = SENK_OK, maxN = 0;
     for (int k = 0; k < nmat && st == SENK_OK; k++) {
-        if (!As[k].v32 || !Ms[k].l32) st = SENK_ERR_ARG;
+        if (!As[k].v32 || (!Ms[k].l32 && !(Zs && Zs[k]))) st = SENK_ERR_ARG;
         else st = gnn_graph_build(c, As[k].N, As[k].rptr, As[k].cind, As[k].v32, true, false, &gs[k]);
         maxN = std::max(maxN, As[k].N);
     }
@@ -988,8 +991,11 @@ int gnn_train(senk_gnn_s *net, int nmat, const MatView *As, const PrecView *Ms, 
         for (const auto &sp : steps) {
             const int k = sp.first, nb = std::min(batch, nvecs[k] - sp.second);
             k_zero_d<<<(gnn::kParams + 255) / 256, 256, 0, c->stream>>>(gnn::kParams, net->grad);
-            for (int b = 0; b < nb && st == SENK_OK; b++)
-                st = gnn_step(c, net, gs[k], net->w, As[k], Ms[k], Rs[k] + (size_t)(sp.second + b) * As[k].N, nb);
+            for (int b = 0; b < nb && st == SENK_OK; b++) {
+                const size_t off = (size_t)(sp.second + b) * As[k].N;
+                st = gnn_step(c, net, gs[k], net->w, As[k], Ms[k], Rs[k] + off,
+                              Zs && Zs[k] ? Zs[k] + off : nullptr, nb);
+            }
             nterms += nb;
             if (st) break;
             net->adam_t++;

@@ -1925,6 +1925,18 @@ SolveParams inner_params(const senk_solve_opts &o) {
     return ip;
 }
 
+// The learned correction of a mixed form: with a flexible inner solver (FGCR, FGMRES) it
+// is added to every ILU application of the inner solve; with any other inner solver it is
+// added once to the inner solve's result d (the refinement's correction, or the outer
+// flexible GCR's preconditioner application), from the inner right-hand side r and d.
+// Returns the preconditioner the inner solve uses.
+PrecView inner_prec(const PrecView &M, const senk_solve_opts &o, bool *learned_after) {
+    PrecView Mi = M;
+    *learned_after = M.learned && o.solver != SENK_FGCR && o.solver != SENK_FGMRES;
+    if (*learned_after) Mi.learned = nullptr;
+    return Mi;
+}
+
 int read_scalar(Ctx *c, const double *d, double *h) {
     SENK_HIP(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SENK_HIP(hipStreamSynchronize(c->stream));
@@ -1951,6 +1963,8 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
     SolveTimer tm;
     SENK_TRY(tm.start(c, true));
     const SolveParams ip = inner_params(o);
+    bool learned_after = false;
+    const PrecView Mi = inner_prec(M, o, &learned_after);
     const int max_steps = std::max(1, (o.max_iter + o.inner_max - 1) / std::max(1, o.inner_max)) + 8;
     int steps = 0, inner_total = 0, converged = 0, status = SENK_OK;
     double relres = 0;
@@ -1981,8 +1995,9 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
         SENK_TRY(launch_convert<double, float>(c, N, r64, r32, dsc + 1));
         SENK_HIP(hipMemsetAsync(d32, 0, sizeof(float) * N, c->stream));
         senk_result ir;
-        SENK_TRY(krylov_solve<float>(c, A, M, r32, d32, ip, &ir, nullptr, 0));
+        SENK_TRY(krylov_solve<float>(c, A, Mi, r32, d32, ip, &ir, nullptr, 0));
         inner_total += ir.iters;
+        if (learned_after && !M.learned->off) SENK_TRY(learned_apply_f32(c, M, r32, nullptr, d32, nullptr));
         k_ir_update<<<ew_grid(N), kBlock, 0, c->stream>>>(N, d32, dsc, x);
         SENK_LAUNCH_CHECK();
         steps++;
@@ -2010,7 +2025,8 @@ int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, doubl
 // discarded between inner solves.  The FP64 residual is recomputed at every restart
 // and before convergence is reported.
 int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x,
-                 const senk_solve_opts &o, double nrm_b, senk_result *res, double *hist, int hist_cap) {
+                 const senk_solve_opts &o, double nrm_b, senk_result *res, double *hist, int hist_cap,
+                 NestedRecord *rec) {
     std::memset(res, 0, sizeof(*res));
     const int N = A.N;
     const int mo = o.outer_m > 0 ? o.outer_m : 20;
@@ -2028,6 +2044,8 @@ int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, d
     SolveTimer tm;
     SENK_TRY(tm.start(c, true));
     const SolveParams ip = inner_params(o);
+    bool learned_after = false;
+    const PrecView Mi = inner_prec(M, o, &learned_after);
     const double tol = o.eps * nrm_b;
     int inner_total = 0, outer_its = 0, cycles = 0, converged = 0, status = SENK_OK, hn = 0;
     double rho = 0;
@@ -2055,8 +2073,14 @@ int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, d
             SENK_TRY(launch_convert<double, float>(c, N, r, r32, sc + 1));
             SENK_HIP(hipMemsetAsync(d32, 0, sizeof(float) * N, c->stream));
             senk_result ir;
-            SENK_TRY(krylov_solve<float>(c, A, M, r32, d32, ip, &ir, nullptr, 0));
+            SENK_TRY(krylov_solve<float>(c, A, Mi, r32, d32, ip, &ir, nullptr, 0));
             inner_total += ir.iters;
+            if (rec && rec->count < rec->cap) {
+                SENK_TRY(launch_copy<float>(c, N, r32, rec->R + (size_t)rec->count * N));
+                SENK_TRY(launch_copy<float>(c, N, d32, rec->Z + (size_t)rec->count * N));
+                rec->count++;
+            }
+            if (learned_after && !M.learned->off) SENK_TRY(learned_apply_f32(c, M, r32, nullptr, d32, nullptr));
             SENK_TRY(launch_convert<float, double>(c, N, d32, z, nullptr));
             SENK_TRY(spmv<double>(c, A, z, q));
             for (int pass = 0; pass < 2 && j > 0; pass++) {

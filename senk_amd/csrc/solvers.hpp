@@ -87,9 +87,16 @@ int krylov_solve(Ctx *c, const MatView &A, const PrecView &M, const T *b, T *x, 
 int ir_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x, const senk_solve_opts &o,
              double nrm_b, senk_result *res, double *hist, int hist_cap);
 
+// Rows recorded at every outer step of the nested solve: the normalised FP32 residual the
+// inner solve received and the inner solve's result (before any learned correction).
+struct NestedRecord {
+    float *R = nullptr, *Z = nullptr;  // device, cap rows of N
+    int cap = 0, count = 0;
+};
 // FP64 flexible GCR(o.outer_m) whose preconditioner is an FP32 inner Krylov solve.
 int nested_solve(Ctx *c, const MatView &A, const PrecView &M, const double *b, double *x,
-                 const senk_solve_opts &o, double nrm_b, senk_result *res, double *hist, int hist_cap);
+                 const senk_solve_opts &o, double nrm_b, senk_result *res, double *hist, int hist_cap,
+                 NestedRecord *rec = nullptr);
 
 // Learned correction (gnn.hip).
 int gnn_create(Ctx *c, senk_gnn_s **out);
@@ -99,10 +106,12 @@ int gnn_get_params(senk_gnn_s *net, float *host_theta);
 int gnn_bind(senk_gnn_s *net, int N, const int *drptr, const int *dcind, const float *dval);
 LearnedPrec *gnn_learned(senk_gnn_s *net);
 int gnn_apply_f32(senk_gnn_s *net, const float *r, float *z);
-int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const float *R, int nvec, double *loss,
-                  double *grad_host);
+// Z (optional): row k holds z0 for row k of R (else z0 = M's FP32 ILU application).
+int gnn_loss_grad(senk_gnn_s *net, const MatView &A, const PrecView &M, const float *R, const float *Z, int nvec,
+                  double *loss, double *grad_host);
 int gnn_train(senk_gnn_s *net, int nmat, const MatView *As, const PrecView *Ms, const float *const *Rs,
-              const int *nvecs, int epochs, int batch, double lr, unsigned seed, double *loss_hist, int verbose);
+              const float *const *Zs, const int *nvecs, int epochs, int batch, double lr, unsigned seed,
+              double *loss_hist, int verbose);
 
 // Restart residuals (and errors against xstar) of the plain FP64 GMRES(m) with M.
 int collect_pairs(Ctx *c, const MatView &A, const PrecView &M, const double *b, const double *xstar, int m,
