@@ -368,12 +368,15 @@ def cpu_baseline(dev, gpu_iters, info):
     return out
 
 
-def load_learned(ctx):
+def load_learned(ctx, nested=False):
+    """The trained correction: of the ILU application (flexible solvers), or with
+    nested=True of the nested solve's inner result."""
     try:
         from senk_amd import gnn as G
-        if not G.DEFAULT_WEIGHTS.exists():
+        path = G.NESTED_WEIGHTS if nested else G.DEFAULT_WEIGHTS
+        if not path.exists():
             return None, None
-        theta, meta = G.load()
+        theta, meta = G.load(path)
         return G.Gnn(ctx, theta), meta
     except Exception as e:  # pragma: no cover
         log(f"[bench] learned correction unavailable: {e}")
@@ -566,6 +569,16 @@ def main():
             ctx, lambda: S.call("senk_gnn_apply_f32", net.h, dx32.p, dz32.p), 10)
         S.call("senk_prec_attach_gnn", d8.M, None)
         net.close()
+    # the correction of the nested solve's inner result (FP64 flexible GCR around the FP32
+    # split inner CG), trained on that solve's own (residual, inner result) pairs
+    netn, meta_n = load_learned(ctx, nested=True)
+    if netn is not None:
+        netn.bind(d8.A)
+        netn.attach(d8.M)
+        runs["learned_mixed_cg_b8"] = timed(d8, K, W, solver=S.SENK_CG, inner_max=INNER_CG, use_learned=1, **MIXED)
+        log(f"[bench] learned_mixed_cg_b8: {json.dumps(slim(runs['learned_mixed_cg_b8']))}")
+        S.call("senk_prec_attach_gnn", d8.M, None)
+        netn.close()
 
     def ok(r):  # a solve that counts: converged, FP64 true residual at the target
         return r["converged"] == 1 and r["true_relres"] <= 1e-10
@@ -602,12 +615,13 @@ def main():
         "speedup_vs_fp64_gmres30": round(base["t_solve_ms_median"] / value, 3),
         "speedup_vs_fp64_same_method": None if fam_key not in runs else {
             "fp64": fam_key, "speedup": round(runs[fam_key]["t_solve_ms_median"] / value, 3)},
-        "learned_vs_fp64_gmres30": None if "learned_mixed_fgcr_b8" not in runs else round(
-            base["t_solve_ms_median"] / runs["learned_mixed_fgcr_b8"]["t_solve_ms_median"], 3),
+        "learned_vs_fp64_gmres30": {k: round(base["t_solve_ms_median"] / runs[k]["t_solve_ms_median"], 3)
+                                    for k in ("learned_mixed_fgcr_b8", "learned_mixed_cg_b8") if k in runs},
         "runs": {k: slim(v) for k, v in runs.items()},
         "kernels": kern, "copy_calibration": cal,
-        "learned_training": None if meta is None else {k: meta[k] for k in ("count", "sizes", "epochs", "loss_hist",
-                                                                              "t_train_s", "t_data_s") if k in meta},
+        "learned_training": {name: {k: m_[k] for k in ("mode", "count", "sizes", "epochs", "loss_hist", "t_train_s",
+                                                       "t_data_s", "n_residuals") if k in m_}
+                             for name, m_ in (("ilu", meta), ("nested", meta_n)) if m_ is not None},
         "plain_fp64_c2": c2, "mixed_c3": c3, "block_c5": c5,
         "setup_s": {"abmc": sy8["t_abmc"], "reorder": sy8["t_reorder"], "upload": d8.t_upload,
                     "device_ilu0_split_layout": d8.t_ilu},
