@@ -231,6 +231,14 @@ int senk_convert_f32_f64(senk_ctx ctx, const float *din, double *dout, int n);
  * "# test %e". */
 int senk_relative_residual_error(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
                                  const double *db, const double *dx, int N, int print, double *relres);
+/* No reference counterpart: the same ||b - A x|| / ||b|| with no allocation at all, for
+ * callers that cannot allocate (code under stream capture, tight serving loops).  The
+ * result goes to device memory (*drelres) without synchronising; the partial sums live in
+ * a 16 KiB array of the library's own on each device (looked up by senk_ctx_create), so
+ * any N > 0 runs (at most 1024 workgroups, grid-stride over the rows), and calls that
+ * overlap on one device (two streams or host threads) race on that array. */
+int senk_relative_residual_error_noalloc(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                                         const double *db, const double *dx, int N, double *drelres);
 
 /* ------------------------------------------------------------------ matrix setup
  * senk_matrix.hpp.  Host arrays; integer outputs are bit-identical to the reference. */

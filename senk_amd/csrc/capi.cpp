@@ -295,6 +295,9 @@ int senk_ctx_create(int device, senk_ctx *out) {
     SENK_HIP(hipHostMalloc((void **)&c.hpinned_i, sizeof(int) * 64, hipHostMallocDefault));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c.num_cus = prop.multiProcessorCount;
+    // the no-allocation entry point's scratch array on this device (once per device); a
+    // failure here leaves every other entry point working and makes that one report it
+    (void)noalloc_init(device);
     SENK_HIP(hipDeviceSynchronize());
     *out = ctx;
     return SENK_OK;
@@ -843,6 +846,12 @@ int senk_relative_residual_error(senk_ctx ctx, const double *dval, const int *dc
         std::fflush(stdout);
     }
     return SENK_OK;
+}
+int senk_relative_residual_error_noalloc(senk_ctx ctx, const double *dval, const int *dcind, const int *drptr,
+                                         const double *db, const double *dx, int N, double *drelres) {
+    CTX_CHECK(ctx);
+    if (N <= 0 || !dval || !dcind || !drptr || !db || !dx || !drelres) return SENK_ERR_ARG;
+    return relres_noalloc(&ctx->c, dval, dcind, drptr, db, dx, N, drelres);
 }
 static int finish(senk_ctx ctx, const MatView &A, const double *db, const double *dx, double nrm_b, senk_result *res) {
     double tr = 0;

@@ -120,4 +120,10 @@ int collect_pairs(Ctx *c, const MatView &A, const PrecView &M, const double *b, 
 // ||b - A x|| / nrm_b in FP64 (synchronises).
 int true_relres(Ctx *c, const MatView &A, const double *b, const double *x, double nrm_b, double *out);
 
+// noalloc.hip: the address of the file-scope scratch array on `device` (current on entry),
+// looked up once per device; ||b - A x|| / ||b|| into device memory with that scratch.
+int noalloc_init(int device);
+int relres_noalloc(Ctx *c, const double *dval, const int *dcind, const int *drptr, const double *db,
+                   const double *dx, int N, double *drelres);
+
 }  // namespace senk_amd
