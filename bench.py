@@ -71,7 +71,7 @@ def ilu_bytes(n, lnnz, unnz, sv, si=4):
     return (lnnz + unnz) * (sv + si) + 2 * (n + 1) * si + 4 * n * sv
 
 
-def build_system_kind(kind, n, bsize, seed=1):
+def build_system_kind(kind, n, bsize, seed=1, method="connect"):
     """kind "convdiff": 27-point convection-diffusion on n^3, Pe 10; "vardiff": 7-point
     -div(k grad u) on n^3 with k = exp(Gaussian random field, correlation length 8, sigma
     1.5); "unstructured": n rows, lognormal row lengths, banded-random columns, diagonal 1.1x
@@ -91,14 +91,14 @@ def build_system_kind(kind, n, bsize, seed=1):
     N = len(rptr) - 1
     xs = S.gen_uniform(1, N)
     t1 = time.perf_counter()
-    nc, cptr, LP, RP = S.abmc_permutation(cind, rptr, N, bsize, False, "connect")
+    nc, cptr, LP, RP = S.abmc_permutation(cind, rptr, N, bsize, False, method)
     t2 = time.perf_counter()
     val2, cind2, rptr2 = S.reordering(val, cind, rptr, LP, RP, N)
     t3 = time.perf_counter()
     rows = np.repeat(np.arange(N), np.diff(rptr))
     b = np.bincount(rows, weights=val * xs[cind], minlength=N)
     return dict(N=N, nnz=int(rptr[-1]), rptr=rptr2, cind=cind2, val=val2, b=b[LP].copy(), xs=xs[LP].copy(), nc=nc,
-                cptr=cptr, bsize=bsize, t_gen=t1 - t0, t_abmc=t2 - t1, t_reorder=t3 - t2)
+                cptr=cptr, bsize=bsize, method=method, t_gen=t1 - t0, t_abmc=t2 - t1, t_reorder=t3 - t2)
 
 
 class Device:
@@ -453,6 +453,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--ordering", choices=("connect", "simple"), default="connect",
+                    help="--kernels-only / --learned-only: ABMC blocking of the blocks-of-8 system")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the unstructured 4,194,304-row workload")
@@ -480,7 +482,8 @@ def main():
     n = args.n
     ctx = S.Context(0)
     log(f"[bench] building the {n}^3 variable-coefficient diffusion system (ABMC blocks of 8 and 4)")
-    sy8 = build_system_kind("vardiff", n, 8)
+    sy8 = build_system_kind("vardiff", n, 8, method=args.ordering if args.kernels_only or args.learned_only
+                            else "connect")
     d8 = Device(ctx, sy8)
     if args.learned_only:
         # one layer pass per application: its kernels at this size for a trace or counters
@@ -505,6 +508,10 @@ def main():
         return
     sy4 = build_system_kind("vardiff", n, 4)
     d4 = Device(ctx, sy4)
+    # "simple" ABMC blocking (blocks of 8 consecutive rows): the block graph of a 7-point
+    # grid is 2-colorable, so each triangular solve has 2 levels instead of 10
+    sys8 = build_system_kind("vardiff", n, 8, method="simple")
+    ds8 = Device(ctx, sys8)
     K, W = args.steps, args.warmup
     if args.run:
         # one configuration alone (for a kernel trace of exactly that solve)
@@ -512,6 +519,8 @@ def main():
                 "mixed_bicgstab_b4": (d4, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)),
                 "mixed_bicgstab_b8": (d8, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)),
                 "mixed_cg_b8": (d8, dict(solver=S.SENK_CG, inner_max=INNER_CG, **MIXED)),
+                "mixed_cg_s8": (ds8, dict(solver=S.SENK_CG, inner_max=INNER_CG, **MIXED)),
+                "mixed_bicgstab_s8": (ds8, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)),
                 "learned_fgcr30_fp64_b8": (d8, dict(solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
                                                     check_every=1, use_learned=1)),
                 "learned_mixed_fgcr_b8": (d8, dict(solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED))}
@@ -534,6 +543,7 @@ def main():
         print(json.dumps({"run": args.run, **r}), flush=True)
         d8.close()
         d4.close()
+        ds8.close()
         return
     runs = {}
     # plain FP64 (the reference's method) at both block sizes; the baseline is the faster
@@ -545,11 +555,14 @@ def main():
                                          ortho=S.SENK_CGS2)
     runs["fp64_bicgstab_b8"] = timed(d8, K, W, solver=S.SENK_BICGSTAB, m=30, eps=1e-10, max_iter=6000)
     # conjugate gradients with IC(0) (the reference announces both, has neither)
-    dic = Device(ctx, d8.sy, kind=S.SENK_IC0)
-    runs["fp64_ic0_cg_b8"] = timed(dic, K, W, solver=S.SENK_CG, eps=1e-10, max_iter=6000)
-    log(f"[bench] fp64_ic0_cg_b8: {json.dumps(slim(runs['fp64_ic0_cg_b8']))}")
-    dic.close()
-    for tag, dev in (("b8", d8), ("b4", d4)):
+    for tag, sy_ in (("b8", d8.sy), ("s8", ds8.sy)):
+        dic = Device(ctx, sy_, kind=S.SENK_IC0)
+        runs[f"fp64_ic0_cg_{tag}"] = timed(dic, K, W, solver=S.SENK_CG, eps=1e-10, max_iter=6000)
+        log(f"[bench] fp64_ic0_cg_{tag}: {json.dumps(slim(runs[f'fp64_ic0_cg_{tag}']))}")
+        dic.close()
+    runs["fp64_bicgstab_s8"] = timed(ds8, K, W, solver=S.SENK_BICGSTAB, m=30, eps=1e-10, max_iter=6000)
+    runs["fp64_gmres30_s8"] = timed(ds8, K, W, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000)
+    for tag, dev in (("b8", d8), ("b4", d4), ("s8", ds8)):
         runs[f"mixed_bicgstab_{tag}"] = timed(dev, K, W, solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)
         runs[f"mixed_fgcr_{tag}"] = timed(dev, K, W, solver=S.SENK_FGCR, inner_max=30, **MIXED)
         runs[f"mixed_cg_{tag}"] = timed(dev, K, W, solver=S.SENK_CG, inner_max=INNER_CG, **MIXED)
@@ -583,23 +596,35 @@ def main():
     def ok(r):  # a solve that counts: converged, FP64 true residual at the target
         return r["converged"] == 1 and r["true_relres"] <= 1e-10
 
-    base_key = min((k for k in ("fp64_gmres30_b8", "fp64_gmres30_b4") if ok(runs[k])),
+    # the FP64 baseline: the fastest of the reference's method over the orderings tried
+    base_key = min((k for k in ("fp64_gmres30_b8", "fp64_gmres30_b4", "fp64_gmres30_s8") if ok(runs[k])),
                    key=lambda k: runs[k]["t_solve_ms_median"])
     mixed_key = min((k for k in runs if k.startswith(("mixed_", "learned_mixed_")) and ok(runs[k])),
                     key=lambda k: runs[k]["t_solve_ms_median"])
     base, best = runs[base_key], runs[mixed_key]
     # the FP64 solve of the same method family as the headline solve (CG with IC(0) for the
     # inner-CG forms, BiCGStab for the inner-BiCGStab forms, FGCR for the FGCR forms)
-    family = {"cg": "fp64_ic0_cg_b8", "bicgstab": "fp64_bicgstab_b8", "fgcr": "plain_fgcr30_fp64_b8"}
-    fam_key = next((v for k, v in family.items() if f"_{k}_" in mixed_key + "_"), None)
+    family = {"cg": ("fp64_ic0_cg_b8", "fp64_ic0_cg_s8"), "bicgstab": ("fp64_bicgstab_b8", "fp64_bicgstab_s8"),
+              "fgcr": ("plain_fgcr30_fp64_b8",)}
+    fam = next((v for k, v in family.items() if f"_{k}_" in mixed_key + "_"), ())
+    fam_key = min((k for k in fam if k in runs and ok(runs[k])), key=lambda k: runs[k]["t_solve_ms_median"],
+                  default=None)
     info = host_info()
     cpu = None
+    devs = {"b8": d8, "b4": d4, "s8": ds8}
     if not args.no_cpu:
         log("[bench] CPU baseline (reference OpenMP AbmcIluGmresm)")
-        cpu = cpu_baseline(d8 if base_key.endswith("b8") else d4, base["iters"], info)
+        cpu = cpu_baseline(devs[base_key[-2:]], base["iters"], info)
         log(f"[bench] cpu: {json.dumps(cpu)}")
+    # the roofline entries describe the kernels of the headline solve's system
+    head = devs[mixed_key[-2:]]
+    hkern = kern if head is d8 else kernel_rates(ctx, head)
+    hname = {"b8": "ABMC connect, blocks of 8", "b4": "ABMC connect, blocks of 4",
+             "s8": "ABMC simple, blocks of 8"}[mixed_key[-2:]]
+    hprof = {"b8": "vardiff128", "b4": "vardiff128_b4", "s8": "vardiff128_simple8"}[mixed_key[-2:]]
     d8.close()
     d4.close()
+    ds8.close()
     Ks = min(K, 5)  # the other workloads: median of at most 5 solves each
     c2 = None if args.no_c2 else config_c2(ctx, n, Ks, args.no_cpu)
     c3 = None if args.no_c3 else config_c3(ctx, Ks, args.no_cpu)
@@ -609,8 +634,8 @@ def main():
         "metric": METRIC, "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 inner solve + f64 outer iteration", "data": "synthetic",
-        "config": {"workload": f"vardiff7 {n}^3 (k = exp(GRF, corr 8, sigma 1.5), seed 1), ABMC connect, ILU(0), "
-                               f"tol 1e-10", "n": sy8["N"], "nnz": sy8["nnz"], "mixed": mixed_key,
+        "config": {"workload": f"vardiff7 {n}^3 (k = exp(GRF, corr 8, sigma 1.5), seed 1), ILU(0), tol 1e-10",
+                   "n": sy8["N"], "nnz": sy8["nnz"], "mixed": mixed_key, "mixed_ordering": hname,
                    "fp64_baseline": base_key},
         "speedup_vs_fp64_gmres30": round(base["t_solve_ms_median"] / value, 3),
         "speedup_vs_fp64_same_method": None if fam_key not in runs else {
@@ -618,7 +643,7 @@ def main():
         "learned_vs_fp64_gmres30": {k: round(base["t_solve_ms_median"] / runs[k]["t_solve_ms_median"], 3)
                                     for k in ("learned_mixed_fgcr_b8", "learned_mixed_cg_b8") if k in runs},
         "runs": {k: slim(v) for k, v in runs.items()},
-        "kernels": kern, "copy_calibration": cal,
+        "kernels": kern, "kernels_headline_system": hkern, "copy_calibration": cal,
         "learned_training": {name: {k: m_[k] for k in ("mode", "count", "sizes", "epochs", "loss_hist", "t_train_s",
                                                        "t_data_s", "n_residuals") if k in m_}
                              for name, m_ in (("ilu", meta), ("nested", meta_n)) if m_ is not None},
@@ -628,14 +653,14 @@ def main():
         # the dominant kernels of the headline solve: the FP32 ILU(0) application (one L sweep
         # and one U sweep; with the split inner form they are most of an inner step), and the
         # FP32 SELL SpMV beside it
-        "roofline": {"kernel": "ilu0_apply_f32: L + U sweeps, per-XCD ticket streams (vardiff7 128^3, ABMC blocks of 8)",
-                     "bound": "hbm", "achieved": round(kern["ilu_apply_f32_GBps"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(kern["ilu_apply_f32_GBps"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile("ilu_apply_f32_vardiff128.md")},
-        "roofline_spmv": {"kernel": "spmv_sellcs_f32 (vardiff7 128^3)", "bound": "hbm",
-                          "achieved": round(kern["spmv_sell_f32_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(kern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4),
-                          "traffic": traffic_from_profile("spmv_sell_f32_vardiff128.md")},
+        "roofline": {"kernel": f"ilu0_apply_f32: L + U sweeps, per-XCD ticket streams (vardiff7 {n}^3, {hname})",
+                     "bound": "hbm", "achieved": round(hkern["ilu_apply_f32_GBps"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(hkern["ilu_apply_f32_GBps"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile(f"ilu_apply_f32_{hprof}.md")},
+        "roofline_spmv": {"kernel": f"spmv_sellcs_f32 (vardiff7 {n}^3, {hname})", "bound": "hbm",
+                          "achieved": round(hkern["spmv_sell_f32_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(hkern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4),
+                          "traffic": traffic_from_profile(f"spmv_sell_f32_{hprof}.md")},
         "cpu_baseline": cpu,
         "host": info,
         # host-side MatrixMarket ingestion (64^3 file; --ingest --n 128 for the full size)

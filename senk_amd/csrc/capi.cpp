@@ -28,7 +28,7 @@ struct senk_mat_s {
     float *v32 = nullptr;
     bool has_sell = false;
     senk_sellcs sell{};
-    int *sell_order = nullptr;  // device: slice order of the SpMV (null: slice order)
+    int *sell_order[2] = {nullptr, nullptr};  // device: slice orders by 8 and by 64 regions of every color
     senk_amd::MatView view;
 };
 struct senk_prec_s {
@@ -1257,7 +1257,8 @@ int senk_mat_destroy(senk_mat A) {
     (void)hipFree(A->v64);
     (void)hipFree(A->v32);
     if (A->has_sell) senk_sellcs_free_device(A->ctx, &A->sell);
-    (void)hipFree(A->sell_order);
+    (void)hipFree(A->sell_order[0]);
+    (void)hipFree(A->sell_order[1]);
     delete A;
     return SENK_OK;
 }
@@ -1280,6 +1281,46 @@ int senk_mat_nnz(senk_mat A, int *nnz) {
 // color comes in the j-th eighth of the grid, which the workgroup numbering gives to
 // XCD j: an XCD then works on one region of the domain and finds the gathered entries
 // of x in its own L2.  Each row is still computed alone, so results do not change.
+}  // extern "C"
+// Average time (ms) of `reps` SpMVs of A in one configuration (HIP events on the stream).
+template <typename T>
+static int time_sell(senk_mat A, const T *x, T *y, const int *order, bool nt, int reps, float *ms) {
+    Ctx *c = &A->ctx->c;
+    const senk_sellcs &d = A->sell;
+    const T *v = std::is_same<T, double>::value ? (const T *)d.val64 : (const T *)d.val32;
+    const double w = (double)d.nnz_padded / ((double)d.nslices * d.C);
+    auto run = [&]() {
+        return launch_spmv_sell<T>(c, d.N, d.C, d.nslices, d.slice_ptr, d.cind, v, d.perm, x, y, false, nullptr, w,
+                                   order, nt);
+    };
+    SENK_TRY(run());  // warm-up
+    hipEvent_t e0, e1;
+    SENK_HIP(hipEventCreate(&e0));
+    SENK_HIP(hipEventCreate(&e1));
+    int st = SENK_OK;
+    if (hipEventRecord(e0, c->stream) != hipSuccess) st = SENK_ERR_HIP;
+    for (int k = 0; k < reps && !st; k++) st = run();
+    if (!st && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+        st = SENK_ERR_HIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+// The order in which the SELL SpMV hands out its slices once a colored preconditioner is
+// built on A.  With an ABMC ordering every color spans the whole domain, so a contiguous
+// part of the slice sequence gathers x from everywhere; here the slices are grouped by
+// region instead: the k-th of R equal parts of every color (by row position inside the
+// color) make region k, regions in order, colors in order inside a region.  The
+// dispatcher hands workgroup b to XCD b mod 8 and each XCD works through a contiguous
+// eighth of the renumbered grid (xcd_block), so an XCD covers R/8 regions of the domain
+// and gathers x mostly through its own L2.  R = 8 (one region per XCD) or 64 (eight per
+// XCD, each a smaller footprint of x), and plain or non-temporal loads of the matrix
+// stream: which of the four is fastest depends on the matrix (measured on MI355X: 128^3
+// 7-point FP64 46.4 us with 8 regions, 39.4 with 64; 27-point FP64 154.6 / 160.4 us,
+// 145.9 with 8 regions and non-temporal loads), so each value type takes the fastest of
+// ten products of each.  Every row keeps its sum order: the results are the same bits.
 static int set_sell_order(senk_mat A, const int *cptr, int cnum, int bsize) {
     if (!A->has_sell || !cptr || cnum <= 0) return SENK_OK;
     const senk_sellcs &d = A->sell;
@@ -1288,27 +1329,66 @@ static int set_sell_order(senk_mat A, const int *cptr, int cnum, int bsize) {
     Ctx *c = &A->ctx->c;
     SENK_HIP(hipMemcpyAsync(perm.data(), d.perm, perm.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SENK_HIP(hipStreamSynchronize(c->stream));
-    std::vector<int> eighth(ns);
+    // position of each slice inside its color, as a fraction in [0, 1)
+    std::vector<double> frac(ns);
     for (int q = 0; q < ns; q++) {
         const int row = perm[(size_t)q * C];
         int col = (int)(std::upper_bound(cptr, cptr + cnum + 1, row / bsize) - cptr) - 1;
         col = std::max(0, std::min(cnum - 1, col));
         const long long r0 = (long long)cptr[col] * bsize, r1 = (long long)cptr[col + 1] * bsize;
-        const long long e = r1 > r0 ? (8 * (row - r0)) / (r1 - r0) : 0;
-        eighth[q] = (int)std::max(0LL, std::min(7LL, e));
+        frac[q] = r1 > r0 ? (double)(row - r0) / (double)(r1 - r0) : 0.0;
     }
-    std::vector<int> order;
-    order.reserve(ns);
-    for (int e = 0; e < 8; e++)
-        for (int q = 0; q < ns; q++)
-            if (eighth[q] == e) order.push_back(q);
-    if (!A->sell_order) SENK_TRY(dalloc(&A->sell_order, (size_t)ns));
-    SENK_HIP(hipMemcpyAsync(A->sell_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
-                            c->stream));
+    const int regions[2] = {8, 64};
+    std::vector<int> orders[2];  // kept until the copies are done
+    for (int k = 0; k < 2; k++) {
+        const int R = regions[k];
+        std::vector<int> reg(ns);
+        std::vector<int> &order = orders[k];
+        order.reserve(ns);
+        for (int q = 0; q < ns; q++) reg[q] = std::max(0, std::min(R - 1, (int)(frac[q] * R)));
+        for (int e = 0; e < R; e++)
+            for (int q = 0; q < ns; q++)
+                if (reg[q] == e) order.push_back(q);
+        if (!A->sell_order[k]) SENK_TRY(dalloc(&A->sell_order[k], (size_t)ns));
+        SENK_HIP(hipMemcpyAsync(A->sell_order[k], order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
+                                c->stream));
+    }
+    // time the four configurations per value type on a zero vector (the matrix values
+    // and the access pattern, not x, set the time)
+    void *buf = nullptr;
+    const size_t ld = ((size_t)A->N + 3) & ~(size_t)3;  // 16-byte aligned second vector
+    SENK_TRY(ctx_work(c, 2 * ld * sizeof(double), &buf));
+    double *x = static_cast<double *>(buf), *y = x + ld;
+    SENK_HIP(hipMemsetAsync(x, 0, sizeof(double) * A->N, c->stream));
+    for (int t = 0; t < 2; t++) {
+        const bool f32 = t == 1;
+        if (f32 && !d.val32) continue;
+        float best = 0;
+        int best_k = 0;
+        bool best_nt = false;
+        for (int k = 0; k < 2; k++)
+            for (int nt = 0; nt < 2; nt++) {
+                float ms = 0;
+                SENK_TRY(f32 ? time_sell<float>(A, (const float *)x, (float *)y, A->sell_order[k], nt, 10, &ms)
+                             : time_sell<double>(A, x, y, A->sell_order[k], nt, 10, &ms));
+                if ((k == 0 && nt == 0) || ms < best) {
+                    best = ms;
+                    best_k = k;
+                    best_nt = nt;
+                }
+            }
+        if (f32) {
+            A->view.sell_order32 = A->sell_order[best_k];
+            A->view.sell_nt32 = best_nt;
+        } else {
+            A->view.sell_order64 = A->sell_order[best_k];
+            A->view.sell_nt64 = best_nt;
+        }
+    }
     SENK_HIP(hipStreamSynchronize(c->stream));
-    A->view.sell_order = A->sell_order;
     return SENK_OK;
 }
+extern "C" {
 
 int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *cind, const double *val,
                          const int *cptr, int cnum, int bsize, const senk_ilu_opts *opts, senk_prec *out) {

@@ -114,8 +114,11 @@ struct TriLayout {
     int mode = kTrsvXcd;
     unsigned ticket_base = 0;  // ticketed sweep: workgroups launched so far (mod 2^32)
     // per-XCD ticket streams (mode 3): stream x holds the x-th eighth of every level's
-    // slices, each level padded with -1 to the same number of 8-slice tickets in every
-    // stream; xwg tickets per stream, xslice[x * 8 * xwg + k]
+    // slices, each level padded to the same number of 8-slice tickets in every stream;
+    // xwg tickets per stream.  tord: slices numbered in ticket order (wave w of ticket j
+    // of stream x solves slice 64 j + 8 x + w; bands padded with empty slices); else
+    // slice order and the table xslice[x * 8 * xwg + 8 j + w] (-1: none)
+    bool tord = false;
     int *xslice = nullptr;
     int xwg = 0;
     unsigned xticket_base = 0;
@@ -153,7 +156,7 @@ int launch_spmv_bcsr(Ctx *c, int N, int bnl, int bnw, const int *brptr, const in
 template <typename T>
 int launch_spmv_sell(Ctx *c, int N, int C, int nslices, const int *wid, const int *cind, const T *val,
                      const int *perm, const T *x, T *y, bool last_slice_short, const int *stop = nullptr,
-                     double avg_width = 0, const int *order = nullptr);
+                     double avg_width = 0, const int *order = nullptr, bool nontemporal = false);
 
 template <typename T>
 int launch_sptrsv(Ctx *c, const Schedule &s, bool upper, const int *rptr, const int *cind, const T *val,

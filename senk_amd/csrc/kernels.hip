@@ -99,7 +99,7 @@ int launch_spmv_csr_diag_f64(Ctx *c, int N, const int *rptr, const int *cind, co
 // a contiguous 64-element access.  Slice s has rows h = C (or N % C for a short last
 // slice in the reference's Csr2Sell layout) and slot j of row k at
 // wid[s]*C + j*h + k.  perm (optional) maps a packed row to its output row.
-template <typename T, int CH>
+template <typename T, int CH, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices, const int *__restrict__ wid,
                                                       const int *__restrict__ cind, const T *__restrict__ val,
                                                       const int *__restrict__ perm, const T *__restrict__ x,
@@ -117,9 +117,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
     const int w = wid[s + 1] - wid[s];
     T acc = 0;
     // slot 0 starts the sum (as the reference's SpmvSell), later slots in order; CH slots
-    // of indices and values are loaded before their gathers are issued.  (Plain loads:
-    // with non-temporal hints the FP32 product on the 128^3 7-point system took 38 us
-    // instead of 29.5.)
+    // of indices and values are loaded before their gathers are issued.  NT: the matrix
+    // stream is read with non-temporal loads, which leave more of the L2 to x; whether that
+    // pays depends on the matrix (measured per matrix, see launch_spmv_sell's callers).
     for (int j0 = 0; j0 < w; j0 += CH) {
         int cc[CH];
         T vv[CH], xv[CH];
@@ -128,8 +128,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
             const int j = j0 + u;
             if (j < w) {
                 const size_t o = base + (size_t)j * h;
-                cc[u] = cind[o];
-                vv[u] = val[o];
+                if (NT) {
+                    cc[u] = __builtin_nontemporal_load(cind + o);
+                    vv[u] = __builtin_nontemporal_load(val + o);
+                } else {
+                    cc[u] = cind[o];
+                    vv[u] = val[o];
+                }
             }
         }
 #pragma unroll
@@ -145,28 +150,40 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, int C, int nslices,
     const int row = perm ? perm[(size_t)s * C + k] : s * C + k;
     if (row >= 0) y[row] = acc;
 }
+template <typename T, bool NT>
+static void spmv_sell_launch(Ctx *c, int grid, bool short_rows, int N, int C, int nslices, const int *wid,
+                             const int *cind, const T *val, const int *perm, const T *x, T *y, int last_short,
+                             const int *stop, const int *order) {
+    if (short_rows)
+        k_spmv_sell<T, 8, NT><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y, last_short,
+                                                              stop, order);
+    else
+        k_spmv_sell<T, 16, NT><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y, last_short,
+                                                               stop, order);
+}
 template <typename T>
 int launch_spmv_sell(Ctx *c, int N, int C, int nslices, const int *wid, const int *cind, const T *val,
                      const int *perm, const T *x, T *y, bool last_short, const int *stop, double avg_width,
-                     const int *order) {
+                     const int *order, bool nontemporal) {
     if (N <= 0 || nslices <= 0) return SENK_OK;
     long long threads = (long long)nslices * C;
     int grid = (int)((threads + kBlock - 1) / kBlock);
     // short rows (7-point stencils) keep all their slots in one chunk of 8; longer rows
     // take chunks of 16
-    if (avg_width > 0 && avg_width <= 8.5)
-        k_spmv_sell<T, 8><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y,
-                                                          last_short ? 1 : 0, stop, order);
+    const bool short_rows = avg_width > 0 && avg_width <= 8.5;
+    if (nontemporal)
+        spmv_sell_launch<T, true>(c, grid, short_rows, N, C, nslices, wid, cind, val, perm, x, y, last_short ? 1 : 0,
+                                  stop, order);
     else
-        k_spmv_sell<T, 16><<<grid, kBlock, 0, c->stream>>>(N, C, nslices, wid, cind, val, perm, x, y,
-                                                           last_short ? 1 : 0, stop, order);
+        spmv_sell_launch<T, false>(c, grid, short_rows, N, C, nslices, wid, cind, val, perm, x, y,
+                                   last_short ? 1 : 0, stop, order);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
 template int launch_spmv_sell<double>(Ctx *, int, int, int, const int *, const int *, const double *, const int *,
-                                      const double *, double *, bool, const int *, double, const int *);
+                                      const double *, double *, bool, const int *, double, const int *, bool);
 template int launch_spmv_sell<float>(Ctx *, int, int, int, const int *, const int *, const float *, const int *,
-                                     const float *, float *, bool, const int *, double, const int *);
+                                     const float *, float *, bool, const int *, double, const int *, bool);
 
 // =============================================================== SpMV BCSR
 // G lanes per block row; lane q takes blocks brptr[g]+q, +G, ...; each block is

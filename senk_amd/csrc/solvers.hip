@@ -475,7 +475,8 @@ int spmv(Ctx *c, const MatView &A, const T *x, T *y, const int *stop) {
             return launch_spmv_sell<T>(c, A.N, A.sell->C, A.sell->nslices, A.sell->slice_ptr, A.sell->cind, sv,
                                        A.sell->perm, x, y, false, stop,
                                        (double)A.sell->nnz_padded / ((double)A.sell->nslices * A.sell->C),
-                                       A.sell_order);
+                                       std::is_same<T, double>::value ? A.sell_order64 : A.sell_order32,
+                                       std::is_same<T, double>::value ? A.sell_nt64 : A.sell_nt32);
     }
     if (A.bnl) {
         const T *bv = bvals<T>(A);

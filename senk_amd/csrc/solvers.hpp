@@ -13,10 +13,13 @@ struct MatView {
     const double *v64 = nullptr;
     const float *v32 = nullptr;
     double avg_row = 0;
-    // optional SELL-C-sigma copy (device) and the order in which its slices are handed
-    // to the workgroups (null: slice order)
+    // optional SELL-C-sigma copy (device); per value type, the order in which its slices
+    // are handed to the workgroups (null: slice order) and whether the matrix stream is
+    // read with non-temporal loads (both chosen by timing when a colored preconditioner
+    // is built on the matrix; neither changes a result bit)
     const senk_sellcs *sell = nullptr;
-    const int *sell_order = nullptr;
+    const int *sell_order64 = nullptr, *sell_order32 = nullptr;
+    bool sell_nt64 = false, sell_nt32 = false;
     // optional BCSR form used for SpMV (device); bnl == 0 when absent
     int bnl = 0, bnw = 0;
     const int *brptr = nullptr, *bcind = nullptr;

@@ -248,6 +248,10 @@ __device__ __forceinline__ double canonical(double v) { return __builtin_canonic
 // sync words: error flag, the ticket counter of mode 2, and one counter per XCD stream of
 // mode 3 (128 bytes apart)
 constexpr int kSweepErr = 0, kSweepTicket = 1, kSweepXTicket = 32, kSweepWords = 32 + 32 * kXcds;
+// workgroup size (waves = slices per ticket) of the per-XCD ticket streams: with one
+// counter per XCD, 8-wave workgroups measured best (FP32 application at 128^3: 71.5 us,
+// 77.4 with 16 waves, 73.0 with 4)
+constexpr int kXTicketWaves = 8;
 // polls of one pending gather before the wave gives up and raises the time-out flag;
 // a poll takes about a microsecond, so this bounds a wait at a few seconds
 constexpr int kSweepSpinLimit = 1 << 22;
@@ -403,7 +407,7 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
 
 // One launch = one schedule level (slices slice0 .. slice0 + nsl).
 template <typename T, typename V, int BS, bool UPPER>
-__global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, const int4 *__restrict__ sinfo,
+__global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, int tord, const int4 *__restrict__ sinfo,
                                                         const int *__restrict__ blk_list,
                                                         const typename RowWord<BS>::W *__restrict__ rw,
                                                         const int *__restrict__ next, const int *__restrict__ scol,
@@ -413,9 +417,18 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, con
                                                         const double *__restrict__ in_scale,
                                                         const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) return;
-    // each XCD takes a contiguous part of the level: within one color, neighbouring
-    // slices cover neighbouring parts of the domain, so their gathers share that L2
-    const int w = xcd_block() * kWavesPerBlock + (threadIdx.x >> 6);
+    // each XCD takes one band of the level: within one color, neighbouring slices cover
+    // neighbouring parts of the domain, so their gathers share that L2.  In slice order
+    // the band is a contiguous part of the level; in ticket order (tord) XCD x holds
+    // slices 64k + 8x .. 64k + 8x + 7 of every group of 64, two workgroups per 8 slices.
+    int w;
+    if (tord) {
+        const int b = (int)blockIdx.x, i = b / kXcds;
+        w = (i >> 1) * (kXcds * kXTicketWaves) + (b % kXcds) * kXTicketWaves + (i & 1) * kWavesPerBlock +
+            (int)(threadIdx.x >> 6);
+    } else {
+        w = xcd_block() * kWavesPerBlock + (int)(threadIdx.x >> 6);
+    }
     if (w >= nsl) return;
     solve_slice<T, V, BS, UPPER, false>(slice0 + w, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y,
                                         in_scale, nullptr, nullptr);
@@ -484,12 +497,15 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
 // b mod 8 (the XCD the dispatcher hands it to) and draws that stream's tickets in
 // arrival order; stream x solves the x-th eighth of every level (a band of the domain),
 // so most gathers find their rows in the XCD's own L2.  Every stream has the same
-// number of tickets per level (slices padded with -1), so ticket j of any stream is in
-// the same level: a wait is only on tickets < j of the other streams, whose workgroups
-// were handed out before this one.
-// workgroup size of mode 3: with one counter per XCD, 8-wave workgroups measured best
-// (FP32 application at 128^3: 71.5 us, 77.4 with 16 waves, 73.0 with 4)
-constexpr int kXTicketWaves = 8;
+// number of tickets per level (bands padded with empty slices), so ticket j of any
+// stream is in the same level: a wait is only on tickets < j of the other streams, whose
+// workgroups were handed out before this one.
+// In ticket order (xslice null) the slices themselves are numbered so that wave w of
+// ticket j of stream x solves slice 64 j + 8 x + w: the wave goes from its ticket to its
+// slice descriptor and row words without a lookup.  Otherwise xslice[x][j][w] names the
+// slice (-1: none).  The stop flag is read while the ticket is drawn; a stopped launch
+// still draws one ticket per workgroup, so every launch advances each stream's counter by
+// exactly xwg.
 template <typename T, typename V, int BS, bool UPPER>
 __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     int xwg, unsigned base, const int *__restrict__ xslice, const int4 *__restrict__ sinfo,
@@ -500,20 +516,19 @@ __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     const int *__restrict__ stop_flag) {
     const int xs = (int)(blockIdx.x % kXcds);
     unsigned *ctr = sync + kSweepXTicket + 32 * xs;
-    if (stop_flag && *stop_flag) {
-        if (blockIdx.x < kXcds && threadIdx.x == 0)
-            __hip_atomic_fetch_add(ctr, (unsigned)xwg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return;
-    }
+    const int stopped = stop_flag ? *stop_flag : 0;
     __shared__ unsigned ticket;
     if (threadIdx.x == 0) ticket = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base;
     __syncthreads();
+    if (stopped) return;
     const unsigned q = ticket;
     if (q >= (unsigned)xwg) {  // launches out of step with the host's count
         if (threadIdx.x == 0) __hip_atomic_store(sync + kSweepErr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    const int slice = xslice[(size_t)xs * kXTicketWaves * xwg + (size_t)q * kXTicketWaves + (threadIdx.x >> 6)];
+    const int wv = (int)(threadIdx.x >> 6);
+    const int slice = xslice ? xslice[(size_t)xs * kXTicketWaves * xwg + (size_t)q * kXTicketWaves + wv]
+                             : (int)q * (kXcds * kXTicketWaves) + xs * kXTicketWaves + wv;
     if (slice >= 0)
         solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
                                            reset, sync + kSweepErr, copy, pre_e, pre_w);
@@ -567,10 +582,12 @@ int solve_levels(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
         const int sl0 = L.step_slice[q], nsl = L.step_slice[q + 1] - sl0;
         if (L.upper)
             k_sptrsv_sell<T, V, BS, true><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y, nullptr, stop);
+                sl0, nsl, L.tord ? 1 : 0, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
+                nullptr, stop);
         else
             k_sptrsv_sell<T, V, BS, false><<<waves_grid(nsl), kBlock, 0, c->stream>>>(
-                sl0, nsl, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y, in_scale, stop);
+                sl0, nsl, L.tord ? 1 : 0, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
+                in_scale, stop);
         SENK_LAUNCH_CHECK();
     }
     return SENK_OK;
@@ -644,7 +661,7 @@ int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V
                    const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
                    const T *pre_e, T *pre_w, const int *stop) {
     using W = typename RowWord<BS>::W;
-    if (!L.xslice || L.xwg <= 0) return SENK_ERR_ARG;
+    if ((!L.xslice && !L.tord) || L.xwg <= 0) return SENK_ERR_ARG;
     const unsigned base = L.xticket_base;
     k_sptrsv_xticket<T, V, BS, UPPER><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(
         L.xwg, base, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
@@ -718,41 +735,65 @@ int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rp
     L->bsize = BS;
     L->upper = upper;
     L->n = N;
-    // steps = non-empty levels in solve order; slices numbered in that order
-    std::vector<int4> si;
-    L->step_slice.assign(1, 0);
-    const int per = 64 / BS;
+    // steps = non-empty levels in solve order; slices numbered in that order.  Mode 3
+    // streams: stream x takes the x-th eighth (band) of every step's slices, every step
+    // padded to the same number of 8-slice tickets in every stream.
+    const int per = 64 / BS, group = kXcds * kXTicketWaves;
+    std::vector<int> lev_of, nsl_of, tk_of;
+    long long plain = 0, padded = 0;
     for (int q = 0; q < s.nlev; q++) {
         const int l = upper ? s.nlev - 1 - q : q;
-        const int b0 = s.lev_ptr[l], nb = s.lev_ptr[l + 1] - b0;
-        const int nsl = slices_of(nb, BS);
+        const int nsl = slices_of(s.lev_ptr[l + 1] - s.lev_ptr[l], BS);
         if (nsl <= 0) continue;
-        for (int w = 0; w < nsl; w++) si.push_back(make_int4(0, 0, b0 + w * per, std::min(per, nb - w * per)));
+        int tk = 0;
+        for (int x = 0; x < kXcds; x++) {
+            const int cnt = (int)((long long)nsl * (x + 1) / kXcds - (long long)nsl * x / kXcds);
+            tk = std::max(tk, (cnt + kXTicketWaves - 1) / kXTicketWaves);
+        }
+        lev_of.push_back(l);
+        nsl_of.push_back(nsl);
+        tk_of.push_back(tk);
+        plain += nsl;
+        padded += (long long)tk * group;
+    }
+    // Ticket order (slice 64 j + 8 x + w = wave w of ticket j of stream x, empty slices
+    // where a band runs short) when the padding at most doubles the slices; otherwise
+    // slice order and a ticket -> slice table (many small levels, e.g. level schedules).
+    L->tord = padded <= 2 * plain && padded <= INT32_MAX;
+    std::vector<int4> si;
+    std::vector<int> table[kXcds];
+    L->step_slice.assign(1, 0);
+    for (size_t q = 0; q < lev_of.size(); q++) {
+        const int l = lev_of[q], nsl = nsl_of[q], tk = tk_of[q];
+        const int b0 = s.lev_ptr[l], nb = s.lev_ptr[l + 1] - b0;
+        auto slice_of = [&](int w) { return make_int4(0, 0, b0 + w * per, std::min(per, nb - w * per)); };
+        if (L->tord) {
+            for (int k = 0; k < tk; k++)
+                for (int x = 0; x < kXcds; x++) {
+                    const int lo = (int)((long long)nsl * x / kXcds), hi = (int)((long long)nsl * (x + 1) / kXcds);
+                    for (int w = 0; w < kXTicketWaves; w++) {
+                        const int ls = lo + k * kXTicketWaves + w;
+                        si.push_back(ls < hi ? slice_of(ls) : make_int4(0, 0, 0, 0));
+                    }
+                }
+        } else {
+            const int a = (int)si.size();
+            for (int w = 0; w < nsl; w++) si.push_back(slice_of(w));
+            for (int x = 0; x < kXcds; x++) {
+                const int lo = a + (int)((long long)nsl * x / kXcds), hi = a + (int)((long long)nsl * (x + 1) / kXcds);
+                for (int k = 0; k < tk * kXTicketWaves; k++) table[x].push_back(lo + k < hi ? lo + k : -1);
+            }
+        }
         L->step_slice.push_back((int)si.size());
     }
     L->nsteps = (int)L->step_slice.size() - 1;
     L->nslices = (int)si.size();
     if (L->nslices == 0) return SENK_OK;
-    // mode 3 streams: the x-th eighth of every step's slices, steps padded to the same
-    // number of tickets in every stream
-    {
-        std::vector<std::vector<int>> xs(kXcds);
-        for (int q = 0; q < L->nsteps; q++) {
-            const int a = L->step_slice[q], n = L->step_slice[q + 1] - a;
-            int tk = 0;
-            for (int x = 0; x < kXcds; x++) {
-                const int cnt = (int)((long long)n * (x + 1) / kXcds - (long long)n * x / kXcds);
-                tk = std::max(tk, (cnt + kXTicketWaves - 1) / kXTicketWaves);
-            }
-            for (int x = 0; x < kXcds; x++) {
-                const int lo = a + (int)((long long)n * x / kXcds), hi = a + (int)((long long)n * (x + 1) / kXcds);
-                for (int k = 0; k < tk * kXTicketWaves; k++) xs[x].push_back(lo + k < hi ? lo + k : -1);
-            }
-        }
-        L->xwg = (int)(xs[0].size() / kXTicketWaves);
+    L->xwg = (int)(padded / group);
+    if (!L->tord) {
         std::vector<int> flat;
-        flat.reserve((size_t)kXcds * xs[0].size());
-        for (int x = 0; x < kXcds; x++) flat.insert(flat.end(), xs[x].begin(), xs[x].end());
+        flat.reserve((size_t)padded);
+        for (int x = 0; x < kXcds; x++) flat.insert(flat.end(), table[x].begin(), table[x].end());
         if (hipMalloc(&L->xslice, flat.size() * sizeof(int)) != hipSuccess ||
             hipMemcpy(L->xslice, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
             tri_layout_free(L);

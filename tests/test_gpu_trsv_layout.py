@@ -122,6 +122,18 @@ def test_layout_natural_order_levels(ctx, bsize):
     _check(ctx, sy, S.SENK_ILU0, 0, bsize, colors=False)
 
 
+@pytest.mark.parametrize("method,n,bsize", [("simple", 32, 8), ("simple", 24, 4), ("connect", 40, 8)])
+def test_layout_ticket_order(ctx, method, n, bsize):
+    # levels of hundreds of slices: padding every level to whole groups of 64 slices (8
+    # streams x 8 waves) adds less than the slices themselves, so the layout numbers its
+    # slices in ticket order (32^3 simple: 2 levels of 256 slices, no padding; 24^3
+    # simple: 108 -> 128 per level; 40^3 connect: ~100 -> 128).  The 16^3 cases above
+    # (a few slices per level) keep slice order and the ticket table.
+    rptr, cind, val = S.gen_poisson7(n)
+    sy = abmc_system(rptr, cind, val, bsize, method=method)
+    _check(ctx, sy, S.SENK_ILU0, 0, bsize, colors=True)
+
+
 def test_layout_partial_last_slice(ctx):
     # colors whose row counts are not multiples of 64 leave partial slices
     rptr, cind, val = S.gen_poisson7(10, 9, 7)
