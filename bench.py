@@ -3,8 +3,10 @@
 Headline workload (one GPU): the 3D 7-point variable-coefficient diffusion system
 -div(k grad u) = f on a 128^3 grid (n = 2,097,152, nnz = 14,581,760; k = exp(Gaussian
 random field, correlation length 8 cells, sigma 1.5), seed 1; harmonic-mean face
-coefficients), ABMC ordering ("connect"), ILU(0), right-hand side b = A x* with
-x* ~ U[-1,1] from seed 1, x0 = 0, target 1e-10 relative residual.  One step is one full
+coefficients), ILU(0), right-hand side b = A x* with x* ~ U[-1,1] from seed 1, x0 = 0,
+target 1e-10 relative residual, under three ABMC orderings: "connect" blocks of 8 (_b8)
+and of 4 (_b4), and "simple" blocks of 8 consecutive rows (_s8: the block graph of the
+7-point grid is 2-colorable, so each triangular solve has 2 levels).  One step is one full
 solve with every input resident in HBM (setup timed separately, as the reference's
 driver times only the solve).
 
@@ -15,19 +17,24 @@ driver times only the solve).
                  --steps); config.mixed names the solve
   fp64_gmres30   the plain FP64 ILU(0)-GMRES(30) solve it is compared with (the
                  reference's AbmcIluGmresm algorithm, modified Gram-Schmidt), at its
-                 fastest ABMC block size; fp64_gmres30_cgs2 the same with two-pass
-                 classical Gram-Schmidt (reported beside it, not the baseline)
+                 fastest ordering; fp64_gmres30_cgs2 the same with two-pass classical
+                 Gram-Schmidt (reported beside it, not the baseline)
   fp64_ic0_cg    FP64 conjugate gradients with the incomplete Cholesky factor (the system
                  is symmetric positive definite); mixed_cg_* the nested mixed form with an
                  FP32 inner CG
-  learned_*      the same with the trained message-passing correction of the ILU
-                 application (weights from senk_amd/data/gnn_vardiff.npz)
+  learned_*      with the trained message-passing correction: learned_fgcr30 /
+                 learned_mixed_fgcr correct every ILU application of an FGCR (weights
+                 senk_amd/data/gnn_vardiff.npz); learned_mixed_cg corrects the inner
+                 CG's result once per outer FGCR step of the nested solve (weights
+                 senk_amd/data/gnn_vardiff_nested.npz, trained on that solve's pairs)
   plain_fp64_c2  the 27-point convection-diffusion 128^3 system (Pe 10, ABMC blocks of
                  16): FP64 ILU(0)-GMRES(30) and ILU(0)-BiCGStab, and SpMV rates
-  roofline       FP32 ILU(0) application of the headline solve (L and U sweeps, the dominant
-                 work of an inner step): algorithmic bytes / average application
-                 time (HIP events on the library stream) against 8 TB/s; roofline_spmv the
-                 same for the FP32 SELL-C-sigma SpMV
+  roofline       FP32 ILU(0) application of the headline solve's system (L and U sweeps,
+                 the dominant work of an inner step): algorithmic bytes / average
+                 application time (HIP events on the library stream, back-to-back) against
+                 8 TB/s; roofline_spmv the same for the FP32 SELL-C-sigma SpMV; *_cold the
+                 same for single launches after a 1 GiB stream (nothing left in the 256 MB
+                 last-level cache)
   cpu_baseline   the reference's own OpenMP AbmcIluGmresm (oracle/_ref) on the host cores,
                  the full FP64 solve of the same system and factors
   ingest         MatrixMarket reading + symmetric expansion on the host: the library's
@@ -165,7 +172,32 @@ def time_kernel(ctx, fn, reps):
     return ms.value / reps
 
 
-def kernel_rates(ctx, dev, reps=50, sweep_modes=False):
+def time_cold(ctx, fn, reps, flush):
+    """Average time of fn() in ms, each call timed alone with HIP events on the library's
+    stream right after flush(), which reads 1 GiB: none of fn's data is then left in the
+    256 MB last-level cache, so every byte comes from HBM."""
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    S.call("senk_event_create", ctx.h, C.byref(e0))
+    S.call("senk_event_create", ctx.h, C.byref(e1))
+    fn()
+    total = 0.0
+    for _ in range(reps):
+        flush()
+        S.call("senk_event_record", ctx.h, e0)
+        fn()
+        S.call("senk_event_record", ctx.h, e1)
+        ms = C.c_float()
+        S.call("senk_event_elapsed_ms", e0, e1, C.byref(ms))
+        total += ms.value
+    S.lib().senk_event_destroy(e0)
+    S.lib().senk_event_destroy(e1)
+    return total / reps
+
+
+def kernel_rates(ctx, dev, reps=50, sweep_modes=False, cold=True):
+    """Back-to-back launches (warm: the 128^3 7-point matrix, 142 MB in FP32, stays largely
+    in the 256 MB last-level cache from one launch to the next, as it does between the
+    products of a solve) and, with cold, single launches after a 1 GiB stream."""
     N, nnz = dev.N, dev.sy["nnz"]
     dx = ctx.array(np.random.default_rng(0).uniform(-1, 1, N))
     dy = ctx.zeros(N)
@@ -175,11 +207,24 @@ def kernel_rates(ctx, dev, reps=50, sweep_modes=False):
     t32 = time_kernel(ctx, lambda: S.call("senk_mat_spmv_f32", dev.A, dx32.p, dy32.p), reps)
     p64 = time_kernel(ctx, lambda: S.call("senk_prec_apply_f64", dev.M, dx.p, dy.p), 20)
     p32 = time_kernel(ctx, lambda: S.call("senk_prec_apply_f32", dev.M, dx32.p, dy32.p), 20)
-    out = {"spmv_sell_f64_us": t64 * 1e3, "spmv_sell_f64_GBps": spmv_bytes(N, nnz, 8) / t64 / 1e6,
-           "spmv_sell_f32_us": t32 * 1e3, "spmv_sell_f32_GBps": spmv_bytes(N, nnz, 4) / t32 / 1e6,
-           "ilu_apply_f64_us": p64 * 1e3, "ilu_apply_f64_GBps": ilu_bytes(N, dev.lnnz, dev.unnz, 8) / p64 / 1e6,
-           "ilu_apply_f32_us": p32 * 1e3, "ilu_apply_f32_GBps": ilu_bytes(N, dev.lnnz, dev.unnz, 4) / p32 / 1e6,
+    b64, b32 = spmv_bytes(N, nnz, 8), spmv_bytes(N, nnz, 4)
+    i64, i32 = ilu_bytes(N, dev.lnnz, dev.unnz, 8), ilu_bytes(N, dev.lnnz, dev.unnz, 4)
+    out = {"spmv_sell_f64_us": t64 * 1e3, "spmv_sell_f64_GBps": b64 / t64 / 1e6,
+           "spmv_sell_f32_us": t32 * 1e3, "spmv_sell_f32_GBps": b32 / t32 / 1e6,
+           "ilu_apply_f64_us": p64 * 1e3, "ilu_apply_f64_GBps": i64 / p64 / 1e6,
+           "ilu_apply_f32_us": p32 * 1e3, "ilu_apply_f32_GBps": i32 / p32 / 1e6,
            "colors": dev.sy["nc"], "bsize": dev.sy["bsize"]}
+    if cold:
+        n = (1 << 29) // 8  # two vectors of 512 MiB, read by a dot product (no dirty lines)
+        fa, fb, fr = ctx.zeros(n), ctx.zeros(n), ctx.zeros(1)
+        flush = lambda: S.call("senk_dot_f64_dev", ctx.h, fa.p, fb.p, n, fr.p)
+        c64 = time_cold(ctx, lambda: S.call("senk_mat_spmv_f64", dev.A, dx.p, dy.p), 10, flush)
+        c32 = time_cold(ctx, lambda: S.call("senk_mat_spmv_f32", dev.A, dx32.p, dy32.p), 10, flush)
+        q32 = time_cold(ctx, lambda: S.call("senk_prec_apply_f32", dev.M, dx32.p, dy32.p), 10, flush)
+        out["cold"] = {"spmv_sell_f64_us": c64 * 1e3, "spmv_sell_f64_GBps": b64 / c64 / 1e6,
+                       "spmv_sell_f32_us": c32 * 1e3, "spmv_sell_f32_GBps": b32 / c32 / 1e6,
+                       "ilu_apply_f32_us": q32 * 1e3, "ilu_apply_f32_GBps": i32 / q32 / 1e6}
+        del fa, fb, fr
     if not sweep_modes:
         return out
     # triangular-solve launch modes: 1 one cooperative sweep per triangle, 3 one launch per
@@ -461,6 +506,8 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the 96^3 block (3-dof elasticity) workload")
     ap.add_argument("--no-ingest", action="store_true", help="skip the MatrixMarket ingestion entry")
     ap.add_argument("--kernels-only", action="store_true", help="kernel rates and the copy calibration only")
+    ap.add_argument("--cold", action="store_true",
+                    help="--kernels-only: also the single-launch rates after a 1 GiB read (the full run always)")
     ap.add_argument("--learned-only", action="store_true",
                     help="time the learned correction's application alone (FP32, 10 applications)")
     ap.add_argument("--run", default=None, help="time only the solve configuration of this name (profiling)")
@@ -500,12 +547,13 @@ def main():
         net.close()
         d8.close()
         return
-    kern = kernel_rates(ctx, d8, sweep_modes=args.sweep_modes)
-    cal = copy_rate(ctx)
-    log(f"[bench] kernels: {json.dumps(kern)} copy: {json.dumps(cal)}")
-    if args.kernels_only:
-        print(json.dumps({"kernels": kern, "copy": cal}), flush=True)
-        return
+    if not args.run:  # (a --run trace holds that solve's kernels only)
+        kern = kernel_rates(ctx, d8, sweep_modes=args.sweep_modes, cold=args.cold or not args.kernels_only)
+        cal = copy_rate(ctx)
+        log(f"[bench] kernels: {json.dumps(kern)} copy: {json.dumps(cal)}")
+        if args.kernels_only:
+            print(json.dumps({"kernels": kern, "copy": cal}), flush=True)
+            return
     sy4 = build_system_kind("vardiff", n, 4)
     d4 = Device(ctx, sy4)
     # "simple" ABMC blocking (blocks of 8 consecutive rows): the block graph of a 7-point
@@ -523,7 +571,9 @@ def main():
                 "mixed_bicgstab_s8": (ds8, dict(solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED)),
                 "learned_fgcr30_fp64_b8": (d8, dict(solver=S.SENK_FGCR, m=30, eps=1e-10, max_iter=6000,
                                                     check_every=1, use_learned=1)),
-                "learned_mixed_fgcr_b8": (d8, dict(solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED))}
+                "learned_mixed_fgcr_b8": (d8, dict(solver=S.SENK_FGCR, inner_max=30, use_learned=1, **MIXED)),
+                "learned_mixed_cg_b8": (d8, dict(solver=S.SENK_CG, inner_max=INNER_CG, use_learned=1, **MIXED)),
+                "learned_mixed_cg_s8": (ds8, dict(solver=S.SENK_CG, inner_max=INNER_CG, use_learned=1, **MIXED))}
         if args.run == "fp64_ic0_cg_b8":
             d4.close()
             d4 = Device(ctx, d8.sy, kind=S.SENK_IC0)  # the IC(0) factor of the same system
@@ -531,7 +581,7 @@ def main():
         dev, kw = cfgs[args.run]
         net = None
         if kw.get("use_learned"):
-            net, _ = load_learned(ctx)
+            net, _ = load_learned(ctx, nested=args.run.startswith("learned_mixed_cg"))
             if net is None:
                 raise SystemExit("no trained network for " + args.run)
             net.bind(dev.A)
@@ -586,11 +636,13 @@ def main():
     # split inner CG), trained on that solve's own (residual, inner result) pairs
     netn, meta_n = load_learned(ctx, nested=True)
     if netn is not None:
-        netn.bind(d8.A)
-        netn.attach(d8.M)
-        runs["learned_mixed_cg_b8"] = timed(d8, K, W, solver=S.SENK_CG, inner_max=INNER_CG, use_learned=1, **MIXED)
-        log(f"[bench] learned_mixed_cg_b8: {json.dumps(slim(runs['learned_mixed_cg_b8']))}")
-        S.call("senk_prec_attach_gnn", d8.M, None)
+        for tag, dev in (("b8", d8), ("s8", ds8)):
+            netn.bind(dev.A)
+            netn.attach(dev.M)
+            runs[f"learned_mixed_cg_{tag}"] = timed(dev, K, W, solver=S.SENK_CG, inner_max=INNER_CG, use_learned=1,
+                                                    **MIXED)
+            log(f"[bench] learned_mixed_cg_{tag}: {json.dumps(slim(runs[f'learned_mixed_cg_{tag}']))}")
+            S.call("senk_prec_attach_gnn", dev.M, None)
         netn.close()
 
     def ok(r):  # a solve that counts: converged, FP64 true residual at the target
@@ -641,7 +693,8 @@ def main():
         "speedup_vs_fp64_same_method": None if fam_key not in runs else {
             "fp64": fam_key, "speedup": round(runs[fam_key]["t_solve_ms_median"] / value, 3)},
         "learned_vs_fp64_gmres30": {k: round(base["t_solve_ms_median"] / runs[k]["t_solve_ms_median"], 3)
-                                    for k in ("learned_mixed_fgcr_b8", "learned_mixed_cg_b8") if k in runs},
+                                    for k in ("learned_mixed_fgcr_b8", "learned_mixed_cg_b8", "learned_mixed_cg_s8")
+                                    if k in runs},
         "runs": {k: slim(v) for k, v in runs.items()},
         "kernels": kern, "kernels_headline_system": hkern, "copy_calibration": cal,
         "learned_training": {name: {k: m_[k] for k in ("mode", "count", "sizes", "epochs", "loss_hist", "t_train_s",
@@ -656,11 +709,15 @@ def main():
         "roofline": {"kernel": f"ilu0_apply_f32: L + U sweeps, per-XCD ticket streams (vardiff7 {n}^3, {hname})",
                      "bound": "hbm", "achieved": round(hkern["ilu_apply_f32_GBps"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(hkern["ilu_apply_f32_GBps"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile(f"ilu_apply_f32_{hprof}.md")},
+                     "traffic": traffic_from_profile(f"ilu_apply_f32_{hprof}.md"),
+                     "achieved_cold": round(hkern["cold"]["ilu_apply_f32_GBps"], 1),
+                     "frac_cold": round(hkern["cold"]["ilu_apply_f32_GBps"] / HBM_PEAK_GBS, 4)},
         "roofline_spmv": {"kernel": f"spmv_sellcs_f32 (vardiff7 {n}^3, {hname})", "bound": "hbm",
                           "achieved": round(hkern["spmv_sell_f32_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(hkern["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4),
-                          "traffic": traffic_from_profile(f"spmv_sell_f32_{hprof}.md")},
+                          "traffic": traffic_from_profile(f"spmv_sell_f32_{hprof}.md"),
+                          "achieved_cold": round(hkern["cold"]["spmv_sell_f32_GBps"], 1),
+                          "frac_cold": round(hkern["cold"]["spmv_sell_f32_GBps"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "host": info,
         # host-side MatrixMarket ingestion (64^3 file; --ingest --n 128 for the full size)
@@ -696,23 +753,37 @@ def reference_entry(r, gpu_iters):
     return out
 
 
+# the nested mixed BiCGStab on the 27-point system: 120 inner steps to 1e-4 (sweeps of
+# 60 / 120 / 300 steps to 1e-3 / 1e-4 / 1e-6, and GMRES(30) inner solves: this the fastest)
+C2_INNER = dict(inner_max=120, inner_eps=1e-4)
+
+
 def config_c2(ctx, n, K, no_cpu):
     """27-point convection-diffusion 128^3 (Pe 10), ABMC blocks of 16: plain FP64
     ILU(0)-GMRES(30) and ILU(0)-BiCGStab, the mixed form, SpMV and ILU rates, and the
-    reference's AbmcIluGmresm count on the same system and factors."""
+    reference's AbmcIluGmresm count on the same system and factors; then the same solves
+    with "simple" blocks of 16 (8 colors instead of 13)."""
     log(f"[bench] building the {n}^3 27-point convection-diffusion system (ABMC blocks of 16)")
     sy = build_system_kind("convdiff", n, 16)
     dc = Device(ctx, sy)
     out = {"workload": f"convdiff27 {n}^3 (Pe 10), ABMC connect blocks of 16, ILU(0), tol 1e-10",
            "n": sy["N"], "nnz": sy["nnz"], "kernels": kernel_rates(ctx, dc)}
+    mixed = dict(MIXED, **C2_INNER)
     out["fp64_gmres30"] = slim(timed(dc, K, 1, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000))
     out["fp64_bicgstab"] = slim(timed(dc, K, 1, solver=S.SENK_BICGSTAB, eps=1e-10, max_iter=6000))
-    out["mixed_bicgstab"] = slim(timed(dc, K, 1, solver=S.SENK_BICGSTAB, inner_max=INNER_BICGSTAB, **MIXED))
+    out["mixed_bicgstab"] = slim(timed(dc, K, 1, solver=S.SENK_BICGSTAB, **mixed))
     if not no_cpu:
         gi = out["fp64_gmres30"]["iters"]
         out["reference_gmres30"] = reference_entry(ref_abmc_gmres(dc, 30, 1e-10, 40.0, gi), gi)
-    log(f"[bench] c2: {json.dumps(out)}")
     dc.close()
+    ss = build_system_kind("convdiff", n, 16, method="simple")
+    ds = Device(ctx, ss)
+    out["simple16"] = {"colors": ss["nc"], "kernels": kernel_rates(ctx, ds, cold=False),
+                       "fp64_gmres30": slim(timed(ds, K, 1, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000)),
+                       "fp64_bicgstab": slim(timed(ds, K, 1, solver=S.SENK_BICGSTAB, eps=1e-10, max_iter=6000)),
+                       "mixed_bicgstab": slim(timed(ds, K, 1, solver=S.SENK_BICGSTAB, **mixed))}
+    ds.close()
+    log(f"[bench] c2: {json.dumps(out)}")
     return out
 
 
@@ -748,7 +819,7 @@ def config_c3(ctx, K, no_cpu, n=4194304):
 def config_c5(ctx, K, no_cpu, n=96):
     """Block path: 3 dof per node on n^3 (3x3-block 7-point stencil, nu 0.3, lognormal
     moduli), ABMC blocks of 48: ILUB (3x3 block fill) and ILU(1) over the colors, FP32
-    factors + FP64 refinement + FGMRES(50), with and without the learned correction; error
+    factors + FP64 refinement + FGMRES(50), with ILUB also with the learned correction; error
     against the FP64 ILUB-GMRES(50) solve to 1e-14; the plain FP64 ILUB-GMRES(50) and the
     reference's AbmcIlubGmresm<double,3,3> count beside it."""
     log(f"[bench] building the {n}^3 3-dof block system (ABMC blocks of 48)")
@@ -772,11 +843,11 @@ def config_c5(ctx, K, no_cpu, n=96):
         r = timed(dev, K, 1, keep_x=True, **mixed)
         r["err_vs_fp64_solve"] = relerr(r.pop("x"), x64)
         out[f"mixed_{tag}_fgmres50"] = slim(r)
-        net, _ = load_learned(ctx)
+        net, _ = load_learned(ctx) if tag == "ilub" else (None, None)
         if net is not None:
             net.bind(dev.A)
             net.attach(dev.M)
-            r = timed(dev, K, 1, keep_x=True, use_learned=1, **mixed)
+            r = timed(dev, min(K, 3), 1, keep_x=True, use_learned=1, **mixed)
             r["err_vs_fp64_solve"] = relerr(r.pop("x"), x64)
             out[f"learned_mixed_{tag}_fgmres50"] = slim(r)
             S.call("senk_prec_attach_gnn", dev.M, None)

@@ -235,3 +235,32 @@ def test_pairs_with_ilu_outputs_equal_the_ilu_path(ctx):
         net.close()
         for s in sets:
             s.close()
+
+
+def test_shipped_nested_network_against_plain(ctx):
+    """A/B of the stored nested-setting network (senk_amd/data/gnn_vardiff_nested.npz,
+    trained on the GPU on the pairs the nested solve itself produces) on held-out
+    matrices of the family at the nested solve's own settings: with and without the
+    correction both reach the FP64 tolerance, the stagnation monitor keeps it on, and it
+    costs at most a few inner iterations.  Measured, it gains none either: the inner CG
+    leaves the smooth error components, which three hops of message passing cannot
+    represent (the iteration counts are printed)."""
+    if not N.NESTED_WEIGHTS.exists():
+        pytest.skip("no stored nested-setting network")
+    theta, _ = N.load(N.NESTED_WEIGHTS)
+    net = N.Gnn(ctx, theta)
+    try:
+        for n, seed in ((24, 901), (32, 902)):
+            held = N.System(ctx, n, seed, bsize=8)
+            try:
+                plain = N.nested_solve(ctx, held)
+                learned = N.nested_solve(ctx, held, net)
+                print(f"\n[learned nested] {n}^3: inner iterations plain {plain['inner']}, learned "
+                      f"{learned['inner']}; outer steps {plain['outer']} / {learned['outer']}")
+                assert plain["converged"] and learned["converged"]
+                assert learned["true_relres"] <= 1e-10 and learned["fell_back"] == 0
+                assert learned["inner"] <= plain["inner"] + max(3, plain["inner"] // 20)
+            finally:
+                held.close()
+    finally:
+        net.close()
