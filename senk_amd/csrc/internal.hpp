@@ -115,13 +115,12 @@ struct TriLayout {
     unsigned ticket_base = 0;  // ticketed sweep: workgroups launched so far (mod 2^32)
     // per-XCD ticket streams (mode 3): stream x holds the x-th eighth of every level's
     // slices, each level padded to the same number of 8-slice tickets in every stream;
-    // xwg tickets per stream.  tord: slices numbered in ticket order (wave w of ticket j
+    // xwg tickets (workgroups) per stream.  tord: slices numbered in ticket order (wave w of ticket j
     // of stream x solves slice 64 j + 8 x + w; bands padded with empty slices); else
     // slice order and the table xslice[x * 8 * xwg + 8 j + w] (-1: none)
     bool tord = false;
     int *xslice = nullptr;
     int xwg = 0;
-    unsigned xticket_base = 0;
 };
 int tri_layout_check(Ctx *c, TriLayout *L);
 int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rptr, const int *cind,
@@ -138,11 +137,12 @@ int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr
 // out = self^-1 in for one triangle alone (self L or U, other the opposite triangle of
 // the same factorization); in may equal out.  Calls on one factorization must alternate
 // L, U, L, ... (an ILU application counts as L then U).
-// With pre_w (single-launch modes only; *fused says whether it was done) the sweep also
-// writes pre_w = in - pre_e * out.
+// With both epilogue vectors (single-launch modes only; *fused says whether they were
+// used) a U sweep also writes ep_b = in - ep_a * out, and an L sweep writes
+// out = ep_a * ep_b + L^-1 in (ep_b read; not the same vector as out).
 template <typename T>
 int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
-                      const int *stop, const T *pre_e = nullptr, T *pre_w = nullptr, bool *fused = nullptr);
+                      const int *stop, const T *ep_a = nullptr, T *ep_b = nullptr, bool *fused = nullptr);
 
 // ----------------------------------------------------------------- launchers (kernels.hip)
 template <typename T>

@@ -244,13 +244,11 @@ __device__ __forceinline__ void store_agent(T *p, T v) {
 __device__ __forceinline__ float canonical(float v) { return __builtin_canonicalizef(v); }
 __device__ __forceinline__ double canonical(double v) { return __builtin_canonicalize(v); }
 
-// Sweep words (unsigned, device): time-out flag, ticket counter, spares.
-// sync words: error flag, the ticket counter of mode 2, and one counter per XCD stream of
-// mode 3 (128 bytes apart)
-constexpr int kSweepErr = 0, kSweepTicket = 1, kSweepXTicket = 32, kSweepWords = 32 + 32 * kXcds;
-// workgroup size (waves = slices per ticket) of the per-XCD ticket streams: with one
-// counter per XCD, 8-wave workgroups measured best (FP32 application at 128^3: 71.5 us,
-// 77.4 with 16 waves, 73.0 with 4)
+// Sweep words (unsigned, device): the time-out flag and the ticket counter of mode 2.
+constexpr int kSweepErr = 0, kSweepTicket = 1, kSweepWords = 32;
+// workgroup size (waves = slices per ticket) of the per-XCD ticket streams: 8-wave
+// workgroups measured best (FP32 application at 128^3, connect blocks of 8, with one
+// ticket counter per XCD: 71.5 us, 77.4 with 16 waves, 73.0 with 4)
 constexpr int kXTicketWaves = 8;
 // polls of one pending gather before the wave gives up and raises the time-out flag;
 // a poll takes about a microsecond, so this bounds a wait at a few seconds
@@ -271,8 +269,8 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
                                             const V *__restrict__ ve, const V *__restrict__ vi,
                                             const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx,
                                             const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-                                            unsigned *err, T *copy = nullptr, const T *pre_e = nullptr,
-                                            T *pre_w = nullptr) {
+                                            unsigned *err, T *copy = nullptr, const T *ep_a = nullptr,
+                                            T *ep_b = nullptr) {
     using U = typename Bits<T>::U;
     const int4 si = sinfo[slice];
     const LaneRow<BS> lr(si, blk_list);
@@ -282,13 +280,17 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
     RowWord<BS>::unpack(rw[slot], next, slot, ne, mk);
     const T dv = UPPER ? (T)dinv[slot] : T(0);
     const int ex = kHalfV<V> ? (int)sx[slot] : 0;
-    T t = 0;
+    T t = 0, pa = 0, pb = 0;
     if (lr.active) {
         t = x[lr.row];
         if (!UPPER && in_scale) t = (T)((double)t * *in_scale);
         if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
+        if (!UPPER && SWEEP && ep_a) {  // requested with the right-hand side, used at the end
+            pa = ep_a[lr.row];
+            pb = ep_b[lr.row];
+        }
     }
-    const T rhs = UPPER ? t : T(0);  // kept for pre_w (U only)
+    const T rhs = UPPER ? t : T(0);  // kept for ep_b (U only)
     constexpr int CH = 8;
     constexpr int IC = BS < 8 ? BS : 8;
     int pos = si.x, k0 = 0;
@@ -393,11 +395,15 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
     if (lr.active) {
         if (SWEEP) {
             store_agent(y + lr.row, canonical(t));
-            if (copy) copy[lr.row] = t;
-            // optional second output of a U sweep, pre_w = rhs - pre_e * t (the vector pass
-            // that follows a U solve in the split form of BiCGStab)
+            // optional epilogue vectors (single-launch modes): a U sweep also writes
+            // ep_b = rhs - ep_a * t (the vector pass that follows a U solve in the split
+            // forms); an L sweep writes ep_a * ep_b + t instead of t to copy (the split
+            // CG's product D t + L^-1 w, ep_a = D, ep_b = t)
             if constexpr (UPPER) {
-                if (pre_w) pre_w[lr.row] = rhs - pre_e[lr.row] * t;
+                if (copy) copy[lr.row] = t;
+                if (ep_b) ep_b[lr.row] = rhs - ep_a[lr.row] * t;
+            } else {
+                if (copy) copy[lr.row] = ep_a ? pa * pb + t : t;
             }
         } else {
             y[lr.row] = t;
@@ -449,13 +455,13 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sweep(int nslices, const int4
                                                          const DiagT<V> *__restrict__ dinv,
                                                          const signed char *__restrict__ sx, const T *x, T *y,
                                                          const double *__restrict__ in_scale, T *reset,
-                                                         unsigned *err, T *copy, const T *pre_e, T *pre_w,
+                                                         unsigned *err, T *copy, const T *ep_a, T *ep_b,
                                                          const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) return;
     const int nw = (int)gridDim.x * kWavesPerBlock;
     for (int t = (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6); t < nslices; t += nw)
         solve_slice<T, V, BS, UPPER, true>(t, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, err, copy, pre_e, pre_w);
+                                           reset, err, copy, ep_a, ep_b);
 }
 
 // The whole triangle in one launch of 16-wave workgroups, each solving 16 consecutive
@@ -472,7 +478,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
     const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
     const V *__restrict__ ve, const V *__restrict__ vi, const DiagT<V> *__restrict__ dinv,
     const signed char *__restrict__ sx, const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-    unsigned *sync, T *copy, const T *pre_e, T *pre_w, const int *__restrict__ stop_flag) {
+    unsigned *sync, T *copy, const T *ep_a, T *ep_b, const int *__restrict__ stop_flag) {
     if (stop_flag && *stop_flag) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             __hip_atomic_fetch_add(sync + kSweepTicket, gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -490,48 +496,42 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
     const int slice = (int)q * kTicketWaves + (int)(threadIdx.x >> 6);
     if (slice < nslices)
         solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, sync + kSweepErr, copy, pre_e, pre_w);
+                                           reset, sync + kSweepErr, copy, ep_a, ep_b);
 }
 
-// Mode 3: the same with one ticket stream per XCD.  Workgroup b belongs to stream
-// b mod 8 (the XCD the dispatcher hands it to) and draws that stream's tickets in
-// arrival order; stream x solves the x-th eighth of every level (a band of the domain),
-// so most gathers find their rows in the XCD's own L2.  Every stream has the same
-// number of tickets per level (bands padded with empty slices), so ticket j of any
-// stream is in the same level: a wait is only on tickets < j of the other streams, whose
-// workgroups were handed out before this one.
+// Mode 3: one stream of tickets per XCD, and no counter.  Workgroup b belongs to stream
+// x = b mod 8 (the XCD the dispatcher hands it to) and holds that stream's ticket
+// j = b / 8; stream x solves the x-th eighth of every level (a band of the domain), so
+// most gathers find their rows in the XCD's own L2.  Every stream has the same number of
+// tickets per level (bands padded with empty slices), so ticket j of any stream is in the
+// same level, and a wait is only on rows of tickets < j, i.e. of workgroups with smaller
+// numbers.  Progress needs only that each dispatcher hands out its workgroups in
+// increasing order: the smallest unfinished workgroup then has its dispatcher's
+// resources (every earlier workgroup of that dispatcher is finished) and waits on nothing
+// unfinished.  Any other order would end in the bounded poll's SENK_ERR_TIMEOUT, not a
+// hang.  (Tickets drawn from one atomic counter per XCD, which need only the in-order
+// dispatch across XCDs, cost an atomic round trip before each workgroup's first load:
+// FP32 application at 128^3, simple blocks of 8, 46.6 against 40.3 us.)
 // In ticket order (xslice null) the slices themselves are numbered so that wave w of
-// ticket j of stream x solves slice 64 j + 8 x + w: the wave goes from its ticket to its
-// slice descriptor and row words without a lookup.  Otherwise xslice[x][j][w] names the
-// slice (-1: none).  The stop flag is read while the ticket is drawn; a stopped launch
-// still draws one ticket per workgroup, so every launch advances each stream's counter by
-// exactly xwg.
+// ticket j of stream x solves slice 64 j + 8 x + w: the wave goes from its workgroup
+// number to its slice descriptor and row words without a lookup.  Otherwise
+// xslice[x][j][w] names the slice (-1: none).
 template <typename T, typename V, int BS, bool UPPER>
 __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
-    int xwg, unsigned base, const int *__restrict__ xslice, const int4 *__restrict__ sinfo,
-    const int *__restrict__ blk_list, const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next,
-    const int *__restrict__ scol, const V *__restrict__ ve, const V *__restrict__ vi,
-    const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx, const T *x, T *y,
-    const double *__restrict__ in_scale, T *reset, unsigned *sync, T *copy, const T *pre_e, T *pre_w,
-    const int *__restrict__ stop_flag) {
-    const int xs = (int)(blockIdx.x % kXcds);
-    unsigned *ctr = sync + kSweepXTicket + 32 * xs;
-    const int stopped = stop_flag ? *stop_flag : 0;
-    __shared__ unsigned ticket;
-    if (threadIdx.x == 0) ticket = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base;
-    __syncthreads();
-    if (stopped) return;
-    const unsigned q = ticket;
-    if (q >= (unsigned)xwg) {  // launches out of step with the host's count
-        if (threadIdx.x == 0) __hip_atomic_store(sync + kSweepErr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return;
-    }
+    int xwg, const int *__restrict__ xslice, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
+    const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
+    const V *__restrict__ ve, const V *__restrict__ vi, const DiagT<V> *__restrict__ dinv,
+    const signed char *__restrict__ sx, const T *x, T *y, const double *__restrict__ in_scale, T *reset,
+    unsigned *sync, T *copy, const T *ep_a, T *ep_b, const int *__restrict__ stop_flag) {
+    if (stop_flag && *stop_flag) return;
+    const int xs = (int)(blockIdx.x % kXcds), q = (int)(blockIdx.x / kXcds);
+    if (q >= xwg) return;
     const int wv = (int)(threadIdx.x >> 6);
     const int slice = xslice ? xslice[(size_t)xs * kXTicketWaves * xwg + (size_t)q * kXTicketWaves + wv]
-                             : (int)q * (kXcds * kXTicketWaves) + xs * kXTicketWaves + wv;
+                             : q * (kXcds * kXTicketWaves) + xs * kXTicketWaves + wv;
     if (slice >= 0)
         solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, sync + kSweepErr, copy, pre_e, pre_w);
+                                           reset, sync + kSweepErr, copy, ep_a, ep_b);
 }
 
 template <typename T>
@@ -623,7 +623,7 @@ int sweep_grid() {
 template <typename T, typename V, int BS, bool UPPER>
 int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                  const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                 const T *pre_e, T *pre_w, const int *stop) {
+                 const T *ep_a, T *ep_b, const int *stop) {
     using W = typename RowWord<BS>::W;
     int grid = sweep_grid<T, V, BS, UPPER>();
     if (grid <= 0) return SENK_ERR_HIP;
@@ -635,7 +635,7 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
     unsigned *err = L.sync + kSweepErr;
     const double *sc = UPPER ? nullptr : in_scale;
     void *args[] = {&nslices, &sinfo, &blk,  &rw,   &next,  &scol,  &ve,    &vi,  &dinv, &sx,
-                    &x,       &y,     &sc,   &reset, &err, &copy, &pre_e, &pre_w, &stop};
+                    &x,       &y,     &sc,   &reset, &err, &copy, &ep_a, &ep_b, &stop};
     SENK_HIP(hipLaunchCooperativeKernel((const void *)k_sptrsv_sweep<T, V, BS, UPPER>, dim3(grid), dim3(kBlock), args,
                                         0, c->stream));
     return SENK_OK;
@@ -644,13 +644,13 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
 template <typename T, typename V, int BS, bool UPPER>
 int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                   const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                  const T *pre_e, T *pre_w, const int *stop) {
+                  const T *ep_a, T *ep_b, const int *stop) {
     using W = typename RowWord<BS>::W;
     const unsigned grid = (unsigned)((L.nslices + kTicketWaves - 1) / kTicketWaves);
     const unsigned base = L.ticket_base;
     k_sptrsv_ticket<T, V, BS, UPPER><<<grid, 64 * kTicketWaves, 0, c->stream>>>(
         L.nslices, base, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-        UPPER ? nullptr : in_scale, reset, L.sync, copy, pre_e, pre_w, stop);
+        UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop);
     SENK_LAUNCH_CHECK();
     L.ticket_base = base + grid;
     return SENK_OK;
@@ -659,41 +659,39 @@ int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V 
 template <typename T, typename V, int BS, bool UPPER>
 int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                    const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                   const T *pre_e, T *pre_w, const int *stop) {
+                   const T *ep_a, T *ep_b, const int *stop) {
     using W = typename RowWord<BS>::W;
     if ((!L.xslice && !L.tord) || L.xwg <= 0) return SENK_ERR_ARG;
-    const unsigned base = L.xticket_base;
     k_sptrsv_xticket<T, V, BS, UPPER><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(
-        L.xwg, base, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-        UPPER ? nullptr : in_scale, reset, L.sync, copy, pre_e, pre_w, stop);
+        L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
+        UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop);
     SENK_LAUNCH_CHECK();
-    L.xticket_base = base + (unsigned)L.xwg;
     return SENK_OK;
 }
 
 template <typename T, typename V, int BS>
 int solve_sweep(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                 const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                const int *stop, const T *pre_e = nullptr, T *pre_w = nullptr) {
+                const int *stop, const T *ep_a = nullptr, T *ep_b = nullptr) {
     const int mode = L.mode;
     if (mode == kTrsvXcd) {
         if (L.upper)
-            return launch_xticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e,
-                                                  pre_w, stop);
-        return launch_xticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e, pre_w,
+            return launch_xticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a,
+                                                  ep_b, stop);
+        return launch_xticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b,
                                                stop);
     }
     if (mode == kTrsvTicket) {
         if (L.upper)
-            return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e,
-                                                 pre_w, stop);
-        return launch_ticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e, pre_w,
+            return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a,
+                                                 ep_b, stop);
+        return launch_ticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b,
                                               stop);
     }
     if (L.upper)
-        return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e, pre_w,
+        return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b,
                                             stop);
-    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, pre_e, pre_w, stop);
+    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b, stop);
 }
 
 template <typename T>
@@ -955,14 +953,16 @@ int launch_ilu_layout(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule 
     if (!lve || !lvi || !uve || !uvi || !udv) return SENK_ERR_ARG;
     return ilu_layout_apply<T, T>(c, ls, lt, us, ut, lve, lvi, uve, uvi, udv, nullptr, nullptr, in, out, in_scale, stop);
 }
-// One triangle on its own, in = out allowed; in the single-launch modes, with pre_w,
-// also pre_w = in - pre_e * out (*fused tells whether it was written).  The single-launch modes keep their
+// One triangle on its own, in = out allowed.  In the single-launch modes, given both
+// epilogue vectors (*fused tells whether they were used): a U sweep also writes
+// ep_b = in - ep_a * out; an L sweep writes out = ep_a * ep_b + L^-1 in (ep_b read, and
+// then not the same vector as out).  The single-launch modes keep their
 // invariant as long as the triangles of one factorization alternate (L, U, L, ...):
 // each sweep writes its own coherent vector, which the other triangle's last sweep
 // left all-sentinel, and returns the other triangle's vector to the sentinel.
 template <typename T>
 int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
-                      const int *stop, const T *pre_e, T *pre_w, bool *fused) {
+                      const int *stop, const T *ep_a, T *ep_b, bool *fused) {
     if (fused) *fused = false;
     if (self.nslices == 0) return SENK_OK;
     if (self.bsize != s.bsize || self.n != other.n || self.upper == other.upper) return SENK_ERR_ARG;
@@ -974,10 +974,10 @@ int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &oth
     T *mine = (T *)(f64 ? self.mid64 : self.mid32);
     T *theirs = (T *)(f64 ? other.mid64 : other.mid32);
     if (self.mode != kTrsvLevels && mine && theirs && self.sync) {
-        if (!self.upper) pre_w = nullptr;
-        if (fused) *fused = pre_w != nullptr;
+        if (!ep_a || !ep_b) ep_a = ep_b = nullptr;
+        if (fused) *fused = ep_b != nullptr;
         SENK_BS_DISPATCH(self.bsize, (solve_sweep<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, mine, nullptr, theirs,
-                                                           out, stop, pre_e, pre_w)))
+                                                           out, stop, ep_a, ep_b)))
     }
     SENK_BS_DISPATCH(self.bsize, (solve_levels<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, out, nullptr, stop)))
 }
@@ -1000,7 +1000,6 @@ int tri_layout_check(Ctx *c, TriLayout *L) {
     // start the next application from a clean state
     SENK_HIP(hipMemsetAsync(L->sync, 0, kSweepWords * sizeof(unsigned), c->stream));
     L->ticket_base = 0;
-    L->xticket_base = 0;
     if (L->mid64) SENK_TRY(fill_sentinel<double>(c, L->n, (double *)L->mid64));
     if (L->mid32) SENK_TRY(fill_sentinel<float>(c, L->n, (float *)L->mid32));
     SENK_HIP(hipStreamSynchronize(c->stream));
