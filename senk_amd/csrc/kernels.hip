@@ -667,7 +667,8 @@ template int launch_sptrsv_bcsr<float>(Ctx *, const Schedule &, bool, int, int, 
 
 // =============================================================== ILU(0) numeric
 // One wave per schedule block; the block's rows are factored in order.  A row of up
-// to 64*R entries lives in registers (lane q holds entries q, q+64, ...).  For every
+// to 64*R entries lives in registers (lane q holds entries q, q+64, ...; R = 4, 16 or 64
+// by the longest row of the factor, so rows of up to 4096 entries).  For every
 // strictly-lower entry k (ascending): the owner lane divides it by the pivot row's
 // diagonal, the quotient is broadcast, and every later entry of the row subtracts
 // quotient * (pivot row entry of the same column), found by binary search in the pivot
@@ -675,8 +676,7 @@ template int launch_sptrsv_bcsr<float>(Ctx *, const Schedule &, bool, int, int, 
 // the reference's IKJ order (senk_matrix.hpp:418-448).  Pivot rows are final: earlier
 // levels were written by earlier launches, earlier rows of the same block by this
 // wave followed by a workgroup-scope fence.
-constexpr int kIluRegs = 4;
-template <typename T>
+template <typename T, int kIluRegs>
 __global__ __launch_bounds__(kBlock) void k_ilu0(int nblk, int blk0, const int *__restrict__ blk_list, int BS,
                                                  const int *__restrict__ rptr, const int *__restrict__ cind,
                                                  const int *__restrict__ dpos, T *val, int *err) {
@@ -751,18 +751,46 @@ __global__ __launch_bounds__(kBlock) void k_ilu0(int nblk, int blk0, const int *
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
 }
-template <typename T>
-int launch_ilu0(Ctx *c, const Schedule &s, int N, const int *rptr, const int *cind, const int *dpos, T *val,
-                int *derr) {
-    (void)N;
+__global__ __launch_bounds__(kBlock) void k_max_row_len(int N, const int *__restrict__ rptr, int *out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    int len = i < N ? rptr[i + 1] - rptr[i] : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, 64));
+    if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(out, len);
+}
+template <typename T, int R>
+static int ilu0_levels(Ctx *c, const Schedule &s, const int *rptr, const int *cind, const int *dpos, T *val,
+                       int *derr) {
     for (int l = 0; l < s.nlev; l++) {
         const int b0 = s.lev_ptr[l], nb = s.lev_ptr[l + 1] - b0;
         if (nb <= 0) continue;
         const int grid = (nb + kWavesPerBlock - 1) / kWavesPerBlock;
-        k_ilu0<T><<<grid, kBlock, 0, c->stream>>>(nb, b0, s.dblk, s.bsize, rptr, cind, dpos, val, derr);
+        k_ilu0<T, R><<<grid, kBlock, 0, c->stream>>>(nb, b0, s.dblk, s.bsize, rptr, cind, dpos, val, derr);
         SENK_LAUNCH_CHECK();
     }
     return SENK_OK;
+}
+template <typename T>
+int launch_ilu0(Ctx *c, const Schedule &s, int N, const int *rptr, const int *cind, const int *dpos, T *val,
+                int *derr) {
+    if (N <= 0) return SENK_OK;
+    // the longest row picks the register count (a row beyond 4096 entries: err = 2)
+    int *dmax = nullptr, maxlen = 0;
+    SENK_HIP(hipMalloc(&dmax, sizeof(int)));
+    int st = SENK_OK;
+    if (hipMemsetAsync(dmax, 0, sizeof(int), c->stream) != hipSuccess) st = SENK_ERR_HIP;
+    if (!st) {
+        k_max_row_len<<<(N + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(N, rptr, dmax);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&maxlen, dmax, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            st = SENK_ERR_HIP;
+    }
+    (void)hipFree(dmax);
+    if (st) return st;
+    if (maxlen <= 64 * 4) return ilu0_levels<T, 4>(c, s, rptr, cind, dpos, val, derr);
+    if (maxlen <= 64 * 16) return ilu0_levels<T, 16>(c, s, rptr, cind, dpos, val, derr);
+    return ilu0_levels<T, 64>(c, s, rptr, cind, dpos, val, derr);
 }
 template int launch_ilu0<double>(Ctx *, const Schedule &, int, const int *, const int *, const int *, double *,
                                  int *);

@@ -262,24 +262,45 @@ constexpr int kSweepSpinLimit = 1 << 22;
 // column base+c subtract it.  SWEEP: gathers wait on the sentinel, the row is stored
 // device-coherent, and reset[row] (when given) receives the sentinel once x[row] has
 // been read.
-template <typename T, typename V, int BS, bool UPPER, bool SWEEP>
-__device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
-                                            const typename RowWord<BS>::W *__restrict__ rw,
-                                            const int *__restrict__ next, const int *__restrict__ scol,
-                                            const V *__restrict__ ve, const V *__restrict__ vi,
-                                            const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx,
-                                            const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-                                            unsigned *err, T *copy = nullptr, const T *ep_a = nullptr,
-                                            T *ep_b = nullptr) {
-    using U = typename Bits<T>::U;
-    const int4 si = sinfo[slice];
-    const LaneRow<BS> lr(si, blk_list);
-    const size_t slot = (size_t)slice * 64 + (threadIdx.x & 63);
+// What a wave reads about its slice before it can address the slice's entries: the
+// descriptor and, per lane, the row word (external count, in-block mask), the inverted
+// diagonal (U) and the FP16 row exponent.
+template <typename V, int BS>
+struct SliceHead {
+    int4 si;
     int ne;
     MaskT<BS> mk;
-    RowWord<BS>::unpack(rw[slot], next, slot, ne, mk);
-    const T dv = UPPER ? (T)dinv[slot] : T(0);
-    const int ex = kHalfV<V> ? (int)sx[slot] : 0;
+    DiagT<V> dv;
+    int ex;
+};
+template <typename V, int BS, bool UPPER>
+__device__ __forceinline__ SliceHead<V, BS> slice_head(int slice, const int4 *__restrict__ sinfo,
+                                                       const typename RowWord<BS>::W *__restrict__ rw,
+                                                       const int *__restrict__ next,
+                                                       const DiagT<V> *__restrict__ dinv,
+                                                       const signed char *__restrict__ sx) {
+    SliceHead<V, BS> h;
+    h.si = sinfo[slice];
+    const size_t slot = (size_t)slice * 64 + (threadIdx.x & 63);
+    RowWord<BS>::unpack(rw[slot], next, slot, h.ne, h.mk);
+    h.dv = UPPER ? dinv[slot] : DiagT<V>(0);
+    h.ex = kHalfV<V> ? (int)sx[slot] : 0;
+    return h;
+}
+
+template <typename T, typename V, int BS, bool UPPER, bool SWEEP>
+__device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, const int *__restrict__ blk_list,
+                                                 const int *__restrict__ scol, const V *__restrict__ ve,
+                                                 const V *__restrict__ vi, const T *x, T *y,
+                                                 const double *__restrict__ in_scale, T *reset, unsigned *err,
+                                                 T *copy, const T *ep_a, T *ep_b) {
+    using U = typename Bits<T>::U;
+    const int4 si = h.si;
+    const LaneRow<BS> lr(si, blk_list);
+    const int ne = h.ne;
+    const MaskT<BS> mk = h.mk;
+    const T dv = UPPER ? (T)h.dv : T(0);
+    const int ex = h.ex;
     T t = 0, pa = 0, pb = 0;
     if (lr.active) {
         t = x[lr.row];
@@ -411,6 +432,19 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
     }
 }
 
+template <typename T, typename V, int BS, bool UPPER, bool SWEEP>
+__device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
+                                            const typename RowWord<BS>::W *__restrict__ rw,
+                                            const int *__restrict__ next, const int *__restrict__ scol,
+                                            const V *__restrict__ ve, const V *__restrict__ vi,
+                                            const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx,
+                                            const T *x, T *y, const double *__restrict__ in_scale, T *reset,
+                                            unsigned *err, T *copy = nullptr, const T *ep_a = nullptr,
+                                            T *ep_b = nullptr) {
+    solve_slice_body<T, V, BS, UPPER, SWEEP>(slice_head<V, BS, UPPER>(slice, sinfo, rw, next, dinv, sx), blk_list,
+                                             scol, ve, vi, x, y, in_scale, reset, err, copy, ep_a, ep_b);
+}
+
 // One launch = one schedule level (slices slice0 .. slice0 + nsl).
 template <typename T, typename V, int BS, bool UPPER>
 __global__ __launch_bounds__(kBlock) void k_sptrsv_sell(int slice0, int nsl, int tord, const int4 *__restrict__ sinfo,
@@ -523,15 +557,18 @@ __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     const V *__restrict__ ve, const V *__restrict__ vi, const DiagT<V> *__restrict__ dinv,
     const signed char *__restrict__ sx, const T *x, T *y, const double *__restrict__ in_scale, T *reset,
     unsigned *sync, T *copy, const T *ep_a, T *ep_b, const int *__restrict__ stop_flag) {
-    if (stop_flag && *stop_flag) return;
     const int xs = (int)(blockIdx.x % kXcds), q = (int)(blockIdx.x / kXcds);
     if (q >= xwg) return;
     const int wv = (int)(threadIdx.x >> 6);
     const int slice = xslice ? xslice[(size_t)xs * kXTicketWaves * xwg + (size_t)q * kXTicketWaves + wv]
                              : q * (kXcds * kXTicketWaves) + xs * kXTicketWaves + wv;
-    if (slice >= 0)
-        solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, sync + kSweepErr, copy, ep_a, ep_b);
+    if (slice < 0) return;
+    // the stop flag is read together with the slice's head (both addresses are known)
+    const int stopped = stop_flag ? *stop_flag : 0;
+    const SliceHead<V, BS> h = slice_head<V, BS, UPPER>(slice, sinfo, rw, next, dinv, sx);
+    if (stopped) return;
+    solve_slice_body<T, V, BS, UPPER, true>(h, blk_list, scol, ve, vi, x, y, in_scale, reset, sync + kSweepErr, copy,
+                                            ep_a, ep_b);
 }
 
 template <typename T>

@@ -311,6 +311,41 @@ def test_ilu0_zero_pivot_reported(ctx):
     assert st == 3  # SENK_ERR_ZERO_PIVOT
 
 
+@pytest.mark.parametrize("width", [300, 1500, 3000])
+def test_ilu0_device_long_rows(ctx, width):
+    """Rows longer than 256 entries (ILU(p) fill of wide rows): the factorization keeps a
+    row in registers, 64 lanes x R entries, with R chosen from the longest row (4, 16,
+    64).  A banded matrix whose last rows are dense over their first `width` columns, in
+    FP64 and FP32, bit for bit against the oracle's Ilu0."""
+    rng = np.random.default_rng(width)
+    N = width + 40
+    rows = []
+    for i in range(N):
+        cols = set(range(max(0, i - 2), min(N, i + 3)))
+        if i >= N - 4:
+            cols |= set(range(0, width))
+        rows.append(sorted(cols))
+    rptr = np.zeros(N + 1, np.int32)
+    rptr[1:] = np.cumsum([len(r) for r in rows])
+    cind = np.array([c for r in rows for c in r], np.int32)
+    val = rng.uniform(-1, 1, len(cind))
+    for i in range(N):  # diagonally dominant
+        s, e = rptr[i], rptr[i + 1]
+        d = s + rows[i].index(i)
+        val[d] = np.sum(np.abs(val[s:e])) + 1.0
+    assert np.max(np.diff(rptr)) >= width
+    fo = oracle_ilu0(val, cind, rptr)
+    dv, dc, dr = ctx.array(val), ctx.array(cind), ctx.array(rptr)
+    S.call("senk_ilu0_f64", ctx.h, dv.p, dc.p, dr.p, N, None, 0, 1, O.P(rptr), O.P(cind))
+    assert np.array_equal(dv.download(), fo)
+    v32 = val.astype(np.float32)
+    fo32 = v32.copy()
+    assert O.orc().or_ilu0_f(O.P(fo32), O.P(cind), O.P(rptr), N) == 0
+    d32 = ctx.array(v32)
+    S.call("senk_ilu0_f32", ctx.h, d32.p, dc.p, dr.p, N, None, 0, 1, O.P(rptr), O.P(cind))
+    assert np.array_equal(d32.download(), fo32)
+
+
 def test_blas1(ctx):
     rng = np.random.default_rng(12)
     N = 100003
