@@ -71,6 +71,72 @@ __device__ __forceinline__ int lanes_below(unsigned long long b) {
     return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
 }
 
+// Row c of every block of BS consecutive lanes, broadcast to the block's lanes (the
+// source row of an in-block step), without the LDS crossbar of a general shuffle: DPP
+// inside rows of 16 lanes (quad_perm for blocks of 2 and 4, row_newbcast for 8 and 16)
+// and v_readlane for 32 and 64.  c is a compile-time constant once the in-block loop is
+// unrolled, so each switch folds to one instruction.  Every lane of the wave must be
+// active (the in-block loop runs in uniform control flow).
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int row_bcast16(int v, int n) {
+    switch (n) {
+        case 0: return dpp_mov<0x150>(v);
+        case 1: return dpp_mov<0x151>(v);
+        case 2: return dpp_mov<0x152>(v);
+        case 3: return dpp_mov<0x153>(v);
+        case 4: return dpp_mov<0x154>(v);
+        case 5: return dpp_mov<0x155>(v);
+        case 6: return dpp_mov<0x156>(v);
+        case 7: return dpp_mov<0x157>(v);
+        case 8: return dpp_mov<0x158>(v);
+        case 9: return dpp_mov<0x159>(v);
+        case 10: return dpp_mov<0x15a>(v);
+        case 11: return dpp_mov<0x15b>(v);
+        case 12: return dpp_mov<0x15c>(v);
+        case 13: return dpp_mov<0x15d>(v);
+        case 14: return dpp_mov<0x15e>(v);
+        default: return dpp_mov<0x15f>(v);
+    }
+}
+template <int BS>
+__device__ __forceinline__ int block_bcast32(int v, int c) {
+    const int lane = threadIdx.x & 63;
+    if constexpr (BS == 2) {  // quad_perm [c, c, 2 + c, 2 + c]
+        return c == 0 ? dpp_mov<0xa0>(v) : dpp_mov<0xf5>(v);
+    } else if constexpr (BS == 4) {  // quad_perm [c, c, c, c]
+        switch (c) {
+            case 0: return dpp_mov<0x00>(v);
+            case 1: return dpp_mov<0x55>(v);
+            case 2: return dpp_mov<0xaa>(v);
+            default: return dpp_mov<0xff>(v);
+        }
+    } else if constexpr (BS == 8) {
+        const int a = row_bcast16(v, c), b = row_bcast16(v, c + 8);
+        return (lane & 8) ? b : a;
+    } else if constexpr (BS == 16) {
+        return row_bcast16(v, c);
+    } else if constexpr (BS == 32) {
+        const int a = __builtin_amdgcn_readlane(v, c), b = __builtin_amdgcn_readlane(v, 32 + c);
+        return lane < 32 ? a : b;
+    } else {
+        return __builtin_amdgcn_readlane(v, c);
+    }
+}
+template <int BS, typename T>
+__device__ __forceinline__ T block_bcast(T v, int c) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, block_bcast32<BS>(__builtin_bit_cast(int, v), c));
+    } else {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)block_bcast32<BS>((int)(unsigned)u, c);
+        const unsigned hi = (unsigned)block_bcast32<BS>((int)(unsigned)(u >> 32), c);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    }
+}
+
 // Row of this lane in a slice whose descriptor is si = {external offset, in-block
 // offset, first block entry, blocks}: blocks blk_list[si.z ..] (blk_list null:
 // consecutive block numbers from si.z).
@@ -385,7 +451,6 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
         more = __ballot(k0 < ne) != 0;
         if (more) load_chunk();
     }
-    const int lbase = (threadIdx.x & 63) - lr.r;
 #pragma unroll
     for (int i0 = 0; i0 < BS; i0 += IC) {
         T iv[IC];
@@ -408,7 +473,7 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
             const int c = UPPER ? BS - 1 - (i0 + q) : i0 + q;
             if (UPPER && lr.r == c) t = t * dv;
             if (BS > 1) {
-                const T yc = __shfl(t, lbase + c, 64);
+                const T yc = block_bcast<BS>(t, c);
                 if (onq[q]) t -= iv[q] * yc;
             }
         }
