@@ -140,9 +140,24 @@ int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr
 // With both epilogue vectors (single-launch modes only; *fused says whether they were
 // used) a U sweep also writes ep_b = in - ep_a * out, and an L sweep writes
 // out = ep_a * ep_b + L^-1 in (ep_b read; not the same vector as out).
+// A U solve may first update its right-hand side (pre.u == in, non-null): with
+// a = pre.ab[0], b = pre.ab[1], per row yd += a u, r = r - a q, u = r + b u, and the
+// solve uses the new u (the split CG's vector step folded into the next product).  The
+// single-launch modes do it in the sweep as each row's right-hand side is read; the
+// level-by-level mode in a pass before the solve.  Skipped, like the solve, when *stop is
+// set.
+template <typename T>
+struct SweepPre {
+    T *u = nullptr;
+    T *r = nullptr;
+    const T *q = nullptr;
+    T *yd = nullptr;
+    const T *ab = nullptr;
+};
 template <typename T>
 int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
-                      const int *stop, const T *ep_a = nullptr, T *ep_b = nullptr, bool *fused = nullptr);
+                      const int *stop, const T *ep_a = nullptr, T *ep_b = nullptr, bool *fused = nullptr,
+                      const SweepPre<T> &pre = SweepPre<T>{});
 
 // ----------------------------------------------------------------- launchers (kernels.hip)
 template <typename T>

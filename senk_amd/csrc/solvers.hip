@@ -1513,11 +1513,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int N, const T *__restrict__ z,
 // yd = D y instead of p and y: B p = L^-1 A U^-1 u (the split product: one U and one L
 // solve, no product with A), D z = r, so u <- r + beta u, yd <- yd + alpha u, and at the
 // end x = U^-1 yd.  The residual it stops on is L^-1 (b - A x), relative to ||L^-1 b||.
-// Per step: the U sweep (with w = u - E t in its epilogue), the L sweep (with q = D t +
-// L^-1 w in its epilogue), one pass for all seven dot products the step needs, and one
-// pass for the updates, whose new (r, r/d) and (r, r) come from the expansion of
-// r - alpha q: 88 MB of vector traffic per step at 128^3 (FP32) against 124 MB for the
-// textbook order of three passes (q and its dot; alpha, yd, r and their dots; beta, u).
+// Per step: the U sweep (taking the previous step's yd, r, u update as it reads u, and
+// with w = u - E t in its epilogue), the L sweep (with q = D t + L^-1 w in its epilogue),
+// one pass for all seven dot products the step needs and a one-workgroup kernel for the
+// scalars; the new (r, r/d) and (r, r) come from the expansion of r - alpha q.  No vector
+// pass of its own for the update: 128^3 nested solve 20.8 ms against 21.1 ms with one
+// (a pass that also summed the partials in every workgroup), the U sweep taking 27.6
+// instead of 22.4 us.
 namespace {
 // partials of (r, r) and (r, r/d)
 __global__ __launch_bounds__(kBlock) void k_cgs_start(int N, const float *__restrict__ r,
@@ -1571,16 +1573,13 @@ __global__ __launch_bounds__(kBlock) void k_cgs_q(int N, const float *__restrict
     }
 }
 // One reading pass for the whole step: the partials of (q, u/d), (r, r/d), (q, r/d),
-// (q, q/d), (r, r), (q, r), (q, q)
+// (q, q/d), (r, r), (q, r), (q, q).
 constexpr int kCgsDots = 7;
-// state[4] := state[0] for the step kernel that follows: that kernel sets state[0] itself,
-// so its workgroups must not read it to learn whether an earlier step stopped.
+constexpr int kCgsMaxB = 2 * kBlock;  // workgroups of k_cgs_dots: two partials per thread in k_cgs_scalars
 __global__ __launch_bounds__(kBlock) void k_cgs_dots(int N, const float *__restrict__ q,
                                                      const float *__restrict__ u, const float *__restrict__ r,
-                                                     const float *__restrict__ d, double *Pout, int *state) {
-    const int stopped = state[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0) state[4] = stopped;
-    if (stopped) return;
+                                                     const float *__restrict__ d, double *P, const int *state) {
+    if (state[0]) return;
     double a[kCgsDots] = {};
     SENK_FOR4(i0, N) {
         float qv[4], uv[4], rv[4], dv[4];
@@ -1601,43 +1600,50 @@ __global__ __launch_bounds__(kBlock) void k_cgs_dots(int N, const float *__restr
             a[6] += qk * qk;
         }
     }
-    store_partials<kCgsDots>(a, Pout);
+    store_partials<kCgsDots>(a, P);
 }
-// The rest of the step from those sums: alpha = (r, r/d) / (q, u/d); the new residual's
-// (r, r/d) and (r, r) by expansion, r' = r - alpha q giving (r,r/d) - 2 alpha (q,r/d) +
-// alpha^2 (q,q/d); the stop test on ||r'|| (tolerance formed on the device); then
-// yd += alpha u, r = r', u = r' + beta u with beta = (r', r'/d) / (r, r/d).
-__global__ __launch_bounds__(kBlock) void k_cgs_step(int N, float *u, const float *__restrict__ q, float *yd,
-                                                     float *r, const double *Pin, int B, const double *dtol, int iter,
-                                                     int *state, double *hist, int hcap, unsigned long long *mark,
-                                                     unsigned seq) {
-    if (state[4]) {  // stopped at an earlier step (state[0] is written by this kernel)
-        if (mark && blockIdx.x == 0 && threadIdx.x == 0) write_mark(mark, seq, 1);
+// The step's scalars from those sums (one workgroup): alpha = (r, r/d) / (q, u/d); the
+// new residual's (r, r/d) and (r, r) by expansion, r' = r - alpha q giving (r,r/d) -
+// 2 alpha (q,r/d) + alpha^2 (q,q/d); the stop test on ||r'|| (tolerance formed on the
+// device); beta = (r', r'/d) / (r, r/d).  ab = {alpha, beta} for the vector step
+// yd += alpha u, r = r', u = r' + beta u, which the next U sweep does as it reads u
+// (SweepPre) and k_cgs_last does for the step that ends the loop.  The partials are read
+// before the stop flag, all at once.  (Formed instead by the last of k_cgs_dots'
+// workgroups, counted with an atomic: 128^3 nested solve 21.8 ms with the partials
+// passed through the device-coherent level, 26.3 ms with device-scope fences, against
+// 20.8 ms for this kernel.)
+__global__ __launch_bounds__(kBlock) void k_cgs_scalars(const double *__restrict__ P, int B,
+                                                        const double *__restrict__ dtol, int iter, int *state,
+                                                        float *ab, double *hist, int hcap,
+                                                        unsigned long long *mark, unsigned seq) {
+    double pv[2][kCgsDots];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int b = threadIdx.x + j * kBlock;
+#pragma unroll
+        for (int k = 0; k < kCgsDots; k++) pv[j][k] = b < B ? P[(size_t)k * kPartLd + b] : 0.0;
+    }
+    if (state[0]) {  // stopped at an earlier step
+        if (mark && threadIdx.x == 0) write_mark(mark, seq, 1);
         return;
     }
-    // the seven sums at once: one pass over the partials and one workgroup reduction
+    // the sums in slot order, as one pass over the partials forms them
+    __shared__ double sh[kCgsDots][kWavesPerBlock];
+    const int wv = threadIdx.x >> 6;
     double s[kCgsDots];
-    {
-        __shared__ double sh[kCgsDots][kWavesPerBlock];
 #pragma unroll
-        for (int k = 0; k < kCgsDots; k++) s[k] = 0;
-        for (int b = threadIdx.x; b < B; b += kBlock)
+    for (int k = 0; k < kCgsDots; k++) {
+        const double v = wave_sum(pv[0][k] + pv[1][k]);
+        if ((threadIdx.x & 63) == 0) sh[k][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
 #pragma unroll
-            for (int k = 0; k < kCgsDots; k++) s[k] += Pin[(size_t)k * kPartLd + b];
-        const int wv = threadIdx.x >> 6;
+    for (int k = 0; k < kCgsDots; k++) {
+        double t = 0;
 #pragma unroll
-        for (int k = 0; k < kCgsDots; k++) {
-            const double v = wave_sum(s[k]);
-            if ((threadIdx.x & 63) == 0) sh[k][wv] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kCgsDots; k++) {
-            double t = 0;
-#pragma unroll
-            for (int i = 0; i < kWavesPerBlock; i++) t += sh[k][i];
-            s[k] = t;
-        }
+        for (int i = 0; i < kWavesPerBlock; i++) t += sh[k][i];
+        s[k] = t;
     }
     const double sigma = s[0], rho = s[1];
     const double alpha = rho / sigma;
@@ -1649,34 +1655,31 @@ __global__ __launch_bounds__(kBlock) void k_cgs_step(int N, float *u, const floa
     // definite: a pivot d < 0), or (r', r'/d) < 0 short of convergence
     const bool fin = !isfinite(sigma) || !isfinite(rho_new) || !isfinite(rr_new) || !(sigma > 0) ||
                      (rho_new < 0 && !conv);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (hist && iter < hcap) hist[iter] = nr / dtol[1];
-        state[1] = iter + 1;
-        if (fin) {
-            state[2] = 1;
-            state[0] = 1;
-        } else if (conv) {
-            state[0] = 1;
-        }
-        if (mark) write_mark(mark, seq, fin || conv);
+    if (hist && iter < hcap) hist[iter] = nr / dtol[1];
+    state[1] = iter + 1;
+    if (fin) {
+        state[2] = 1;
+        state[0] = 1;
+    } else if (conv) {
+        state[0] = 1;
     }
-    if (fin) return;
-    const float a = (float)alpha, be = (float)(rho_new / rho);
+    ab[0] = (float)alpha;
+    ab[1] = (float)(rho_new / rho);
+    if (mark) write_mark(mark, seq, fin || conv);
+}
+// The vector step of the loop's last step (stopped by convergence or the step limit; a
+// breakdown takes no step): yd += alpha u.
+__global__ __launch_bounds__(kBlock) void k_cgs_last(int N, const float *__restrict__ u, float *yd,
+                                                     const float *__restrict__ ab, const int *__restrict__ state) {
+    if (state[2] || state[1] == 0) return;
+    const float a = ab[0];
     SENK_FOR4(i0, N) {
-        float uv[4], yv[4], qv[4], rv[4];
+        float uv[4], yv[4];
         get4(u, i0, N, uv);
         get4(yd, i0, N, yv);
-        get4(q, i0, N, qv);
-        get4(r, i0, N, rv);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            yv[k] = a * uv[k] + yv[k];
-            rv[k] = (-a) * qv[k] + rv[k];
-            uv[k] = rv[k] + be * uv[k];
-        }
+        for (int k = 0; k < 4; k++) yv[k] = a * uv[k] + yv[k];
         put4(yd, i0, N, yv);
-        put4(r, i0, N, rv);
-        if (!conv) put4(u, i0, N, uv);
     }
 }
 }  // namespace
@@ -1697,14 +1700,14 @@ static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b,
     int *state = cv.take<int>(8);
     double *sc = cv.take<double>(16);
     double *dtol = sc + 8;  // {tolerance, ||L^-1 b||}, formed on the device
+    float *ab = reinterpret_cast<float *>(sc + 12);  // the step's {alpha, beta}
     Poller poll;
     SENK_TRY(poll.init(c, state));
     SolveTimer tm;
     SENK_TRY(tm.start(c));
-    // at most 512 workgroups per reducing pass: each pass starts by summing every
-    // workgroup's partials (128^3 nested solve: 31.0 -> 30.8 ms, the same iterations; 256
-    // was slower)
-    const int B = std::min(red_grid(N), 512);
+    // at most 512 workgroups per reducing pass (128^3 nested solve: 31.0 -> 30.8 ms, the
+    // same iterations; 256 was slower)
+    const int B = std::min(red_grid(N), kCgsMaxB);
     k_state_reset<<<1, 64, 0, c->stream>>>(state);
     SENK_LAUNCH_CHECK();
     // r = L^-1 b; u = r; yd = 0; rho = (r, r/d)
@@ -1719,9 +1722,12 @@ static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b,
     bool stop = false;
     for (int i = 0; i < p.max_iter && !stop; i++) {
         // q = L^-1 A U^-1 u  (t = U^-1 u; w = u - E t; q = D t + L^-1 w), each vector pass
-        // formed in a sweep's epilogue where the launch mode allows
+        // formed in a sweep's epilogue where the launch mode allows; from the second step on
+        // the U solve first takes the previous step's yd, r, u update
         bool fused = false;
-        SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, u, t, state, M.dilu_e, w, &fused));
+        SweepPre<float> pre;
+        if (i > 0) pre = SweepPre<float>{u, r, q, yd, ab};
+        SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, u, t, state, M.dilu_e, w, &fused, pre));
         if (!fused) {
             k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, u, M.dilu_e, t, w, state);
             SENK_LAUNCH_CHECK();
@@ -1735,11 +1741,13 @@ static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b,
         SENK_LAUNCH_CHECK();
         unsigned seq = 0;
         unsigned long long *mark = poll.dev_mark(&seq);
-        k_cgs_step<<<B, kBlock, 0, c->stream>>>(N, u, q, yd, r, red_slot(c, kSlotA), B, dtol, i, state, dhist, hcap,
-                                                mark, seq);
+        k_cgs_scalars<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotA), B, dtol, i, state, ab, dhist, hcap, mark,
+                                                   seq);
         SENK_LAUNCH_CHECK();
         SENK_TRY(poll.poll(kLag, &stop));
     }
+    k_cgs_last<<<ew_grid(N), kBlock, 0, c->stream>>>(N, u, yd, ab, state);
+    SENK_LAUNCH_CHECK();
     // x = U^-1 yd (after the stop flag is set: no flag for this sweep)
     SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, yd, x, nullptr));
     int hstate[4];

@@ -359,7 +359,8 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
                                                  const int *__restrict__ scol, const V *__restrict__ ve,
                                                  const V *__restrict__ vi, const T *x, T *y,
                                                  const double *__restrict__ in_scale, T *reset, unsigned *err,
-                                                 T *copy, const T *ep_a, T *ep_b) {
+                                                 T *copy, const T *ep_a, T *ep_b,
+                                                 const SweepPre<T> &pre = SweepPre<T>{}) {
     using U = typename Bits<T>::U;
     const int4 si = h.si;
     const LaneRow<BS> lr(si, blk_list);
@@ -370,6 +371,17 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
     T t = 0, pa = 0, pb = 0;
     if (lr.active) {
         t = x[lr.row];
+        if constexpr (UPPER && SWEEP) {
+            if (pre.u) {  // the right-hand side's update (x == pre.u), all loads first
+                const T a = pre.ab[0], be = pre.ab[1];
+                const T r0 = pre.r[lr.row], qv = pre.q[lr.row], yv = pre.yd[lr.row];
+                const T rv = (-a) * qv + r0;
+                pre.yd[lr.row] = a * t + yv;
+                pre.r[lr.row] = rv;
+                t = rv + be * t;
+                pre.u[lr.row] = t;
+            }
+        }
         if (!UPPER && in_scale) t = (T)((double)t * *in_scale);
         if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
         if (!UPPER && SWEEP && ep_a) {  // requested with the right-hand side, used at the end
@@ -505,9 +517,9 @@ __device__ __forceinline__ void solve_slice(int slice, const int4 *__restrict__ 
                                             const DiagT<V> *__restrict__ dinv, const signed char *__restrict__ sx,
                                             const T *x, T *y, const double *__restrict__ in_scale, T *reset,
                                             unsigned *err, T *copy = nullptr, const T *ep_a = nullptr,
-                                            T *ep_b = nullptr) {
+                                            T *ep_b = nullptr, const SweepPre<T> &pre = SweepPre<T>{}) {
     solve_slice_body<T, V, BS, UPPER, SWEEP>(slice_head<V, BS, UPPER>(slice, sinfo, rw, next, dinv, sx), blk_list,
-                                             scol, ve, vi, x, y, in_scale, reset, err, copy, ep_a, ep_b);
+                                             scol, ve, vi, x, y, in_scale, reset, err, copy, ep_a, ep_b, pre);
 }
 
 // One launch = one schedule level (slices slice0 .. slice0 + nsl).
@@ -555,12 +567,12 @@ __global__ __launch_bounds__(kBlock) void k_sptrsv_sweep(int nslices, const int4
                                                          const signed char *__restrict__ sx, const T *x, T *y,
                                                          const double *__restrict__ in_scale, T *reset,
                                                          unsigned *err, T *copy, const T *ep_a, T *ep_b,
-                                                         const int *__restrict__ stop_flag) {
+                                                         const int *__restrict__ stop_flag, SweepPre<T> pre) {
     if (stop_flag && *stop_flag) return;
     const int nw = (int)gridDim.x * kWavesPerBlock;
     for (int t = (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6); t < nslices; t += nw)
         solve_slice<T, V, BS, UPPER, true>(t, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, err, copy, ep_a, ep_b);
+                                           reset, err, copy, ep_a, ep_b, pre);
 }
 
 // The whole triangle in one launch of 16-wave workgroups, each solving 16 consecutive
@@ -577,7 +589,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
     const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
     const V *__restrict__ ve, const V *__restrict__ vi, const DiagT<V> *__restrict__ dinv,
     const signed char *__restrict__ sx, const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-    unsigned *sync, T *copy, const T *ep_a, T *ep_b, const int *__restrict__ stop_flag) {
+    unsigned *sync, T *copy, const T *ep_a, T *ep_b, const int *__restrict__ stop_flag, SweepPre<T> pre) {
     if (stop_flag && *stop_flag) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             __hip_atomic_fetch_add(sync + kSweepTicket, gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -595,7 +607,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
     const int slice = (int)q * kTicketWaves + (int)(threadIdx.x >> 6);
     if (slice < nslices)
         solve_slice<T, V, BS, UPPER, true>(slice, sinfo, blk_list, rw, next, scol, ve, vi, dinv, sx, x, y, in_scale,
-                                           reset, sync + kSweepErr, copy, ep_a, ep_b);
+                                           reset, sync + kSweepErr, copy, ep_a, ep_b, pre);
 }
 
 // Mode 3: one stream of tickets per XCD, and no counter.  Workgroup b belongs to stream
@@ -621,7 +633,7 @@ __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
     const V *__restrict__ ve, const V *__restrict__ vi, const DiagT<V> *__restrict__ dinv,
     const signed char *__restrict__ sx, const T *x, T *y, const double *__restrict__ in_scale, T *reset,
-    unsigned *sync, T *copy, const T *ep_a, T *ep_b, const int *__restrict__ stop_flag) {
+    unsigned *sync, T *copy, const T *ep_a, T *ep_b, const int *__restrict__ stop_flag, SweepPre<T> pre) {
     const int xs = (int)(blockIdx.x % kXcds), q = (int)(blockIdx.x / kXcds);
     if (q >= xwg) return;
     const int wv = (int)(threadIdx.x >> 6);
@@ -633,7 +645,22 @@ __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     const SliceHead<V, BS> h = slice_head<V, BS, UPPER>(slice, sinfo, rw, next, dinv, sx);
     if (stopped) return;
     solve_slice_body<T, V, BS, UPPER, true>(h, blk_list, scol, ve, vi, x, y, in_scale, reset, sync + kSweepErr, copy,
-                                            ep_a, ep_b);
+                                            ep_a, ep_b, pre);
+}
+
+// The update of a U solve's right-hand side (SweepPre) as a pass of its own, for the
+// level-by-level mode; the same arithmetic as the sweeps' prologue.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep_pre(int n, SweepPre<T> pre, const int *__restrict__ stop) {
+    if (stop && *stop) return;
+    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (i >= n) return;
+    const T a = pre.ab[0], be = pre.ab[1];
+    const T u = pre.u[i], r0 = pre.r[i], qv = pre.q[i], yv = pre.yd[i];
+    const T rv = (-a) * qv + r0;
+    pre.yd[i] = a * u + yv;
+    pre.r[i] = rv;
+    pre.u[i] = rv + be * u;
 }
 
 template <typename T>
@@ -725,7 +752,7 @@ int sweep_grid() {
 template <typename T, typename V, int BS, bool UPPER>
 int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                  const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                 const T *ep_a, T *ep_b, const int *stop) {
+                 const T *ep_a, T *ep_b, const int *stop, SweepPre<T> pre) {
     using W = typename RowWord<BS>::W;
     int grid = sweep_grid<T, V, BS, UPPER>();
     if (grid <= 0) return SENK_ERR_HIP;
@@ -737,7 +764,7 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
     unsigned *err = L.sync + kSweepErr;
     const double *sc = UPPER ? nullptr : in_scale;
     void *args[] = {&nslices, &sinfo, &blk,  &rw,   &next,  &scol,  &ve,    &vi,  &dinv, &sx,
-                    &x,       &y,     &sc,   &reset, &err, &copy, &ep_a, &ep_b, &stop};
+                    &x,       &y,     &sc,   &reset, &err, &copy, &ep_a, &ep_b, &stop, &pre};
     SENK_HIP(hipLaunchCooperativeKernel((const void *)k_sptrsv_sweep<T, V, BS, UPPER>, dim3(grid), dim3(kBlock), args,
                                         0, c->stream));
     return SENK_OK;
@@ -746,13 +773,13 @@ int launch_sweep(Ctx *c, const Schedule &s, const TriLayout &L, const V *ve, con
 template <typename T, typename V, int BS, bool UPPER>
 int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                   const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                  const T *ep_a, T *ep_b, const int *stop) {
+                  const T *ep_a, T *ep_b, const int *stop, const SweepPre<T> &pre) {
     using W = typename RowWord<BS>::W;
     const unsigned grid = (unsigned)((L.nslices + kTicketWaves - 1) / kTicketWaves);
     const unsigned base = L.ticket_base;
     k_sptrsv_ticket<T, V, BS, UPPER><<<grid, 64 * kTicketWaves, 0, c->stream>>>(
         L.nslices, base, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-        UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop);
+        UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre);
     SENK_LAUNCH_CHECK();
     L.ticket_base = base + grid;
     return SENK_OK;
@@ -761,12 +788,12 @@ int launch_ticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V 
 template <typename T, typename V, int BS, bool UPPER>
 int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                    const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                   const T *ep_a, T *ep_b, const int *stop) {
+                   const T *ep_a, T *ep_b, const int *stop, const SweepPre<T> &pre) {
     using W = typename RowWord<BS>::W;
     if ((!L.xslice && !L.tord) || L.xwg <= 0) return SENK_ERR_ARG;
     k_sptrsv_xticket<T, V, BS, UPPER><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(
         L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-        UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop);
+        UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
@@ -774,26 +801,28 @@ int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V
 template <typename T, typename V, int BS>
 int solve_sweep(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V *vi, const DiagT<V> *dinv,
                 const signed char *sx, const T *x, T *y, const double *in_scale, T *reset, T *copy,
-                const int *stop, const T *ep_a = nullptr, T *ep_b = nullptr) {
+                const int *stop, const T *ep_a = nullptr, T *ep_b = nullptr,
+                const SweepPre<T> &pre = SweepPre<T>{}) {
     const int mode = L.mode;
     if (mode == kTrsvXcd) {
         if (L.upper)
             return launch_xticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a,
-                                                  ep_b, stop);
+                                                  ep_b, stop, pre);
         return launch_xticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b,
-                                               stop);
+                                               stop, SweepPre<T>{});
     }
     if (mode == kTrsvTicket) {
         if (L.upper)
             return launch_ticket<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a,
-                                                 ep_b, stop);
+                                                 ep_b, stop, pre);
         return launch_ticket<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b,
-                                              stop);
+                                              stop, SweepPre<T>{});
     }
     if (L.upper)
         return launch_sweep<T, V, BS, true>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b,
-                                            stop);
-    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b, stop);
+                                            stop, pre);
+    return launch_sweep<T, V, BS, false>(c, s, L, ve, vi, dinv, sx, x, y, in_scale, reset, copy, ep_a, ep_b, stop,
+                                         SweepPre<T>{});
 }
 
 template <typename T>
@@ -1058,14 +1087,17 @@ int launch_ilu_layout(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule 
 // One triangle on its own, in = out allowed.  In the single-launch modes, given both
 // epilogue vectors (*fused tells whether they were used): a U sweep also writes
 // ep_b = in - ep_a * out; an L sweep writes out = ep_a * ep_b + L^-1 in (ep_b read, and
-// then not the same vector as out).  The single-launch modes keep their
+// then not the same vector as out).  A U solve given pre (pre.u == in) first updates its
+// right-hand side: in the sweep where a single launch solves the triangle, else by
+// k_sweep_pre before the levels.  The single-launch modes keep their
 // invariant as long as the triangles of one factorization alternate (L, U, L, ...):
 // each sweep writes its own coherent vector, which the other triangle's last sweep
 // left all-sentinel, and returns the other triangle's vector to the sentinel.
 template <typename T>
 int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
-                      const int *stop, const T *ep_a, T *ep_b, bool *fused) {
+                      const int *stop, const T *ep_a, T *ep_b, bool *fused, const SweepPre<T> &pre) {
     if (fused) *fused = false;
+    if (pre.u && (!self.upper || pre.u != in || !pre.r || !pre.q || !pre.yd || !pre.ab)) return SENK_ERR_ARG;
     if (self.nslices == 0) return SENK_OK;
     if (self.bsize != s.bsize || self.n != other.n || self.upper == other.upper) return SENK_ERR_ARG;
     const bool f64 = std::is_same<T, double>::value;
@@ -1079,14 +1111,18 @@ int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &oth
         if (!ep_a || !ep_b) ep_a = ep_b = nullptr;
         if (fused) *fused = ep_b != nullptr;
         SENK_BS_DISPATCH(self.bsize, (solve_sweep<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, mine, nullptr, theirs,
-                                                           out, stop, ep_a, ep_b)))
+                                                           out, stop, ep_a, ep_b, pre)))
+    }
+    if (pre.u) {
+        k_sweep_pre<T><<<(self.n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(self.n, pre, stop);
+        SENK_LAUNCH_CHECK();
     }
     SENK_BS_DISPATCH(self.bsize, (solve_levels<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, out, nullptr, stop)))
 }
 template int launch_tri_single<double>(Ctx *, const Schedule &, TriLayout &, TriLayout &, const double *, double *,
-                                       const int *, const double *, double *, bool *);
+                                       const int *, const double *, double *, bool *, const SweepPre<double> &);
 template int launch_tri_single<float>(Ctx *, const Schedule &, TriLayout &, TriLayout &, const float *, float *,
-                                      const int *, const float *, float *, bool *);
+                                      const int *, const float *, float *, bool *, const SweepPre<float> &);
 template int launch_ilu_layout<double>(Ctx *, const Schedule &, TriLayout &, const Schedule &, TriLayout &,
                                        const double *, double *, const double *, const int *, bool);
 template int launch_ilu_layout<float>(Ctx *, const Schedule &, TriLayout &, const Schedule &, TriLayout &,
