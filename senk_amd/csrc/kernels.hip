@@ -751,12 +751,22 @@ __global__ __launch_bounds__(kBlock) void k_ilu0(int nblk, int blk0, const int *
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
 }
+// Longest row: a grid-stride pass and one atomic per workgroup (one per wave over 2^21
+// rows took 374 us, the atomics on one word serialised).
+constexpr int kMaxRowLenGrid = 512;
 __global__ __launch_bounds__(kBlock) void k_max_row_len(int N, const int *__restrict__ rptr, int *out) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int len = i < N ? rptr[i + 1] - rptr[i] : 0;
+    __shared__ int sh[kWavesPerBlock];
+    int len = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock)
+        len = max(len, rptr[i + 1] - rptr[i]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, 64));
-    if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(out, len);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = len;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kWavesPerBlock; w++) len = max(len, sh[w]);
+        if (len > 0) atomicMax(out, len);
+    }
 }
 template <typename T, int R>
 static int ilu0_levels(Ctx *c, const Schedule &s, const int *rptr, const int *cind, const int *dpos, T *val,
@@ -780,7 +790,8 @@ int launch_ilu0(Ctx *c, const Schedule &s, int N, const int *rptr, const int *ci
     int st = SENK_OK;
     if (hipMemsetAsync(dmax, 0, sizeof(int), c->stream) != hipSuccess) st = SENK_ERR_HIP;
     if (!st) {
-        k_max_row_len<<<(N + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(N, rptr, dmax);
+        k_max_row_len<<<std::max(1, std::min((N + kBlock - 1) / kBlock, kMaxRowLenGrid)), kBlock, 0, c->stream>>>(
+            N, rptr, dmax);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(&maxlen, dmax, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)

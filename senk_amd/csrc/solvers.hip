@@ -1610,8 +1610,8 @@ __global__ __launch_bounds__(kBlock) void k_cgs_dots(int N, const float *__restr
 // (SweepPre) and k_cgs_last does for the step that ends the loop.  The partials are read
 // before the stop flag, all at once.  (Formed instead by the last of k_cgs_dots'
 // workgroups, counted with an atomic: 128^3 nested solve 21.8 ms with the partials
-// passed through the device-coherent level, 26.3 ms with device-scope fences, against
-// 20.8 ms for this kernel.)
+// passed through the device-coherent level, 26.3 ms with device-scope fences, which
+// write back and invalidate an XCD's L2, against 20.8 ms for this kernel.)
 __global__ __launch_bounds__(kBlock) void k_cgs_scalars(const double *__restrict__ P, int B,
                                                         const double *__restrict__ dtol, int iter, int *state,
                                                         float *ab, double *hist, int hcap,
