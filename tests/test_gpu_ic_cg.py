@@ -246,6 +246,38 @@ def test_split_inner_cg(ctx, kind, mode):
         S.lib().senk_mat_destroy(A)
 
 
+@pytest.mark.parametrize("kind", [S.SENK_IC0, S.SENK_ILU0])
+@pytest.mark.parametrize("inner", [(60, 1e-3), (7, 1e-3)])
+def test_split_inner_cg_modes_bit_identical(ctx, kind, inner):
+    """The split inner CG forms its vector passes inside the sweeps in the single-launch
+    modes (w = u - E t after the U solve, q = D t + L^-1 w after the L solve, and the
+    previous step's yd, r, u update before the U solve reads u) and as separate passes in
+    the per-level mode, with the same arithmetic: every mode gives the same iterations and
+    the same solution bit for bit.  Inner solves that stop on their tolerance and inner
+    solves cut at the step limit (7 steps) both end with the last step's yd update."""
+    sy = _system(18, 8, seed=9)
+    A, M, st = _handles(ctx, sy, kind)
+    assert st == 0
+    try:
+        S.call("senk_prec_set_split", M, 1)
+        out = {}
+        for mode in (0, 1, 2, 3):
+            S.call("senk_prec_set_trsv_mode", M, mode)
+            print(f"split cg mode {mode} inner {inner}", flush=True)
+            st, res, x = _solve(ctx, A, M, sy, solver=S.SENK_CG, eps=1e-10, mixed=2, inner_max=inner[0],
+                                inner_eps=inner[1], outer_m=30, max_iter=3000)
+            assert st == 0 and res.converged == 1 and res.status == 0, mode
+            assert res.true_relres <= 1.5e-10 and relerr(x, sy["xs"]) <= 1e-7
+            out[mode] = (res.iters, res.restarts, x)
+        for mode in (1, 2, 3):
+            assert out[mode][:2] == out[0][:2], (mode, out[mode][:2], out[0][:2])
+            assert np.array_equal(out[mode][2], out[0][2]), mode
+        S.call("senk_prec_check", M)
+    finally:
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
+
+
 @pytest.mark.parametrize("mode", [3, "csr"])
 def test_cg_zero_right_hand_side(ctx, mode):
     """b = 0 from x0 = 0: (r, z) = 0 at the start, so CG stops at once as converged and x
