@@ -154,10 +154,16 @@ struct SweepPre {
     T *yd = nullptr;
     const T *ab = nullptr;
 };
+// out may be null where the triangle is solved by a single-launch sweep: the result is
+// then only in the sweep's own vector, tri_single_sweep_vector(self, other) (null: the
+// per-level mode, which has none), valid until the other triangle's next solve returns it
+// to the sentinel.
 template <typename T>
 int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
                       const int *stop, const T *ep_a = nullptr, T *ep_b = nullptr, bool *fused = nullptr,
                       const SweepPre<T> &pre = SweepPre<T>{});
+template <typename T>
+T *tri_single_sweep_vector(const TriLayout &self, const TriLayout &other);
 
 // ----------------------------------------------------------------- launchers (kernels.hip)
 template <typename T>

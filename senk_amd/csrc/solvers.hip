@@ -1719,6 +1719,11 @@ static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b,
     k_cgs_init<<<1, kBlock, 0, c->stream>>>(red_slot(c, kSlotB), B, sc, state, p.eps, dtol);
     SENK_LAUNCH_CHECK();
     poll.reset();
+    // Where both triangles are solved by single-launch sweeps, t = U^-1 u is read by the L
+    // sweep's epilogue from the U sweep's own vector (before the L sweep returns it to the
+    // sentinel) and is not copied out
+    float *umid = tri_single_sweep_vector<float>(*M.lt, *M.ut) ? tri_single_sweep_vector<float>(*M.ut, *M.lt)
+                                                                : nullptr;
     bool stop = false;
     for (int i = 0; i < p.max_iter && !stop; i++) {
         // q = L^-1 A U^-1 u  (t = U^-1 u; w = u - E t; q = D t + L^-1 w), each vector pass
@@ -1727,12 +1732,13 @@ static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b,
         bool fused = false;
         SweepPre<float> pre;
         if (i > 0) pre = SweepPre<float>{u, r, q, yd, ab};
-        SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, u, t, state, M.dilu_e, w, &fused, pre));
+        SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, u, umid ? nullptr : t, state, M.dilu_e, w, &fused,
+                                          pre));
         if (!fused) {
             k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, u, M.dilu_e, t, w, state);
             SENK_LAUNCH_CHECK();
         }
-        SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, w, q, state, M.dilu_d, t, &fused));
+        SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, w, q, state, M.dilu_d, umid ? umid : t, &fused));
         if (!fused) {
             k_cgs_q<<<ew_grid(N), kBlock, 0, c->stream>>>(N, M.dilu_d, t, q, state);
             SENK_LAUNCH_CHECK();

@@ -383,11 +383,13 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
             }
         }
         if (!UPPER && in_scale) t = (T)((double)t * *in_scale);
-        if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
-        if (!UPPER && SWEEP && ep_a) {  // requested with the right-hand side, used at the end
+        // requested with the right-hand side, used at the end; read before the reset, as
+        // ep_b may be the U sweep's own vector, which this L sweep returns to the sentinel
+        if (!UPPER && SWEEP && ep_a) {
             pa = ep_a[lr.row];
             pb = ep_b[lr.row];
         }
+        if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
     }
     const T rhs = UPPER ? t : T(0);  // kept for ep_b (U only)
     constexpr int CH = 8;
@@ -1094,6 +1096,16 @@ int launch_ilu_layout(Ctx *c, const Schedule &ls, TriLayout &lt, const Schedule 
 // each sweep writes its own coherent vector, which the other triangle's last sweep
 // left all-sentinel, and returns the other triangle's vector to the sentinel.
 template <typename T>
+T *tri_single_sweep_vector(const TriLayout &self, const TriLayout &other) {
+    const bool f64 = std::is_same<T, double>::value;
+    T *mine = (T *)(f64 ? self.mid64 : self.mid32);
+    const void *theirs = f64 ? other.mid64 : other.mid32;
+    return self.mode != kTrsvLevels && mine && theirs && self.sync ? mine : nullptr;
+}
+template float *tri_single_sweep_vector<float>(const TriLayout &, const TriLayout &);
+template double *tri_single_sweep_vector<double>(const TriLayout &, const TriLayout &);
+
+template <typename T>
 int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &other, const T *in, T *out,
                       const int *stop, const T *ep_a, T *ep_b, bool *fused, const SweepPre<T> &pre) {
     if (fused) *fused = false;
@@ -1105,9 +1117,10 @@ int launch_tri_single(Ctx *c, const Schedule &s, TriLayout &self, TriLayout &oth
     const T *vi = (const T *)(f64 ? (void *)self.vi64 : (void *)self.vi32);
     const T *dv = self.upper ? (const T *)(f64 ? (void *)self.d64 : (void *)self.d32) : nullptr;
     if (!ve || !vi || (self.upper && !dv)) return SENK_ERR_ARG;
-    T *mine = (T *)(f64 ? self.mid64 : self.mid32);
+    T *mine = tri_single_sweep_vector<T>(self, other);
     T *theirs = (T *)(f64 ? other.mid64 : other.mid32);
-    if (self.mode != kTrsvLevels && mine && theirs && self.sync) {
+    if (!mine && !out) return SENK_ERR_ARG;
+    if (mine) {
         if (!ep_a || !ep_b) ep_a = ep_b = nullptr;
         if (fused) *fused = ep_b != nullptr;
         SENK_BS_DISPATCH(self.bsize, (solve_sweep<T, T, B>(c, s, self, ve, vi, dv, nullptr, in, mine, nullptr, theirs,
