@@ -98,6 +98,7 @@ struct TriLayout {
     int nsteps = 0;              // non-empty levels, in solve order
     std::vector<int> step_slice; // host, nsteps + 1: first slice of each step
     long long n_ext = 0, n_int = 0;
+    int max_ext = 0;             // largest external entry count of a row
     int4 *sinfo = nullptr;       // per slice: external offset, in-block offset, first block entry, blocks
     void *rw = nullptr;          // per slot (slice*64 + lane): external count and in-block mask
     int *next = nullptr;         // bsize 64 only: per slot external count

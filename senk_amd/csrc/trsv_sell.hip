@@ -154,7 +154,8 @@ struct LaneRow {
     }
 };
 
-// Per row: the row word; per slice: entry counts.  err = 1 when a row breaks the
+// Per row: the row word; per slice: entry counts (ecnt[w] external entries of slice w,
+// ecnt[nslices + w] its rows' largest external count).  err = 1 when a row breaks the
 // assumptions of the layout (strictly increasing columns; L strictly lower; U starting
 // with its diagonal slot, then columns > row), 2 when a count does not fit the word.
 template <int BS, bool UPPER>
@@ -199,8 +200,12 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(int nslices, const int4 *_
     rw[slot] = RowWord<BS>::pack(wide ? 0 : ne, mk);
     if (!RowWord<BS>::kPacked) next[slot] = ne;
     const double se = wave_sum((double)ne), si = wave_sum((double)__popcll((unsigned long long)mk));
+    int mx = ne;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
     if ((threadIdx.x & 63) == 0) {
         ecnt[w] = (int)se;
+        ecnt[nslices + w] = mx;
         icnt[w] = (int)si;
     }
     if (bad) *err = 1;
@@ -354,7 +359,7 @@ __device__ __forceinline__ SliceHead<V, BS> slice_head(int slice, const int4 *__
     return h;
 }
 
-template <typename T, typename V, int BS, bool UPPER, bool SWEEP>
+template <typename T, typename V, int BS, bool UPPER, bool SWEEP, int CH = 8>
 __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, const int *__restrict__ blk_list,
                                                  const int *__restrict__ scol, const V *__restrict__ ve,
                                                  const V *__restrict__ vi, const T *x, T *y,
@@ -392,7 +397,7 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
         if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
     }
     const T rhs = UPPER ? t : T(0);  // kept for ep_b (U only)
-    constexpr int CH = 8;
+    // CH external entries per lane per round of gathers
     constexpr int IC = BS < 8 ? BS : 8;
     int pos = si.x, k0 = 0;
     int cc[CH];
@@ -629,7 +634,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
 // ticket j of stream x solves slice 64 j + 8 x + w: the wave goes from its workgroup
 // number to its slice descriptor and row words without a lookup.  Otherwise
 // xslice[x][j][w] names the slice (-1: none).
-template <typename T, typename V, int BS, bool UPPER>
+template <typename T, typename V, int BS, bool UPPER, int CH>
 __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     int xwg, const int *__restrict__ xslice, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
     const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
@@ -646,8 +651,8 @@ __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     const int stopped = stop_flag ? *stop_flag : 0;
     const SliceHead<V, BS> h = slice_head<V, BS, UPPER>(slice, sinfo, rw, next, dinv, sx);
     if (stopped) return;
-    solve_slice_body<T, V, BS, UPPER, true>(h, blk_list, scol, ve, vi, x, y, in_scale, reset, sync + kSweepErr, copy,
-                                            ep_a, ep_b, pre);
+    solve_slice_body<T, V, BS, UPPER, true, CH>(h, blk_list, scol, ve, vi, x, y, in_scale, reset, sync + kSweepErr,
+                                                copy, ep_a, ep_b, pre);
 }
 
 // The update of a U solve's right-hand side (SweepPre) as a pass of its own, for the
@@ -793,9 +798,19 @@ int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V
                    const T *ep_a, T *ep_b, const int *stop, const SweepPre<T> &pre) {
     using W = typename RowWord<BS>::W;
     if ((!L.xslice && !L.tord) || L.xwg <= 0) return SENK_ERR_ARG;
-    k_sptrsv_xticket<T, V, BS, UPPER><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(
-        L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-        UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre);
+    // FP64 rows of at most 8 external entries: rounds of 4 gathers instead of one of 8
+    // (64-66 VGPRs instead of 96-98: 7-8 waves per SIMD instead of 4-5; FP64 application at
+    // 128^3, blocks of 8, 57.0 against 64.2 us in the simple order, 86.4 against 89.0 us in
+    // the connect order, whose rows have up to 6).  FP32 (67 VGPRs either way) loses with
+    // a second round: 72.3 against 62.9 us in the connect order.
+    if (std::is_same<T, double>::value && L.max_ext <= 8)
+        k_sptrsv_xticket<T, V, BS, UPPER, 4><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(
+            L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
+            UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre);
+    else
+        k_sptrsv_xticket<T, V, BS, UPPER, 8><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(
+            L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
+            UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
@@ -934,7 +949,7 @@ int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rp
     const size_t nslots = (size_t)L->nslices * 64;
     int *ecnt = nullptr, *icnt = nullptr, *err = nullptr;
     int st = SENK_OK;
-    std::vector<int> he(L->nslices), hi(L->nslices);
+    std::vector<int> he(2 * (size_t)L->nslices), hi(L->nslices);
     int herr = 0;
     auto fail = [&](int code) {
         (void)hipFree(ecnt);
@@ -946,7 +961,7 @@ int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rp
     if (hipMalloc(&L->sinfo, si.size() * sizeof(int4)) != hipSuccess ||
         hipMalloc(&L->rw, nslots * row_word_bytes(BS)) != hipSuccess ||
         (BS > 32 && hipMalloc(&L->next, nslots * sizeof(int)) != hipSuccess) ||
-        hipMalloc(&ecnt, L->nslices * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ecnt, 2 * (size_t)L->nslices * sizeof(int)) != hipSuccess ||
         hipMalloc(&icnt, L->nslices * sizeof(int)) != hipSuccess || hipMalloc(&err, sizeof(int)) != hipSuccess ||
         hipMalloc(&L->sync, kSweepWords * sizeof(unsigned)) != hipSuccess)
         return fail(SENK_ERR_ALLOC);
@@ -963,14 +978,17 @@ int tri_layout_build(Ctx *c, const Schedule &s, bool upper, int N, const int *rp
         return fail(SENK_ERR_HIP);
     if (herr) return fail(SENK_ERR_UNSUPPORTED);
     long long ne = 0, ni = 0;
+    int mx = 0;
     for (int i = 0; i < L->nslices; i++) {
         si[i].x = (int)ne;
         si[i].y = (int)ni;
         ne += he[i];
         ni += hi[i];
+        mx = std::max(mx, he[(size_t)L->nslices + i]);
         if (ne > INT32_MAX || ni > INT32_MAX) return fail(SENK_ERR_UNSUPPORTED);
     }
     L->n_ext = ne;
+    L->max_ext = mx;
     L->n_int = ni;
     if (hipMalloc(&L->scol, (size_t)(ne ? ne : 1) * sizeof(int)) != hipSuccess) return fail(SENK_ERR_ALLOC);
     if (hipMemcpyAsync(L->sinfo, si.data(), si.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
