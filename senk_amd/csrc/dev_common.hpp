@@ -148,6 +148,17 @@ __device__ __forceinline__ void get4(const T *p, long long i0, long long n, T (&
         for (int u = 0; u < 4; u++) o[u] = i0 + u < n ? p[i0 + u] : T(0);
     }
 }
+// get4 with `fill` in the places at or beyond n (1 for a divisor: a zero there would turn
+// the group's zero terms into 0 * inf)
+template <typename T>
+__device__ __forceinline__ void get4_fill(const T *p, long long i0, long long n, T (&o)[4], T fill) {
+    if (i0 + 4 <= n && aligned16(p)) {
+        ld4(p + i0, o);
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) o[u] = i0 + u < n ? p[i0 + u] : fill;
+    }
+}
 template <typename T>
 __device__ __forceinline__ void put4(T *p, long long i0, long long n, const T (&o)[4]) {
     if (i0 + 4 <= n && aligned16(p)) {

@@ -1529,7 +1529,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_start(int N, const float *__rest
     SENK_FOR4(i0, N) {
         float rv[4], dv[4];
         get4(r, i0, N, rv);
-        get4(d, i0, N, dv);
+        get4_fill(d, i0, N, dv, 1.0f);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             a0 += (double)rv[u] * (double)rv[u];
@@ -1586,7 +1586,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_dots(int N, const float *__restr
         get4(q, i0, N, qv);
         get4(u, i0, N, uv);
         get4(r, i0, N, rv);
-        get4(d, i0, N, dv);
+        get4_fill(d, i0, N, dv, 1.0f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const double qk = qv[k], uk = uv[k], rk = rv[k], id = 1.0 / (double)dv[k];

@@ -248,14 +248,17 @@ def test_split_inner_cg(ctx, kind, mode):
 
 @pytest.mark.parametrize("kind", [S.SENK_IC0, S.SENK_ILU0])
 @pytest.mark.parametrize("inner", [(60, 1e-3), (7, 1e-3)])
-def test_split_inner_cg_modes_bit_identical(ctx, kind, inner):
+@pytest.mark.parametrize("grid", [(18, 8), (13, 1)])
+def test_split_inner_cg_modes_bit_identical(ctx, kind, inner, grid):
     """The split inner CG forms its vector passes inside the sweeps in the single-launch
     modes (w = u - E t after the U solve, q = D t + L^-1 w after the L solve, and the
     previous step's yd, r, u update before the U solve reads u) and as separate passes in
     the per-level mode, with the same arithmetic: every mode gives the same iterations and
     the same solution bit for bit.  Inner solves that stop on their tolerance and inner
-    solves cut at the step limit (7 steps) both end with the last step's yd update."""
-    sy = _system(18, 8, seed=9)
+    solves cut at the step limit (7 steps) both end with the last step's yd update.  13^3
+    rows in blocks of 1 (N = 2197, not a multiple of 4): the passes' last group of four
+    takes its missing places as zeros, and divides by 1 there, not by 0."""
+    sy = _system(grid[0], grid[1], seed=9)
     A, M, st = _handles(ctx, sy, kind)
     assert st == 0
     try:
