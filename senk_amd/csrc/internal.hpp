@@ -140,7 +140,8 @@ int tri_layout_fill_f16(Ctx *c, const Schedule &s, TriLayout *L, const int *rptr
 // L, U, L, ... (an ILU application counts as L then U).
 // With both epilogue vectors (single-launch modes only; *fused says whether they were
 // used) a U sweep also writes ep_b = in - ep_a * out, and an L sweep writes
-// out = ep_a * ep_b + L^-1 in (ep_b read; not the same vector as out).
+// out = ep_a * ep_b + L^-1 in (ep_b read; not the same vector as out; it may be the U
+// sweep's own vector, each row read before this sweep returns it to the sentinel).
 // A U solve may first update its right-hand side (pre.u == in, non-null): with
 // a = pre.ab[0], b = pre.ab[1], per row yd += a u, r = r - a q, u = r + b u, and the
 // solve uses the new u (the split CG's vector step folded into the next product).  The
