@@ -359,7 +359,7 @@ __device__ __forceinline__ SliceHead<V, BS> slice_head(int slice, const int4 *__
     return h;
 }
 
-template <typename T, typename V, int BS, bool UPPER, bool SWEEP, int CH = 8>
+template <typename T, typename V, int BS, bool UPPER, bool SWEEP, int CH = 8, bool PIPE = false>
 __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, const int *__restrict__ blk_list,
                                                  const int *__restrict__ scol, const V *__restrict__ ve,
                                                  const V *__restrict__ vi, const T *x, T *y,
@@ -404,22 +404,24 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
     bool on[CH];
     T vv[CH];
     bool more = __ballot(0 < ne) != 0;
-    auto load_chunk = [&]() {
+    // entries kk .. kk+CH-1 of every lane (the packed positions advance in entry order)
+    auto load_chunk_at = [&](int kk, int (&c_)[CH], bool (&o_)[CH], T (&v_)[CH]) {
         int idx[CH];
 #pragma unroll
         for (int q = 0; q < CH; q++) {
-            on[q] = k0 + q < ne;
-            const unsigned long long b = __ballot(on[q]);
+            o_[q] = kk + q < ne;
+            const unsigned long long b = __ballot(o_[q]);
             idx[q] = pos + lanes_below(b);
             pos += __popcll(b);
         }
 #pragma unroll
         for (int q = 0; q < CH; q++)
-            if (on[q]) {
-                cc[q] = scol[idx[q]];
-                vv[q] = decode<T>(ve[idx[q]], ex);
+            if (o_[q]) {
+                c_[q] = scol[idx[q]];
+                v_[q] = decode<T>(ve[idx[q]], ex);
             }
     };
+    auto load_chunk = [&]() { load_chunk_at(k0, cc, on, vv); };
     if (more) load_chunk();
     T iv0[IC];
     bool on0[IC];
@@ -434,12 +436,23 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
     }
     while (more) {
         T yv[CH];
+        // PIPE (rows with more than CH external entries): the next round's entries are
+        // requested while this round's gathers are in flight, so a round costs one
+        // dependent memory round trip instead of two
+        int ccn[CH];
+        bool onn[CH];
+        T vvn[CH];
+        bool moren = false;
         if (SWEEP) {
             U yb[CH];
             bool pend = false;
 #pragma unroll
             for (int q = 0; q < CH; q++)
                 if (on[q]) yb[q] = load_local(y + cc[q]);
+            if constexpr (PIPE) {
+                moren = __ballot(k0 + CH < ne) != 0;
+                if (moren) load_chunk_at(k0 + CH, ccn, onn, vvn);
+            }
 #pragma unroll
             for (int q = 0; q < CH; q++) pend |= on[q] && yb[q] == Bits<T>::kSentinel;
             for (int spin = 0; __ballot(pend); spin++) {
@@ -467,8 +480,20 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
         for (int q = 0; q < CH; q++)
             if (on[q]) t -= vv[q] * yv[q];
         k0 += CH;
-        more = __ballot(k0 < ne) != 0;
-        if (more) load_chunk();
+        if constexpr (PIPE && SWEEP) {
+            more = moren;
+            if (more) {
+#pragma unroll
+                for (int q = 0; q < CH; q++) {
+                    cc[q] = ccn[q];
+                    on[q] = onn[q];
+                    vv[q] = vvn[q];
+                }
+            }
+        } else {
+            more = __ballot(k0 < ne) != 0;
+            if (more) load_chunk();
+        }
     }
 #pragma unroll
     for (int i0 = 0; i0 < BS; i0 += IC) {
@@ -634,7 +659,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
 // ticket j of stream x solves slice 64 j + 8 x + w: the wave goes from its workgroup
 // number to its slice descriptor and row words without a lookup.  Otherwise
 // xslice[x][j][w] names the slice (-1: none).
-template <typename T, typename V, int BS, bool UPPER, int CH>
+template <typename T, typename V, int BS, bool UPPER, int CH, bool PIPE>
 __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     int xwg, const int *__restrict__ xslice, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
     const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
@@ -651,8 +676,8 @@ __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     const int stopped = stop_flag ? *stop_flag : 0;
     const SliceHead<V, BS> h = slice_head<V, BS, UPPER>(slice, sinfo, rw, next, dinv, sx);
     if (stopped) return;
-    solve_slice_body<T, V, BS, UPPER, true, CH>(h, blk_list, scol, ve, vi, x, y, in_scale, reset, sync + kSweepErr,
-                                                copy, ep_a, ep_b, pre);
+    solve_slice_body<T, V, BS, UPPER, true, CH, PIPE>(h, blk_list, scol, ve, vi, x, y, in_scale, reset,
+                                                      sync + kSweepErr, copy, ep_a, ep_b, pre);
 }
 
 // The update of a U solve's right-hand side (SweepPre) as a pass of its own, for the
@@ -803,14 +828,17 @@ int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V
     // 128^3, blocks of 8, 57.0 against 64.2 us in the simple order, 86.4 against 89.0 us in
     // the connect order, whose rows have up to 6).  FP32 (67 VGPRs either way) loses with
     // a second round: 72.3 against 62.9 us in the connect order.
-    if (std::is_same<T, double>::value && L.max_ext <= 8)
-        k_sptrsv_xticket<T, V, BS, UPPER, 4><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(
-            L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-            UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre);
-    else
-        k_sptrsv_xticket<T, V, BS, UPPER, 8><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(
-            L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,
-            UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre);
+    const bool ch4 = std::is_same<T, double>::value && L.max_ext <= 8;
+    const bool pipe = L.max_ext > (ch4 ? 4 : 8);
+#define SENK_XT_LAUNCH(CHV, PIPEV)                                                                         \
+    k_sptrsv_xticket<T, V, BS, UPPER, CHV, PIPEV><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(   \
+        L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,          \
+        UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre)
+    if (ch4 && pipe) SENK_XT_LAUNCH(4, true);
+    else if (ch4) SENK_XT_LAUNCH(4, false);
+    else if (pipe) SENK_XT_LAUNCH(8, true);
+    else SENK_XT_LAUNCH(8, false);
+#undef SENK_XT_LAUNCH
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
