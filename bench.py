@@ -760,7 +760,8 @@ C2_INNER = dict(inner_max=120, inner_eps=1e-4)
 
 def config_c2(ctx, n, K, no_cpu):
     """27-point convection-diffusion 128^3 (Pe 10), ABMC blocks of 16: plain FP64
-    ILU(0)-GMRES(30) and ILU(0)-BiCGStab, the mixed form, SpMV and ILU rates, and the
+    ILU(0)-GMRES(30) and ILU(0)-BiCGStab, the nested mixed form, the FP64 BiCGStab with the
+    FP32 factor application, SpMV and ILU rates, and the
     reference's AbmcIluGmresm count on the same system and factors; then the same solves
     with "simple" blocks of 16 (8 colors instead of 13)."""
     log(f"[bench] building the {n}^3 27-point convection-diffusion system (ABMC blocks of 16)")
@@ -772,6 +773,8 @@ def config_c2(ctx, n, K, no_cpu):
     out["fp64_gmres30"] = slim(timed(dc, K, 1, solver=S.SENK_GMRES, m=30, eps=1e-10, max_iter=6000))
     out["fp64_bicgstab"] = slim(timed(dc, K, 1, solver=S.SENK_BICGSTAB, eps=1e-10, max_iter=6000))
     out["mixed_bicgstab"] = slim(timed(dc, K, 1, solver=S.SENK_BICGSTAB, **mixed))
+    # the FP64 BiCGStab with its ILU(0) application on the FP32 factor copies (mixed = 3)
+    out["fp64_bicgstab_f32_ilu"] = slim(timed(dc, K, 1, solver=S.SENK_BICGSTAB, eps=1e-10, max_iter=6000, mixed=3))
     if not no_cpu:
         gi = out["fp64_gmres30"]["iters"]
         out["reference_gmres30"] = reference_entry(ref_abmc_gmres(dc, 30, 1e-10, 40.0, gi), gi)
@@ -791,7 +794,8 @@ def config_c3(ctx, K, no_cpu, n=4194304):
     """Unstructured nonsymmetric matrix, n = 4,194,304: FP32 values and ILU(0) factors,
     FP32 SpMV and inner GMRES(50), FP64 refinement to 1e-10; error against an FP64 solve
     of the same system (the device's FP64 ILU(0)-GMRES(50) to 1e-14); the plain FP64
-    GMRES(50) and the reference's AbmcIluGmresm(m = 50) count beside it."""
+    GMRES(50), the FP64 FGMRES(50) with the FP32 factor application, and the reference's
+    AbmcIluGmresm(m = 50) count beside it."""
     log(f"[bench] building the unstructured system (n = {n}, ABMC blocks of 16)")
     sy = build_system_kind("unstructured", n, 16)
     dev = Device(ctx, sy)
@@ -808,6 +812,10 @@ def config_c3(ctx, K, no_cpu, n=4194304):
               inner_eps=1e-5, inner_max=500)
     m["err_vs_fp64_solve"] = relerr(m.pop("x"), x64)
     out["mixed_ir_gmres50"] = slim(m)
+    # FP64 FGMRES(50) with the ILU(0) application on the FP32 factor copies (mixed = 3)
+    m = timed(dev, K, 1, keep_x=True, solver=S.SENK_FGMRES, m=50, eps=1e-10, max_iter=20000, mixed=3)
+    m["err_vs_fp64_solve"] = relerr(m.pop("x"), x64)
+    out["fp64_fgmres50_f32_ilu"] = slim(m)
     if not no_cpu:
         gi = out["fp64_gmres50"]["iters"]
         out["reference_gmres50"] = reference_entry(ref_abmc_gmres(dev, 50, 1e-10, 40.0, gi), gi)
@@ -820,7 +828,8 @@ def config_c5(ctx, K, no_cpu, n=96):
     """Block path: 3 dof per node on n^3 (3x3-block 7-point stencil, nu 0.3, lognormal
     moduli), ABMC blocks of 48: ILUB (3x3 block fill) and ILU(1) over the colors, FP32
     factors + FP64 refinement + FGMRES(50), with ILUB also with the learned correction; error
-    against the FP64 ILUB-GMRES(50) solve to 1e-14; the plain FP64 ILUB-GMRES(50) and the
+    against the FP64 ILUB-GMRES(50) solve to 1e-14; the FP64 FGMRES(50) with the factor
+    application on the FP32 copies; the plain FP64 ILUB-GMRES(50) and the
     reference's AbmcIlubGmresm<double,3,3> count beside it."""
     log(f"[bench] building the {n}^3 3-dof block system (ABMC blocks of 48)")
     sy = build_system_kind("elasticity", n, 48)
@@ -843,6 +852,10 @@ def config_c5(ctx, K, no_cpu, n=96):
         r = timed(dev, K, 1, keep_x=True, **mixed)
         r["err_vs_fp64_solve"] = relerr(r.pop("x"), x64)
         out[f"mixed_{tag}_fgmres50"] = slim(r)
+        # FP64 FGMRES(50) with the factor application on the FP32 copies (mixed = 3)
+        r = timed(dev, K, 1, keep_x=True, solver=S.SENK_FGMRES, m=50, eps=1e-10, max_iter=20000, mixed=3)
+        r["err_vs_fp64_solve"] = relerr(r.pop("x"), x64)
+        out[f"fp64_{tag}_fgmres50_f32_apply"] = slim(r)
         net, _ = load_learned(ctx) if tag == "ilub" else (None, None)
         if net is not None:
             net.bind(dev.A)

@@ -513,7 +513,11 @@ typedef struct {
     int ortho;           /* senk_ortho (GMRES family)                                    */
     int mixed;           /* 1: FP32 matrix/factors/inner Krylov + FP64 iterative refinement;
                             2: FP32 inner Krylov as the preconditioner of an FP64 flexible
-                               GCR(outer_m) outer iteration (nested)                      */
+                               GCR(outer_m) outer iteration (nested);
+                            3: the FP64 solver with its ILU / ILUB application run on the
+                               FP32 factor copies (FP64 vectors converted on the way in
+                               and out; the preconditioner must keep an FP32 copy; FGMRES,
+                               GCR, FGCR, BiCGStab or CG, not GMRES)                     */
     int factor_prec;     /* senk_factor_precision of the FP32 applications of the mixed
                             forms: F32 (default when the FP32 copy exists) or F16        */
     double inner_eps;    /* mixed: inner relative tolerance per refinement step          */

@@ -1685,7 +1685,27 @@ int senk_solve(senk_ctx ctx, senk_mat A, senk_prec M, const double *db, double *
         if (!o->mixed || !M || !M->lt.ve16 || !M->ut.ve16) return SENK_ERR_ARG;
         pv.f16 = true;
     }
-    if (o->mixed) {
+    if (o->mixed < 0 || o->mixed > 3) return SENK_ERR_ARG;
+    if (o->mixed == 3) {
+        // FP64 Krylov solve whose ILU / ILUB application runs on the FP32 factor copies.
+        // Not right-preconditioned GMRES: it forms x from M^-1 (V y) at the end of a
+        // cycle, a new application whose FP32 rounding the cycle's residual never saw
+        // (true residual ~1e-7 where the recurrence reports 1e-10); the solvers that keep
+        // the preconditioned vectors they use (FGMRES, GCR, FGCR, BiCGStab, CG) stay
+        // consistent.
+        if (!M || o->use_learned || pv.f16 || o->solver == SENK_GMRES) {
+            record_error("mixed = 3: needs FP32 factor copies, no learned correction, no FP16, not GMRES");
+            return SENK_ERR_ARG;
+        }
+        if (!(pv.kind == 1 && pv.l32 && pv.u32) && !(pv.kind == 2 && pv.bl32 && pv.bu32)) return SENK_ERR_ARG;
+        const size_t ld = ((size_t)A->N + 3) & ~(size_t)3;  // 16-byte aligned halves
+        void *buf = nullptr;
+        SENK_TRY(ctx_outer(&ctx->c, 2 * sizeof(float) * ld, &buf));
+        pv.apply32 = true;
+        pv.in32 = (float *)buf;
+        pv.out32 = pv.in32 + ld;
+    }
+    if (o->mixed == 1 || o->mixed == 2) {
         if (!A->v32 || (M && !M->view.l32 && !pv.f16)) return SENK_ERR_ARG;
         if (o->mixed == 2)
             SENK_TRY(nested_solve(&ctx->c, A->view, pv, db, dx, *o, nrm_b, res, hist, hist_cap));

@@ -526,6 +526,16 @@ static bool layout_has(const TriLayout *L) {
 template <typename T>
 int prec_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T *out, T *tmp, const int *stop) {
     const int N = M.N;
+    if constexpr (std::is_same<T, double>::value) {
+        if (M.apply32 && M.kind != 0) {  // FP64 solve, FP32 factor application
+            if (!M.in32 || !M.out32 || M.learned) return SENK_ERR_ARG;
+            PrecView m32 = M;
+            m32.apply32 = false;
+            SENK_TRY(launch_convert<double, float>(c, N, in, M.in32, in_scale));
+            SENK_TRY(prec_apply<float>(c, m32, M.in32, nullptr, M.out32, nullptr, stop));
+            return launch_convert<float, double>(c, N, M.out32, out, nullptr);
+        }
+    }
     if (M.kind == 0) {
         k_scale_copy<T><<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, out, in_scale, stop);
         SENK_LAUNCH_CHECK();

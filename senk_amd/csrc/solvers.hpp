@@ -58,6 +58,10 @@ struct PrecView {
     const float *dilu_d = nullptr, *dilu_e = nullptr;
     // A (to a few ulps) and the factorization symmetric: the split form of the inner CG applies
     bool symmetric = false;
+    // FP64 solves: the ILU / ILUB application in FP32 (the FP32 factor copies) between
+    // conversions through in32 / out32 (N floats each)
+    bool apply32 = false;
+    float *in32 = nullptr, *out32 = nullptr;
     // learned correction added to the ILU application (flexible solvers only)
     LearnedPrec *learned = nullptr;
     int *learned_off = nullptr;  // device flag: 1 = correction disabled by the monitor

@@ -103,3 +103,57 @@ def test_block_path_mixed_learned_fgmres50(ctx, kind):
         net.close()
         S.lib().senk_prec_destroy(M)
         S.lib().senk_mat_destroy(A)
+
+
+@pytest.mark.parametrize("case", ["convdiff27_ilu0", "elasticity_ilub"])
+@pytest.mark.parametrize("solver,m", [(S.SENK_BICGSTAB, 30), (S.SENK_FGMRES, 30), (S.SENK_GCR, 30)])
+def test_fp64_solve_with_fp32_factor_application(ctx, case, solver, m):
+    """mixed = 3: the FP64 Krylov solver with its ILU / ILUB application run on the FP32
+    factor copies (the residual converted to FP32 on the way in, the correction back to
+    FP64).  The FP32 application differs from the FP64 one only by rounding, so the solve
+    reaches the FP64 tolerance in about the FP64 solve's count, and its solution agrees
+    with the FP64 solve's to the level the residual allows.  Without an FP32 copy, with the
+    learned correction, with right-preconditioned GMRES (its end-of-cycle M^-1 (V y) would
+    carry FP32 rounding the cycle's residual never saw: a true residual of ~1e-7 where the
+    recurrence reports 1e-10) or with an unknown mode the call is refused."""
+    if case == "convdiff27_ilu0":
+        rptr, cind, val = S.gen_convdiff27(16, 10.0)
+        sy = _system(rptr, cind, val, 16, 5)
+        A, M = _handles(ctx, sy, S.SENK_ILU0)
+    else:
+        rptr, cind, val = S.gen_elasticity3(12, 2)
+        sy = _system(rptr, cind, val, 48, 2)
+        A, M = _handles(ctx, sy, S.SENK_ILUB, bnl=3, bnw=3)
+    N = sy["N"]
+    try:
+        r64, x64 = _solve(ctx, A, M, sy["b"], N, solver=solver, m=m, eps=1e-10, max_iter=20000)
+        r32, x32 = _solve(ctx, A, M, sy["b"], N, solver=solver, m=m, eps=1e-10, max_iter=20000, mixed=3)
+        print(f"{case} solver {solver}: FP64 factors {r64.iters} its, FP32 factors {r32.iters} its", flush=True)
+        for res in (r64, r32):
+            assert res.converged == 1 and res.status == 0
+            assert res.true_relres <= 1e-9
+        assert r32.iters <= 1.1 * r64.iters + 3, (r32.iters, r64.iters)
+        assert relerr(x32, x64) <= 1e-6, relerr(x32, x64)
+        db, dx = ctx.array(sy["b"]), ctx.zeros(N)
+        res = S.senk_result()
+        for bad in (dict(mixed=4), dict(mixed=3, use_learned=1), dict(mixed=3, solver=S.SENK_GMRES)):
+            o = S.solve_opts(**dict(dict(solver=solver, m=m, eps=1e-10, max_iter=100), **bad))
+            assert S.lib().senk_solve(ctx.h, A, M, db.p, dx.p, C.byref(o), C.byref(res), None, 0) != 0, bad
+    finally:
+        S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
+    # a preconditioner without the FP32 copy
+    A2 = C.c_void_p()
+    S.call("senk_mat_create", ctx.h, N, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]), 1, C.byref(A2))
+    opts = S.senk_ilu_opts(S.SENK_ILU0, 0, 0, 0, 1 << S.SENK_FACTOR_F64, 0)
+    M2 = C.c_void_p()
+    S.call("senk_prec_create_ilu", ctx.h, A2, O.P(sy["rptr"]), O.P(sy["cind"]), O.P(sy["val"]),
+           sy["cptr"].ctypes.data_as(C.c_void_p), sy["nc"], sy["bsize"], C.byref(opts), C.byref(M2))
+    try:
+        db, dx = ctx.array(sy["b"]), ctx.zeros(N)
+        res = S.senk_result()
+        o = S.solve_opts(solver=solver, m=m, eps=1e-10, max_iter=100, mixed=3)
+        assert S.lib().senk_solve(ctx.h, A2, M2, db.p, dx.p, C.byref(o), C.byref(res), None, 0) != 0
+    finally:
+        S.lib().senk_prec_destroy(M2)
+        S.lib().senk_mat_destroy(A2)
