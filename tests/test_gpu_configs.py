@@ -105,8 +105,9 @@ def test_block_path_mixed_learned_fgmres50(ctx, kind):
         S.lib().senk_mat_destroy(A)
 
 
-@pytest.mark.parametrize("case", ["convdiff27_ilu0", "elasticity_ilub"])
-@pytest.mark.parametrize("solver,m", [(S.SENK_BICGSTAB, 30), (S.SENK_FGMRES, 30), (S.SENK_GCR, 30)])
+@pytest.mark.parametrize("case,solver,m", [(c, s, 30) for c in ("convdiff27_ilu0", "elasticity_ilub")
+                                           for s in (S.SENK_BICGSTAB, S.SENK_FGMRES, S.SENK_GCR, S.SENK_FGCR)] +
+                         [("vardiff7_ic0", S.SENK_CG, 30)])
 def test_fp64_solve_with_fp32_factor_application(ctx, case, solver, m):
     """mixed = 3: the FP64 Krylov solver with its ILU / ILUB application run on the FP32
     factor copies (the residual converted to FP32 on the way in, the correction back to
@@ -120,6 +121,10 @@ def test_fp64_solve_with_fp32_factor_application(ctx, case, solver, m):
         rptr, cind, val = S.gen_convdiff27(16, 10.0)
         sy = _system(rptr, cind, val, 16, 5)
         A, M = _handles(ctx, sy, S.SENK_ILU0)
+    elif case == "vardiff7_ic0":
+        rptr, cind, val = S.gen_vardiff7(16, 3, 4.0, 1.5)
+        sy = _system(rptr, cind, val, 8, 3)
+        A, M = _handles(ctx, sy, S.SENK_IC0)
     else:
         rptr, cind, val = S.gen_elasticity3(12, 2)
         sy = _system(rptr, cind, val, 48, 2)
