@@ -834,8 +834,13 @@ int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V
     k_sptrsv_xticket<T, V, BS, UPPER, CHV, PIPEV><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(   \
         L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,          \
         UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre)
+    // rows of more than 32 external entries (an irregular matrix's long tail): rounds of 16,
+    // at 108-161 VGPRs, as the longest row of a level sets its time (unstructured n = 2^22,
+    // blocks of 16: FP32 / FP64 application 944 / 1048 us against 1150 / 1260 us with rounds
+    // of 8; the 27-point system, rows of 17-32, loses with them: FP64 315 against 246 us)
     if (ch4 && pipe) SENK_XT_LAUNCH(4, true);
     else if (ch4) SENK_XT_LAUNCH(4, false);
+    else if (L.max_ext > 32) SENK_XT_LAUNCH(16, true);
     else if (pipe) SENK_XT_LAUNCH(8, true);
     else SENK_XT_LAUNCH(8, false);
 #undef SENK_XT_LAUNCH

@@ -105,6 +105,19 @@ def test_layout_abmc_convdiff27(ctx, bsize):
     _check(ctx, sy, S.SENK_ILU0, 0, bsize, colors=True)
 
 
+def test_layout_abmc_unstructured_long_rows(ctx):
+    # lognormal row lengths: rows of up to 37 (L) and 39 (U) entries outside their block,
+    # so the sweeps take their gathers in pipelined rounds of 16 (more than 32) -- and of 4
+    # for FP64 in the other tests' short rows, of 8 with pipelining in the 27-point ones
+    rptr, cind, val = S.gen_unstructured(16384, 3)
+    sy = abmc_system(rptr, cind, val, 16)
+    rp, ci = sy["rptr"], sy["cind"]
+    ext = [int(np.sum((ci[rp[i]:rp[i + 1]] < (i // 16) * 16) | (ci[rp[i]:rp[i + 1]] >= (i // 16) * 16 + 16)))
+           for i in range(sy["N"])]
+    assert max(ext) > 32
+    _check(ctx, sy, S.SENK_ILU0, 0, 16, colors=True)
+
+
 @pytest.mark.parametrize("p,bsize", [(1, 8), (2, 4)])
 def test_layout_ilup_fill(ctx, p, bsize):
     # ILU(p) fill can couple blocks of one color: the handle falls back to the
