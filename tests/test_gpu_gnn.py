@@ -182,17 +182,32 @@ def test_bad_network_falls_back(ctx):
 
 
 def test_collect_nested_pairs(ctx, small):
+    """The pairs are the outer steps' normalised residuals r and inner results d: every
+    inner CG reduced the residual to about its tolerance, inner_eps = 1e-3 in the early
+    steps and max(1e-3, 0.5 tol / rho) once the outer residual rho nears the tolerance tol
+    (the last step's inner solve needs only what is left).  The outer residuals of the
+    same solve, from its history, give each step's tolerance; the true residual ||A d - r||
+    may exceed the split CG's (L^-1 r, relative) measure by a bounded factor, 50 here."""
     s = N.NestedSet(ctx, 10, 7, bsize=8, cap=12)
     try:
         assert 1 <= s.nvec <= 12
+        db, dx = ctx.array(s.b), ctx.zeros(s.N)
+        o = S.solve_opts(**N.NESTED_OPTS)
+        res = S.senk_result()
+        hist = np.zeros(64)
+        S.call("senk_solve", ctx.h, s.A, s.M, db.p, dx.p, C.byref(o), C.byref(res),
+               hist.ctypes.data_as(C.c_void_p), 64)
+        assert res.converged == 1 and res.restarts == s.nvec
+        rel = hist[: s.nvec]  # the outer residual (relative) at the start of each step
+        tols = np.maximum(1e-3, 0.5 * N.NESTED_OPTS["eps"] / rel)
+        assert tols[0] == 1e-3
         R = s.dR.download()[: s.nvec * s.N].reshape(s.nvec, s.N).astype(np.float64)
         Z = s.dZ.download()[: s.nvec * s.N].reshape(s.nvec, s.N).astype(np.float64)
         rows = np.repeat(np.arange(s.N), np.diff(s.rptr))
-        for r, z in zip(R, Z):
+        for r, z, tol in zip(R, Z, tols):
             assert abs(np.linalg.norm(r) - 1) <= 1e-5  # the outer residual, normalised
             Az = np.bincount(rows, weights=s.val * z[s.cind], minlength=s.N)
-            # the inner CG's result: its FP32 iteration reduced the residual well below 1
-            assert np.linalg.norm(Az - r) <= 0.05, np.linalg.norm(Az - r)
+            assert np.linalg.norm(Az - r) <= min(0.9, 50 * tol), (np.linalg.norm(Az - r), tol)
     finally:
         s.close()
 
