@@ -520,7 +520,10 @@ typedef struct {
                                GCR, FGCR, BiCGStab or CG, not GMRES)                     */
     int factor_prec;     /* senk_factor_precision of the FP32 applications of the mixed
                             forms: F32 (default when the FP32 copy exists) or F16        */
-    double inner_eps;    /* mixed: inner relative tolerance per refinement step          */
+    double inner_eps;    /* mixed 1, 2: inner relative tolerance per outer step; an outer
+                            step that starts at relative residual rho uses
+                            max(inner_eps, 0.5 * eps / rho), so the last inner solves
+                            stop at what the outer tolerance still needs                 */
     int inner_max;       /* mixed: inner iteration budget per refinement step            */
     int check_every;     /* GCR: 1 = test ||r|| every step (reference: restarts only)    */
     int use_learned;     /* 1: add the learned correction: in every ILU application of a
