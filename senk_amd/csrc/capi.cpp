@@ -480,13 +480,13 @@ int senk_spmv_sellcs_f64(senk_ctx ctx, const senk_sellcs *d, const double *dx, d
     CTX_CHECK(ctx);
     if (!d || !d->val64) return SENK_ERR_ARG;
     return launch_spmv_sell<double>(&ctx->c, d->N, d->C, d->nslices, d->slice_ptr, d->cind, d->val64, d->perm, dx,
-                                    dy, false, nullptr, sell_avg_width(*d));
+                                    dy, false, nullptr, sell_avg_width(*d), nullptr, false, d->nnz_padded);
 }
 int senk_spmv_sellcs_f32(senk_ctx ctx, const senk_sellcs *d, const float *dx, float *dy) {
     CTX_CHECK(ctx);
     if (!d || !d->val32) return SENK_ERR_ARG;
     return launch_spmv_sell<float>(&ctx->c, d->N, d->C, d->nslices, d->slice_ptr, d->cind, d->val32, d->perm, dx,
-                                   dy, false, nullptr, sell_avg_width(*d));
+                                   dy, false, nullptr, sell_avg_width(*d), nullptr, false, d->nnz_padded);
 }
 
 // ---------------------------------------------------------------- SpTRSV
@@ -1291,7 +1291,7 @@ static int time_sell(senk_mat A, const T *x, T *y, const int *order, bool nt, in
     const double w = (double)d.nnz_padded / ((double)d.nslices * d.C);
     auto run = [&]() {
         return launch_spmv_sell<T>(c, d.N, d.C, d.nslices, d.slice_ptr, d.cind, v, d.perm, x, y, false, nullptr, w,
-                                   order, nt);
+                                   order, nt, d.nnz_padded);
     };
     SENK_TRY(run());  // warm-up
     hipEvent_t e0, e1;

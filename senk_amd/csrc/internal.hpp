@@ -179,7 +179,8 @@ int launch_spmv_bcsr(Ctx *c, int N, int bnl, int bnw, const int *brptr, const in
 template <typename T>
 int launch_spmv_sell(Ctx *c, int N, int C, int nslices, const int *wid, const int *cind, const T *val,
                      const int *perm, const T *x, T *y, bool last_slice_short, const int *stop = nullptr,
-                     double avg_width = 0, const int *order = nullptr, bool nontemporal = false);
+                     double avg_width = 0, const int *order = nullptr, bool nontemporal = false,
+                     long long slots = -1);  // stored slots (wid[nslices] * C) when known
 
 template <typename T>
 int launch_sptrsv(Ctx *c, const Schedule &s, bool upper, const int *rptr, const int *cind, const T *val,

@@ -487,7 +487,8 @@ int spmv(Ctx *c, const MatView &A, const T *x, T *y, const int *stop) {
                                        A.sell->perm, x, y, false, stop,
                                        (double)A.sell->nnz_padded / ((double)A.sell->nslices * A.sell->C),
                                        std::is_same<T, double>::value ? A.sell_order64 : A.sell_order32,
-                                       std::is_same<T, double>::value ? A.sell_nt64 : A.sell_nt32);
+                                       std::is_same<T, double>::value ? A.sell_nt64 : A.sell_nt32,
+                                       A.sell->nnz_padded);
     }
     if (A.bnl) {
         const T *bv = bvals<T>(A);
