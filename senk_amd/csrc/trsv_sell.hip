@@ -314,6 +314,18 @@ __device__ __forceinline__ void store_agent(T *p, T v) {
 }
 __device__ __forceinline__ float canonical(float v) { return __builtin_canonicalizef(v); }
 __device__ __forceinline__ double canonical(double v) { return __builtin_canonicalize(v); }
+// p + i; with O32 addressed as p's (uniform) base plus a 32-bit byte offset, one address
+// register per access instead of a 64-bit pair (the launch checks that every array of the
+// layout and every vector fits: i * sizeof(P) < 2^32)
+template <bool O32, typename P>
+__device__ __forceinline__ P *ixp(P *p, size_t i) {
+    if constexpr (O32) {
+        using Ch = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+        return reinterpret_cast<P *>(reinterpret_cast<Ch *>(p) + (size_t)((unsigned)i * (unsigned)sizeof(P)));
+    } else {
+        return p + i;
+    }
+}
 
 // Sweep words (unsigned, device): the time-out flag and the ticket counter of mode 2.
 constexpr int kSweepErr = 0, kSweepTicket = 1, kSweepWords = 32;
@@ -344,22 +356,22 @@ struct SliceHead {
     DiagT<V> dv;
     int ex;
 };
-template <typename V, int BS, bool UPPER>
+template <typename V, int BS, bool UPPER, bool O32 = false>
 __device__ __forceinline__ SliceHead<V, BS> slice_head(int slice, const int4 *__restrict__ sinfo,
                                                        const typename RowWord<BS>::W *__restrict__ rw,
                                                        const int *__restrict__ next,
                                                        const DiagT<V> *__restrict__ dinv,
                                                        const signed char *__restrict__ sx) {
     SliceHead<V, BS> h;
-    h.si = sinfo[slice];
+    h.si = *ixp<O32>(sinfo, slice);
     const size_t slot = (size_t)slice * 64 + (threadIdx.x & 63);
-    RowWord<BS>::unpack(rw[slot], next, slot, h.ne, h.mk);
-    h.dv = UPPER ? dinv[slot] : DiagT<V>(0);
-    h.ex = kHalfV<V> ? (int)sx[slot] : 0;
+    RowWord<BS>::unpack(*ixp<O32>(rw, slot), next, slot, h.ne, h.mk);
+    h.dv = UPPER ? *ixp<O32>(dinv, slot) : DiagT<V>(0);
+    h.ex = kHalfV<V> ? (int)*ixp<O32>(sx, slot) : 0;
     return h;
 }
 
-template <typename T, typename V, int BS, bool UPPER, bool SWEEP, int CH = 8, bool PIPE = false>
+template <typename T, typename V, int BS, bool UPPER, bool SWEEP, int CH = 8, bool PIPE = false, bool O32 = false>
 __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, const int *__restrict__ blk_list,
                                                  const int *__restrict__ scol, const V *__restrict__ ve,
                                                  const V *__restrict__ vi, const T *x, T *y,
@@ -375,26 +387,27 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
     const int ex = h.ex;
     T t = 0, pa = 0, pb = 0;
     if (lr.active) {
-        t = x[lr.row];
+        t = *ixp<O32>(x, lr.row);
         if constexpr (UPPER && SWEEP) {
             if (pre.u) {  // the right-hand side's update (x == pre.u), all loads first
                 const T a = pre.ab[0], be = pre.ab[1];
-                const T r0 = pre.r[lr.row], qv = pre.q[lr.row], yv = pre.yd[lr.row];
+                const T r0 = *ixp<O32>(pre.r, lr.row), qv = *ixp<O32>(pre.q, lr.row),
+                        yv = *ixp<O32>(pre.yd, lr.row);
                 const T rv = (-a) * qv + r0;
-                pre.yd[lr.row] = a * t + yv;
-                pre.r[lr.row] = rv;
+                *ixp<O32>(pre.yd, lr.row) = a * t + yv;
+                *ixp<O32>(pre.r, lr.row) = rv;
                 t = rv + be * t;
-                pre.u[lr.row] = t;
+                *ixp<O32>(pre.u, lr.row) = t;
             }
         }
         if (!UPPER && in_scale) t = (T)((double)t * *in_scale);
         // requested with the right-hand side, used at the end; read before the reset, as
         // ep_b may be the U sweep's own vector, which this L sweep returns to the sentinel
         if (!UPPER && SWEEP && ep_a) {
-            pa = ep_a[lr.row];
-            pb = ep_b[lr.row];
+            pa = *ixp<O32>(ep_a, lr.row);
+            pb = *ixp<O32>(ep_b, lr.row);
         }
-        if (SWEEP && reset) reset[lr.row] = __builtin_bit_cast(T, Bits<T>::kSentinel);
+        if (SWEEP && reset) *ixp<O32>(reset, lr.row) = __builtin_bit_cast(T, Bits<T>::kSentinel);
     }
     const T rhs = UPPER ? t : T(0);  // kept for ep_b (U only)
     // CH external entries per lane per round of gathers
@@ -417,8 +430,8 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
 #pragma unroll
         for (int q = 0; q < CH; q++)
             if (o_[q]) {
-                c_[q] = scol[idx[q]];
-                v_[q] = decode<T>(ve[idx[q]], ex);
+                c_[q] = *ixp<O32>(scol, idx[q]);
+                v_[q] = decode<T>(*ixp<O32>(ve, idx[q]), ex);
             }
     };
     auto load_chunk = [&]() { load_chunk_at(k0, cc, on, vv); };
@@ -431,7 +444,7 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
         const int c = UPPER ? BS - 1 - q : q;
         on0[q] = (mk >> c) & 1;
         const unsigned long long b = __ballot(on0[q]);
-        iv0[q] = on0[q] ? decode<T>(vi[ipos + lanes_below(b)], ex) : T(0);
+        iv0[q] = on0[q] ? decode<T>(*ixp<O32>(vi, ipos + lanes_below(b)), ex) : T(0);
         ipos += __popcll(b);
     }
     while (more) {
@@ -448,7 +461,7 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
             bool pend = false;
 #pragma unroll
             for (int q = 0; q < CH; q++)
-                if (on[q]) yb[q] = load_local(y + cc[q]);
+                if (on[q]) yb[q] = load_local(ixp<O32>(y, cc[q]));
             if constexpr (PIPE) {
                 moren = __ballot(k0 + CH < ne) != 0;
                 if (moren) load_chunk_at(k0 + CH, ccn, onn, vvn);
@@ -465,7 +478,7 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
 #pragma unroll
                 for (int q = 0; q < CH; q++)
                     if (on[q] && yb[q] == Bits<T>::kSentinel) {
-                        yb[q] = load_agent(y + cc[q]);
+                        yb[q] = load_agent(ixp<O32>(y, cc[q]));
                         pend |= yb[q] == Bits<T>::kSentinel;
                     }
             }
@@ -508,7 +521,7 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
                 const int c = UPPER ? BS - 1 - (i0 + q) : i0 + q;
                 onq[q] = (mk >> c) & 1;
                 const unsigned long long b = __ballot(onq[q]);
-                iv[q] = onq[q] ? decode<T>(vi[ipos + lanes_below(b)], ex) : T(0);
+                iv[q] = onq[q] ? decode<T>(*ixp<O32>(vi, ipos + lanes_below(b)), ex) : T(0);
                 ipos += __popcll(b);
             }
         }
@@ -524,16 +537,16 @@ __device__ __forceinline__ void solve_slice_body(const SliceHead<V, BS> &h, cons
     }
     if (lr.active) {
         if (SWEEP) {
-            store_agent(y + lr.row, canonical(t));
+            store_agent(ixp<O32>(y, lr.row), canonical(t));
             // optional epilogue vectors (single-launch modes): a U sweep also writes
             // ep_b = rhs - ep_a * t (the vector pass that follows a U solve in the split
             // forms); an L sweep writes ep_a * ep_b + t instead of t to copy (the split
             // CG's product D t + L^-1 w, ep_a = D, ep_b = t)
             if constexpr (UPPER) {
-                if (copy) copy[lr.row] = t;
-                if (ep_b) ep_b[lr.row] = rhs - ep_a[lr.row] * t;
+                if (copy) *ixp<O32>(copy, lr.row) = t;
+                if (ep_b) *ixp<O32>(ep_b, lr.row) = rhs - *ixp<O32>(ep_a, lr.row) * t;
             } else {
-                if (copy) copy[lr.row] = ep_a ? pa * pb + t : t;
+                if (copy) *ixp<O32>(copy, lr.row) = ep_a ? pa * pb + t : t;
             }
         } else {
             y[lr.row] = t;
@@ -659,7 +672,7 @@ __global__ __launch_bounds__(64 * kTicketWaves) void k_sptrsv_ticket(
 // ticket j of stream x solves slice 64 j + 8 x + w: the wave goes from its workgroup
 // number to its slice descriptor and row words without a lookup.  Otherwise
 // xslice[x][j][w] names the slice (-1: none).
-template <typename T, typename V, int BS, bool UPPER, int CH, bool PIPE>
+template <typename T, typename V, int BS, bool UPPER, int CH, bool PIPE, bool O32>
 __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     int xwg, const int *__restrict__ xslice, const int4 *__restrict__ sinfo, const int *__restrict__ blk_list,
     const typename RowWord<BS>::W *__restrict__ rw, const int *__restrict__ next, const int *__restrict__ scol,
@@ -674,10 +687,10 @@ __global__ __launch_bounds__(64 * kXTicketWaves) void k_sptrsv_xticket(
     if (slice < 0) return;
     // the stop flag is read together with the slice's head (both addresses are known)
     const int stopped = stop_flag ? *stop_flag : 0;
-    const SliceHead<V, BS> h = slice_head<V, BS, UPPER>(slice, sinfo, rw, next, dinv, sx);
+    const SliceHead<V, BS> h = slice_head<V, BS, UPPER, O32>(slice, sinfo, rw, next, dinv, sx);
     if (stopped) return;
-    solve_slice_body<T, V, BS, UPPER, true, CH, PIPE>(h, blk_list, scol, ve, vi, x, y, in_scale, reset,
-                                                      sync + kSweepErr, copy, ep_a, ep_b, pre);
+    solve_slice_body<T, V, BS, UPPER, true, CH, PIPE, O32>(h, blk_list, scol, ve, vi, x, y, in_scale, reset,
+                                                           sync + kSweepErr, copy, ep_a, ep_b, pre);
 }
 
 // The update of a U solve's right-hand side (SweepPre) as a pass of its own, for the
@@ -830,10 +843,19 @@ int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V
     // a second round: 72.3 against 62.9 us in the connect order.
     const bool ch4 = std::is_same<T, double>::value && L.max_ext <= 8;
     const bool pipe = L.max_ext > (ch4 ? 4 : 8);
-#define SENK_XT_LAUNCH(CHV, PIPEV)                                                                         \
-    k_sptrsv_xticket<T, V, BS, UPPER, CHV, PIPEV><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(   \
-        L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,          \
+    // 32-bit byte offsets where every array the sweep indexes fits them (elements of at most
+    // 8 bytes; the slice descriptors, 16 bytes each, number at most a 64th of the slots)
+    const long long most = std::max({(long long)L.n, L.n_ext, L.n_int, (long long)L.nslices * 64});
+    const bool o32 = most * 8 < (1LL << 32);
+#define SENK_XT_LAUNCH1(CHV, PIPEV, O32V)                                                                      \
+    k_sptrsv_xticket<T, V, BS, UPPER, CHV, PIPEV, O32V><<<kXcds * L.xwg, 64 * kXTicketWaves, 0, c->stream>>>(  \
+        L.xwg, L.xslice, L.sinfo, s.dblk, (const W *)L.rw, L.next, L.scol, ve, vi, dinv, sx, x, y,              \
         UPPER ? nullptr : in_scale, reset, L.sync, copy, ep_a, ep_b, stop, pre)
+#define SENK_XT_LAUNCH(CHV, PIPEV)              \
+    do {                                        \
+        if (o32) SENK_XT_LAUNCH1(CHV, PIPEV, true);  \
+        else SENK_XT_LAUNCH1(CHV, PIPEV, false); \
+    } while (0)
     // rows of more than 32 external entries (an irregular matrix's long tail): rounds of 16,
     // at 108-161 VGPRs, as the longest row of a level sets its time (unstructured n = 2^22,
     // blocks of 16: FP32 / FP64 application 944 / 1048 us against 1150 / 1260 us with rounds
@@ -844,6 +866,7 @@ int launch_xticket(Ctx *c, const Schedule &s, TriLayout &L, const V *ve, const V
     else if (pipe) SENK_XT_LAUNCH(8, true);
     else SENK_XT_LAUNCH(8, false);
 #undef SENK_XT_LAUNCH
+#undef SENK_XT_LAUNCH1
     SENK_LAUNCH_CHECK();
     return SENK_OK;
 }
