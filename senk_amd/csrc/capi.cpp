@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -197,6 +198,26 @@ Schedule colors(const int *cptr, int cnum, int bsize) {
     s.nlev = cnum;
     s.lev_ptr.assign(cptr, cptr + cnum + 1);
     return s;
+}
+// A schedule of blocks of s.bsize rows (blk: its block list, empty for the identity) as
+// one of sub-blocks of g rows (g divides s.bsize): level l becomes the s.bsize / g levels
+// (l, 0), (l, 1), ..., level (l, q) holding sub-block q of every block of level l.  A row
+// depends on earlier levels and on rows of its own block; those in earlier sub-blocks are
+// now in earlier levels, those in its own sub-block stay in-block, so this is a schedule
+// of the same factor (an upper solve runs the levels backwards, sub-blocks last to first).
+Schedule split_schedule(const Schedule &s, const std::vector<int> &blk, int g, std::vector<int> &sub_blk) {
+    const int k = s.bsize / g;
+    Schedule o;
+    o.bsize = g;
+    o.nlev = s.nlev * k;
+    o.lev_ptr.assign((size_t)o.nlev + 1, 0);
+    sub_blk.clear();
+    for (int l = 0; l < s.nlev; l++)
+        for (int q = 0; q < k; q++) {
+            for (int i = s.lev_ptr[l]; i < s.lev_ptr[l + 1]; i++) sub_blk.push_back((blk.empty() ? i : blk[i]) * k + q);
+            o.lev_ptr[(size_t)l * k + q + 1] = (int)sub_blk.size();
+        }
+    return o;
 }
 template <typename T>
 int dalloc(T **p, size_t n) {
@@ -1525,18 +1546,32 @@ int senk_prec_create_ilu(senk_ctx ctx, senk_mat A, const int *rptr, const int *c
             v.l32 = l32;
             v.u32 = u32;
         }
-        // solve layouts; a pattern the layout cannot hold keeps the CSR kernels
-        if (!opts->csr_only && tri_layout_build(c, v.ls, false, N, lr, lc, &M->lt) == SENK_OK &&
-            tri_layout_build(c, v.us, true, N, ur, uc, &M->ut) == SENK_OK) {
-            if ((st = tri_layout_fill<double>(c, v.ls, &M->lt, lr, lc, l64))) break;
-            if ((st = tri_layout_fill<double>(c, v.us, &M->ut, ur, uc, u64))) break;
+        // solve layouts; a pattern the layout cannot hold keeps the CSR kernels.  A wave holds
+        // 64 / bsize whole blocks: blocks that do not divide 64 (e.g. ABMC blocks of 48) are
+        // solved in sub-blocks of gcd(bsize, 64) rows, each level split in order
+        v.lts = v.ls;
+        v.uts = v.us;
+        if (64 % bsize != 0) {
+            const int g = std::gcd(bsize, 64);
+            std::vector<int> lsub, usub;
+            v.lts = split_schedule(v.ls, use_colors ? noblk : lblk, g, lsub);
+            v.uts = split_schedule(v.us, use_colors ? noblk : ublk, g, usub);
+            if ((st = upload(ctx, &v.lts.dblk, lsub.data(), lsub.size()))) break;
+            M->owned.push_back(v.lts.dblk);
+            if ((st = upload(ctx, &v.uts.dblk, usub.data(), usub.size()))) break;
+            M->owned.push_back(v.uts.dblk);
+        }
+        if (!opts->csr_only && tri_layout_build(c, v.lts, false, N, lr, lc, &M->lt) == SENK_OK &&
+            tri_layout_build(c, v.uts, true, N, ur, uc, &M->ut) == SENK_OK) {
+            if ((st = tri_layout_fill<double>(c, v.lts, &M->lt, lr, lc, l64))) break;
+            if ((st = tri_layout_fill<double>(c, v.uts, &M->ut, ur, uc, u64))) break;
             if (v.l32) {
-                if ((st = tri_layout_fill<float>(c, v.ls, &M->lt, lr, lc, v.l32))) break;
-                if ((st = tri_layout_fill<float>(c, v.us, &M->ut, ur, uc, v.u32))) break;
+                if ((st = tri_layout_fill<float>(c, v.lts, &M->lt, lr, lc, v.l32))) break;
+                if ((st = tri_layout_fill<float>(c, v.uts, &M->ut, ur, uc, v.u32))) break;
             }
             if (opts->precision & (1 << SENK_FACTOR_F16)) {
-                if ((st = tri_layout_fill_f16(c, v.ls, &M->lt, lr, lc, l64))) break;
-                if ((st = tri_layout_fill_f16(c, v.us, &M->ut, ur, uc, u64))) break;
+                if ((st = tri_layout_fill_f16(c, v.lts, &M->lt, lr, lc, l64))) break;
+                if ((st = tri_layout_fill_f16(c, v.uts, &M->ut, ur, uc, u64))) break;
             }
             v.lt = &M->lt;
             v.ut = &M->ut;

@@ -546,9 +546,9 @@ int prec_apply(Ctx *c, const PrecView &M, const T *in, const double *in_scale, T
         if (!lv || !uv) return SENK_ERR_ARG;
         if (std::is_same<T, float>::value && M.f16) {
             if (!M.lt || !M.ut) return SENK_ERR_ARG;
-            SENK_TRY(launch_ilu_layout<T>(c, M.ls, *M.lt, M.us, *M.ut, in, out, in_scale, stop, true));
+            SENK_TRY(launch_ilu_layout<T>(c, M.lts, *M.lt, M.uts, *M.ut, in, out, in_scale, stop, true));
         } else if (layout_has<T>(M.lt) && layout_has<T>(M.ut)) {
-            SENK_TRY(launch_ilu_layout<T>(c, M.ls, *M.lt, M.us, *M.ut, in, out, in_scale, stop));
+            SENK_TRY(launch_ilu_layout<T>(c, M.lts, *M.lt, M.uts, *M.ut, in, out, in_scale, stop));
         } else {
             SENK_TRY(launch_sptrsv<T>(c, M.ls, false, M.lrptr, M.lcind, lv, in, out, in_scale, stop));
             SENK_TRY(launch_sptrsv<T>(c, M.us, true, M.urptr, M.ucind, uv, out, out, nullptr, stop));
@@ -1248,12 +1248,12 @@ static int dilu_product(Ctx *c, const PrecView &M, const float *in, float *v, fl
                         const float *other, double *Pout, const int *stop) {
     const int N = M.N;
     bool fused = false;
-    SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, in, t, stop, M.dilu_e, w, &fused));
+    SENK_TRY(launch_tri_single<float>(c, M.uts, *M.ut, *M.lt, in, t, stop, M.dilu_e, w, &fused));
     if (!fused) {
         k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, in, M.dilu_e, t, w, stop);
         SENK_LAUNCH_CHECK();
     }
-    SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, w, w, stop));
+    SENK_TRY(launch_tri_single<float>(c, M.lts, *M.lt, *M.ut, w, w, stop));
     k_dilu_post<K><<<red_grid(N), kBlock, 0, c->stream>>>(N, M.dilu_d, t, w, v, other, Pout, stop);
     SENK_LAUNCH_CHECK();
     return SENK_OK;
@@ -1287,7 +1287,7 @@ static int bicgstab_split(Ctx *c, const MatView &A, const PrecView &M, const flo
     k_state_reset<<<1, 64, 0, c->stream>>>(state);
     SENK_LAUNCH_CHECK();
     // r = L^-1 b; rstr = r; p = r; y = 0; rr = (r, rstr)
-    SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, b, r, state));
+    SENK_TRY(launch_tri_single<float>(c, M.lts, *M.lt, *M.ut, b, r, state));
     SENK_TRY(launch_copy<float>(c, N, r, rs));
     SENK_TRY(launch_copy<float>(c, N, r, pp));
     SENK_HIP(hipMemsetAsync(y, 0, sizeof(float) * N, c->stream));
@@ -1315,7 +1315,7 @@ static int bicgstab_split(Ctx *c, const MatView &A, const PrecView &M, const flo
         SENK_TRY(poll.poll(kLag, &stop));
     }
     // x = U^-1 y (after the stop flag is set: no flag for this sweep)
-    SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, y, x, nullptr));
+    SENK_TRY(launch_tri_single<float>(c, M.uts, *M.ut, *M.lt, y, x, nullptr));
     int hstate[4];
     SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
     float ms = 0;
@@ -1722,7 +1722,7 @@ static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b,
     k_state_reset<<<1, 64, 0, c->stream>>>(state);
     SENK_LAUNCH_CHECK();
     // r = L^-1 b; u = r; yd = 0; rho = (r, r/d)
-    SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, b, r, state));
+    SENK_TRY(launch_tri_single<float>(c, M.lts, *M.lt, *M.ut, b, r, state));
     SENK_TRY(launch_copy<float>(c, N, r, u));
     SENK_HIP(hipMemsetAsync(yd, 0, sizeof(float) * N, c->stream));
     k_cgs_start<<<B, kBlock, 0, c->stream>>>(N, r, M.dilu_d, red_slot(c, kSlotB), state);
@@ -1743,13 +1743,13 @@ static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b,
         bool fused = false;
         SweepPre<float> pre;
         if (i > 0) pre = SweepPre<float>{u, r, q, yd, ab};
-        SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, u, umid ? nullptr : t, state, M.dilu_e, w, &fused,
+        SENK_TRY(launch_tri_single<float>(c, M.uts, *M.ut, *M.lt, u, umid ? nullptr : t, state, M.dilu_e, w, &fused,
                                           pre));
         if (!fused) {
             k_dilu_pre<<<ew_grid(N), kBlock, 0, c->stream>>>(N, u, M.dilu_e, t, w, state);
             SENK_LAUNCH_CHECK();
         }
-        SENK_TRY(launch_tri_single<float>(c, M.ls, *M.lt, *M.ut, w, q, state, M.dilu_d, umid ? umid : t, &fused));
+        SENK_TRY(launch_tri_single<float>(c, M.lts, *M.lt, *M.ut, w, q, state, M.dilu_d, umid ? umid : t, &fused));
         if (!fused) {
             k_cgs_q<<<ew_grid(N), kBlock, 0, c->stream>>>(N, M.dilu_d, t, q, state);
             SENK_LAUNCH_CHECK();
@@ -1766,7 +1766,7 @@ static int cg_split(Ctx *c, const MatView &A, const PrecView &M, const float *b,
     k_cgs_last<<<ew_grid(N), kBlock, 0, c->stream>>>(N, u, yd, ab, state);
     SENK_LAUNCH_CHECK();
     // x = U^-1 yd (after the stop flag is set: no flag for this sweep)
-    SENK_TRY(launch_tri_single<float>(c, M.us, *M.ut, *M.lt, yd, x, nullptr));
+    SENK_TRY(launch_tri_single<float>(c, M.uts, *M.ut, *M.lt, yd, x, nullptr));
     int hstate[4];
     SENK_HIP(hipMemcpyAsync(hstate, state, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
     float ms = 0;

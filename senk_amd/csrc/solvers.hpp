@@ -38,6 +38,8 @@ struct PrecView {
     int N = 0;
     int kind = 0;  // 0 none, 1 ILU (CSR factors), 2 ILUB (BCSR factors)
     Schedule ls, us;
+    Schedule lts, uts;  // the schedules of the solve layouts lt, ut (sub-blocks where ls / us
+                        // have blocks a wave cannot hold whole)
     // CSR factors: L strict lower, U with the inverted diagonal first
     const int *lrptr = nullptr, *lcind = nullptr, *urptr = nullptr, *ucind = nullptr;
     const double *l64 = nullptr, *u64 = nullptr;
