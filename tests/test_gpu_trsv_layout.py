@@ -98,6 +98,39 @@ def test_layout_abmc_poisson(ctx, bsize):
     _check(ctx, sy, S.SENK_ILU0, 0, bsize, colors=True)
 
 
+@pytest.mark.parametrize("bsize", [6, 12, 24, 48])
+def test_layout_blocks_not_dividing_64(ctx, bsize):
+    """ABMC blocks that a wave of 64 lanes cannot hold whole are laid out as sub-blocks of
+    gcd(bsize, 64) rows, each color split into bsize / gcd levels in sub-block order; every
+    launch mode gives the oracle's sequential solve bit for bit (12^3 = 1728 rows, a multiple
+    of each block size).  The CSR path for such blocks (one wave per block, the row's
+    products summed across the wave in double) is not order-preserving: it agrees to
+    rounding only."""
+    rptr, cind, val = S.gen_poisson7(12)
+    sy = abmc_system(rptr, cind, val, bsize)
+    N = sy["N"]
+    A, (Ml, Mc) = _handles(ctx, sy, S.SENK_ILU0, 0, bsize, True)
+    try:
+        x = np.random.default_rng(3).uniform(-1, 1, N)
+        want = _oracle_apply(_expected(sy, S.SENK_ILU0, 0), x)
+        dx = ctx.array(x)
+        dy = ctx.zeros(N)
+        S.call("senk_prec_apply_f64", Mc, dx.p, dy.p)
+        assert np.max(np.abs(dy.download() - want)) <= 1e-13 * np.max(np.abs(want))
+        for mode in MODES:
+            S.call("senk_prec_set_trsv_mode", Ml, mode)
+            for rep in range(2):
+                dy = ctx.zeros(N)
+                S.call("senk_prec_apply_f64", Ml, dx.p, dy.p)
+                got = dy.download()
+                assert np.array_equal(got, want), (mode, rep, np.max(np.abs(got - want)))
+            S.call("senk_prec_check", Ml)
+    finally:
+        for M in (Ml, Mc):
+            S.lib().senk_prec_destroy(M)
+        S.lib().senk_mat_destroy(A)
+
+
 @pytest.mark.parametrize("bsize", [4, 16])
 def test_layout_abmc_convdiff27(ctx, bsize):
     rptr, cind, val = S.gen_convdiff27(12, 10.0)
