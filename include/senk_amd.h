@@ -450,9 +450,11 @@ typedef struct {
     int p;          /* ILU(p) level                                                    */
     int bnl, bnw;   /* ILUB block shape                                                */
     int precision;  /* bitmask of senk_factor_precision copies kept: 1<<F64 | 1<<F32 ... */
-    int csr_only;   /* 0: also keep the factors in the sliced solve layout (block sizes
-                       dividing 64) and solve with it; 1: CSR factors and kernels only.
-                       Both give the same bits.                                         */
+    int csr_only;   /* 0: also keep the factors in the sliced solve layout and solve with
+                       it (blocks that do not divide 64 as sub-blocks of gcd(bsize, 64)
+                       rows); 1: CSR factors and kernels only.  Both give the same
+                       bits where bsize divides 64; for other blocks the CSR kernel sums
+                       each row across a wave and agrees to rounding.                   */
 } senk_ilu_opts;
 /* Factorize A (already ABMC-reordered with colors cptr/cnum/bsize) on the device:
  * host symbolic pattern (ILU(p)/ILUB), device numeric Ilu0 per color, device split
